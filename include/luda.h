@@ -1,0 +1,181 @@
+/*
+ * luda.h — C-ABI of the MI355X LSM compaction engine.
+ *
+ * The engine is a drop-in for the compaction step of the LevelDB 1.22 fork in
+ * xpsair/luda.  A caller that holds a leveldb::Compaction hands its input
+ * tables to luda_compact() and gets back finished table images plus the
+ * metadata that DBImpl::InstallCompactionResults puts into the VersionEdit.
+ * The output bytes are identical to what the reference's CPU
+ * DBImpl::DoCompactionWork (db/db_impl.cc:1060-1224) writes for the same
+ * inputs.  Nothing here uses C++ types; every pointer is a plain buffer.
+ *
+ * Reference interfaces replaced (all paths relative to the reference tree):
+ *   luda_compact        DBImpl::DoCompactionWork, db/db_impl.cc:1060-1224
+ *                       (iterator through the last FinishCompactionOutputFile,
+ *                       :1079-1197), with Compaction::ShouldStopBefore and
+ *                       Compaction::IsBaseLevelForKey, db/version_set.cc:1535-1579
+ *   luda_build_tables   BuildTable, db/builder.cc:17-77 (one table per call when
+ *                       max_file_size is UINT64_MAX), and TableBuilder,
+ *                       table/table_builder.cc:94-254
+ *   luda_init/shutdown  gpu::HostAndDeviceMemory, cuda/decode_kv.cu:43-168
+ *                       (one engine context per DBImpl, db/db_impl.h:215)
+ */
+#ifndef LUDA_H_
+#define LUDA_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes, one per leveldb::Status kind (include/leveldb/status.h:98-105).
+ * Any code other than LUDA_OK means: run the unchanged CPU compaction. */
+#define LUDA_OK 0
+#define LUDA_NOT_FOUND 1
+#define LUDA_CORRUPTION 2
+#define LUDA_NOT_SUPPORTED 3
+#define LUDA_INVALID_ARGUMENT 4
+#define LUDA_IO_ERROR 5
+
+/* luda_job.flags */
+#define LUDA_JOB_INPUTS_ON_DEVICE 0x1u /* luda_file.data are device pointers */
+#define LUDA_JOB_KEEP_ON_DEVICE 0x2u   /* leave output bytes in HBM only (result.data == NULL) */
+
+/* The subset of leveldb::Options (include/leveldb/options.h:33-143) that
+ * changes table bytes. */
+typedef struct luda_options {
+  uint64_t block_size;             /* Options::block_size, options.h:100 (4096) */
+  uint64_t max_file_size;          /* Options::max_file_size, options.h:115 (2 MiB) */
+  uint32_t block_restart_interval; /* Options::block_restart_interval, options.h:105 (16) */
+  uint32_t bloom_bits_per_key;     /* NewBloomFilterPolicy(bits), util/bloom.cc:17; 0 = no filter_policy */
+  uint32_t compression;            /* Options::compression; only kNoCompression (0) is supported */
+  uint32_t reserved;
+} luda_options;
+
+/* One input table: FileMetaData{number, file_size} plus the file's bytes
+ * (db/version_edit.h:18-27). */
+typedef struct luda_file {
+  const uint8_t* data;
+  uint64_t size;
+  uint64_t number;
+} luda_file;
+
+typedef struct luda_slice {
+  const uint8_t* data;
+  uint64_t size;
+} luda_slice;
+
+/* Compaction::grandparents_ entry as read by ShouldStopBefore
+ * (db/version_set.cc:1556-1579): the file's largest internal key and size. */
+typedef struct luda_grandparent {
+  luda_slice largest;
+  uint64_t file_size;
+} luda_grandparent;
+
+/* A file of a level >= level+2, as read by IsBaseLevelForKey
+ * (db/version_set.cc:1535-1554).  Files of one level are contiguous in the
+ * array and sorted by key, as in Version::files_. */
+typedef struct luda_level_file {
+  luda_slice smallest_user_key;
+  luda_slice largest_user_key;
+  int32_t level;
+  uint32_t reserved;
+} luda_level_file;
+
+/* One compaction (leveldb::Compaction, db/version_set.h:321-391). */
+typedef struct luda_job {
+  int32_t level;              /* Compaction::level() */
+  uint32_t flags;             /* LUDA_JOB_* */
+  uint64_t smallest_snapshot; /* CompactionState::smallest_snapshot, db/db_impl.cc:1073-1077 */
+  luda_options options;
+  const luda_file* inputs[2]; /* Compaction::inputs_[0], inputs_[1], in Compaction order */
+  uint32_t num_inputs[2];
+  const luda_grandparent* grandparents;
+  uint32_t num_grandparents;
+  uint32_t num_deeper;
+  const luda_level_file* deeper;
+} luda_job;
+
+/* One output table: CompactionState::Output (db/db_impl.cc:195-227) without
+ * the file number, which the caller assigns in output order. */
+typedef struct luda_output {
+  uint64_t offset;      /* byte offset of this table inside result.data / result.device_data */
+  uint64_t file_size;   /* Output::file_size (TableBuilder::FileSize after Finish) */
+  uint64_t num_entries; /* TableBuilder::NumEntries */
+  uint64_t smallest_offset; /* Output::smallest, an internal key in result.keys */
+  uint64_t largest_offset;  /* Output::largest */
+  uint32_t smallest_size;
+  uint32_t largest_size;
+} luda_output;
+
+/* Timings and byte counts of one call (CompactionStats, db/db_impl.h:92-104). */
+typedef struct luda_stats {
+  double device_ms;         /* inputs resident in HBM -> outputs resident in HBM */
+  double total_ms;          /* whole call, host to host */
+  double h2d_ms;            /* input upload, 0 when inputs are already on the device */
+  double d2h_ms;            /* output download, 0 with LUDA_JOB_KEEP_ON_DEVICE */
+  double top_kernel_ms;     /* summed duration of the most expensive kernel */
+  uint64_t top_kernel_launches;
+  char top_kernel[48];      /* its name */
+  uint64_t bytes_read;      /* sum of input file sizes */
+  uint64_t bytes_written;   /* sum of output file sizes */
+  uint64_t entries_in;
+  uint64_t entries_kept;
+  uint64_t blocks_in;
+  uint64_t blocks_out;
+  uint32_t attempts;        /* 1 unless a capacity guess had to be raised */
+  uint32_t key_width;       /* user-key bytes held per entry in the key planes */
+} luda_stats;
+
+typedef struct luda_result {
+  uint32_t num_outputs;
+  uint32_t reserved;
+  luda_output* outputs;
+  uint8_t* keys;          /* smallest/largest key bytes */
+  uint8_t* data;          /* host copy of all tables, concatenated; NULL with KEEP_ON_DEVICE */
+  void* device_data;      /* the same bytes in HBM, owned by the result */
+  uint64_t total_bytes;
+  luda_stats stats;
+} luda_result;
+
+typedef struct luda_ctx luda_ctx;
+
+/* Creates an engine on HIP device `device`.  hbm_budget_bytes bounds the
+ * device memory one call may hold (0 = 64 GiB).  Replaces the per-DB
+ * gpu::HostAndDeviceMemory (cuda/decode_kv.cu:43-168, db/db_impl.h:215). */
+int luda_init(int device, uint64_t hbm_budget_bytes, luda_ctx** out, char* err, size_t errlen);
+void luda_shutdown(luda_ctx* ctx);
+
+/* Runs one compaction.  Replaces db/db_impl.cc:1079-1197 (the merge, drop,
+ * cut and build loop of DoCompactionWork).  On LUDA_OK *out holds the output
+ * tables in the order DoCompactionWork would have opened them. */
+int luda_compact(luda_ctx* ctx, const luda_job* job, luda_result** out, char* err, size_t errlen);
+
+/* Builds tables from an already sorted stream of internal keys and values,
+ * with no dropping: the memtable flush of db/builder.cc:17-77 when
+ * opts->max_file_size is UINT64_MAX (one table), otherwise cut into tables at
+ * the point DoCompactionWork would cut (db/db_impl.cc:1173-1180).
+ * Entry i is keys[key_off[i] .. key_off[i+1]) and vals[val_off[i] .. val_off[i+1]).
+ * `flags` takes LUDA_JOB_KEEP_ON_DEVICE. */
+int luda_build_tables(luda_ctx* ctx, const luda_options* opts, uint64_t n, const uint8_t* keys,
+                      const uint64_t* key_off, const uint8_t* vals, const uint64_t* val_off,
+                      uint32_t flags, luda_result** out, char* err, size_t errlen);
+
+void luda_free_result(luda_ctx* ctx, luda_result* res);
+
+/* Device-buffer helpers so that a caller with no HIP code of its own can keep
+ * inputs resident in HBM (LUDA_JOB_INPUTS_ON_DEVICE) and read outputs back. */
+int luda_device_alloc(luda_ctx* ctx, uint64_t bytes, void** dptr);
+int luda_device_free(luda_ctx* ctx, void* dptr);
+int luda_memcpy_h2d(luda_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int luda_memcpy_d2h(luda_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+
+const char* luda_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LUDA_H_ */

@@ -1,0 +1,148 @@
+"""Python-side plumbing around the C-ABI: seeded workloads, input tables and
+luda_job assembly, and reading luda_result back.
+
+This mirrors how DBImpl::DoCompactionWork's caller would fill a luda_job from
+a leveldb::Compaction (inputs_[0..1] in Compaction order, grandparents_,
+Version::files_ of the deeper levels, smallest_snapshot); see INTEGRATION.md
+for the C++ side.  No table bytes are produced here: a table builder (the
+engine's luda_build_tables, or a CPU builder passed in by tests) does that.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+from . import ffi
+
+
+@dataclass
+class Table:
+    data: bytes
+    smallest: bytes
+    largest: bytes
+    num_entries: int
+
+
+def result_tables(res):
+    """Host copies of every table in a luda_result (which must hold host data)."""
+    r = res.contents
+    out = []
+    keys = C.string_at(r.keys, max(1, sum(r.outputs[i].smallest_size + r.outputs[i].largest_size
+                                          for i in range(r.num_outputs)))) if r.num_outputs else b""
+    base = C.addressof(r.data.contents) if r.num_outputs and r.data else None
+    for i in range(r.num_outputs):
+        o = r.outputs[i]
+        data = C.string_at(base + o.offset, o.file_size) if base is not None else b""
+        out.append(Table(data=data,
+                         smallest=keys[o.smallest_offset:o.smallest_offset + o.smallest_size],
+                         largest=keys[o.largest_offset:o.largest_offset + o.largest_size],
+                         num_entries=o.num_entries))
+    return out
+
+
+class Workload:
+    """A seeded job from libluda_workload.so (see luda_amd/workload/workload.h)."""
+
+    def __init__(self, config, seed=None, scale=1.0):
+        lib = ffi.workload_lib()
+        if seed is None:
+            seed = 0x4C554441 + config
+        self._lib = lib
+        self.ptr = lib.lw_make(config, seed, scale)
+        if not self.ptr:
+            raise ValueError(f"unknown workload config {config}")
+        self.w = self.ptr.contents
+        self.config = config
+
+    def close(self):
+        if self.ptr:
+            self._lib.lw_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def groups(self):
+        return [self.w.groups[i] for i in range(self.w.num_groups)]
+
+    def group_entries(self, gi):
+        """(internal key, value) pairs of one group, as bytes."""
+        g = self.w.groups[gi]
+        ko = [g.key_off[i] for i in range(g.n + 1)]
+        vo = [g.val_off[i] for i in range(g.n + 1)]
+        kb = C.string_at(g.keys, ko[-1]) if ko[-1] else b""
+        vb = C.string_at(g.vals, vo[-1]) if vo[-1] else b""
+        return [(kb[ko[i]:ko[i + 1]], vb[vo[i]:vo[i + 1]]) for i in range(g.n)]
+
+
+def group_opts(opts, g):
+    o = ffi.luda_options()
+    C.pointer(o)[0] = opts
+    o.max_file_size = g.max_file_size
+    return o
+
+
+def build_group_tables(builder, opts, g):
+    """builder(opts_ptr, n, keys, key_off, vals, val_off) -> list[Table]."""
+    return builder(C.byref(group_opts(opts, g)), g.n, C.cast(g.keys, C.c_void_p), C.cast(g.key_off, C.c_void_p),
+                   C.cast(g.vals, C.c_void_p), C.cast(g.val_off, C.c_void_p))
+
+
+@dataclass
+class Job:
+    """A luda_job plus the Python objects whose memory it points into."""
+    job: ffi.luda_job
+    keep: list = field(default_factory=list)
+    inputs: tuple = ((), ())
+
+    @property
+    def bytes_in(self):
+        return sum(sum(s) for s in self.inputs)
+
+
+def make_job(wl, opts, inputs, flags=0, device_inputs=None):
+    """inputs: (list of table bytes for inputs_[0], list for inputs_[1]).
+    device_inputs: optional ((ptr, size), ...) pairs per which, replacing the
+    bytes with device pointers (sets LUDA_JOB_INPUTS_ON_DEVICE)."""
+    w = wl.w
+    keep = []
+    job = ffi.luda_job()
+    job.level = w.level
+    job.flags = flags
+    job.smallest_snapshot = w.smallest_snapshot
+    job.options = opts
+    sizes = ([], [])
+    for which in (0, 1):
+        if device_inputs is not None:
+            files = device_inputs[which]
+            arr = (ffi.luda_file * max(1, len(files)))()
+            for i, (ptr, size) in enumerate(files):
+                arr[i].data = ptr
+                arr[i].size = size
+                arr[i].number = 1000 * which + i + 1
+                sizes[which].append(size)
+            n = len(files)
+        else:
+            files = inputs[which]
+            arr = (ffi.luda_file * max(1, len(files)))()
+            for i, t in enumerate(files):
+                buf = C.create_string_buffer(bytes(t), len(t))
+                keep.append(buf)
+                arr[i].data = C.cast(buf, C.c_void_p)
+                arr[i].size = len(t)
+                arr[i].number = 1000 * which + i + 1
+                sizes[which].append(len(t))
+            n = len(files)
+        keep.append(arr)
+        job.inputs[which] = C.cast(arr, C.POINTER(ffi.luda_file))
+        job.num_inputs[which] = n
+    if device_inputs is not None:
+        job.flags |= ffi.LUDA_JOB_INPUTS_ON_DEVICE
+    job.grandparents = w.grandparents
+    job.num_grandparents = w.num_grandparents
+    job.deeper = w.deeper
+    job.num_deeper = w.num_deeper
+    keep.append(wl)
+    return Job(job=job, keep=keep, inputs=(tuple(sizes[0]), tuple(sizes[1])))
