@@ -1,0 +1,228 @@
+#!/usr/bin/env python3
+"""Compaction throughput of the MI355X engine on one GPU.
+
+Workload (default): the L1->L2 compaction of one 25 MiB L1 table and ~115
+overlapping 2 MiB L2 tables, 256 MiB in all (~2.25 M entries of 16-byte
+decimal keys uniform over 1e8 and 100-byte values; 10% of L1 keys overwrite
+L2 keys, 3% of L1 entries are tombstones that stay because an L3 file covers
+the key space).  Inputs are built by the engine's own table writer from a
+seeded generator, so they are valid tables of the LevelDB fork, and stay
+resident in HBM: the timed region is luda_compact() from inputs in HBM to
+outputs in HBM, one compaction per step.
+
+Printed: one JSON line on stdout (progress goes to stderr).  `value` is input
+GB/s of whole compactions; `roofline` is the dominant kernel's algorithmic
+bytes per launch over its average launch time, timed with HIP events on the
+engine's stream during the timed steps; `cpu_baseline` is the CPU compaction
+(the reference's own code when oracle/_ref was built, else the C
+restatement) timed on this host on the same job; `parity` says whether the
+engine's output tables hash the same as the CPU compaction's.
+"""
+import argparse
+import ctypes as C
+import hashlib
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from luda_amd import ffi, jobs  # noqa: E402
+
+METRIC = "compaction input GB/s (byte-identical SSTables) vs CPU; p50/p99 job latency"
+HBM_PEAK_GBPS = 8000.0
+
+WORKLOADS = {
+    0: "L0->L1 compaction: 4 overlapping L0 tables + 6 L1 tables, ~20 MiB, 16-byte keys mod 1e6, "
+       "100-byte values, 20% overwrites, 2% deletes",
+    1: "L1->L2 compaction: 1 L1 table + ~115 L2 tables, 256 MiB, 16-byte keys uniform over 1e8, "
+       "100-byte values, 10% overwrites, 3% tombstones kept (not bottommost)",
+    2: "bottommost L1->L2 compaction, 512 MiB, Zipf(0.99) keys of 8-64 bytes over 1e7 ids, 16-1024-byte "
+       "values, 5% deletes, snapshots at 25/50/75% of sequence numbers",
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def sha(b):
+    return hashlib.sha256(b).hexdigest()
+
+
+def percentile(xs, p):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = (len(xs) - 1) * p / 100.0
+    f = int(k)
+    c = min(f + 1, len(xs) - 1)
+    return xs[f] + (xs[c] - xs[f]) * (k - f)
+
+
+def build_device_inputs(eng, wl, opts):
+    """Input tables built on the GPU and left in HBM: per which, (ptr, size)."""
+    dev = ([], [])
+    keep = []
+    for g in wl.groups:
+        o = jobs.group_opts(opts, g)
+        rc, out, msg = eng.build_tables(C.byref(o), g.n, C.cast(g.keys, C.c_void_p), C.cast(g.key_off, C.c_void_p),
+                                        C.cast(g.vals, C.c_void_p), C.cast(g.val_off, C.c_void_p),
+                                        keep_on_device=True)
+        if rc != ffi.LUDA_OK:
+            raise RuntimeError(f"luda_build_tables failed ({rc}): {msg}")
+        res = out[2]
+        keep.append(res)
+        r = res.contents
+        for i in range(r.num_outputs):
+            dev[g.which].append((r.device_data + r.outputs[i].offset, r.outputs[i].file_size))
+    return dev, keep
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=1, choices=sorted(WORKLOADS))
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: a compaction job runs whole on one device and this build has no "
+              "multi-GPU path; run it without --gpus or with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    t = time.time()
+    wl = jobs.Workload(args.config, scale=args.scale)
+    opts = ffi.default_options()
+    log(f"workload {args.config} scale {args.scale}: {wl.w.total_entries} entries generated in {time.time() - t:.1f}s")
+    eng = jobs.Engine(0)
+    t = time.time()
+    dev, keep = build_device_inputs(eng, wl, opts)
+    bytes_in = sum(s for which in dev for _, s in which)
+    log(f"inputs built on the GPU: {len(dev[0])}+{len(dev[1])} tables, {bytes_in / 2**20:.1f} MiB "
+        f"in {time.time() - t:.1f}s")
+    job = jobs.make_job(wl, opts, None, flags=ffi.LUDA_JOB_KEEP_ON_DEVICE, device_inputs=dev)
+
+    def run_once():
+        rc, out, msg = eng.compact(job.job, keep_on_device=True)
+        if rc != ffi.LUDA_OK:
+            raise RuntimeError(f"luda_compact failed ({rc}): {msg}")
+        return out
+
+    os.environ.pop("LUDA_KERNEL_TIMING", None)
+    for i in range(args.warmup):
+        out = run_once()
+        eng.free(out[2])
+    # which kernel dominates: one untimed pass with every kernel timed
+    os.environ["LUDA_KERNEL_TIMING"] = "1"
+    out = run_once()
+    top = out[1]["top_kernel"]
+    eng.free(out[2])
+    log(f"dominant kernel: {top}")
+    # timed steps, with HIP events around that kernel only
+    os.environ["LUDA_KERNEL_TIMING"] = top
+    lat, dev_ms, kms, kbytes, klaunch = [], [], [], [], []
+    st = None
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        s0 = time.perf_counter()
+        out = run_once()
+        lat.append((time.perf_counter() - s0) * 1e3)
+        st = out[1]
+        dev_ms.append(st["device_ms"])
+        kms.append(st["top_kernel_ms"])
+        kbytes.append(st["top_kernel_bytes"])
+        klaunch.append(st["top_kernel_launches"])
+        if i < args.steps - 1:
+            eng.free(out[2])
+    t1 = time.perf_counter()
+    os.environ.pop("LUDA_KERNEL_TIMING", None)
+    last = out[2]
+    elapsed = t1 - t0
+    value = bytes_in * args.steps / elapsed / 1e9
+    out_bytes = st["bytes_written"]
+    dev_mean = statistics.mean(dev_ms)
+    per_launch_ms = sum(kms) / max(1, sum(klaunch))
+    per_launch_bytes = kbytes[-1] / max(1, klaunch[-1])
+    achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms > 0 else None
+    log(f"{args.steps} steps in {elapsed:.3f}s: {value:.2f} GB/s input, device {dev_mean:.2f} ms/job, "
+        f"{top} {per_launch_ms:.3f} ms/launch")
+
+    # Output tables of the last timed step, read back for the parity check.
+    r = last.contents
+    gpu_tables = []
+    for i in range(r.num_outputs):
+        o = r.outputs[i]
+        gpu_tables.append(eng.d2h(r.device_data + o.offset, o.file_size))
+    eng.free(last)
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        import oracle  # CPU baseline leg only
+        kind = "reference" if oracle.ref_available() else "port"
+        host_inputs = ([eng.d2h(p, s) for p, s in dev[0]], [eng.d2h(p, s) for p, s in dev[1]])
+        hjob = jobs.make_job(wl, opts, host_inputs)
+        times = []
+        cpu_tables = None
+        for i in range(max(1, args.cpu_runs)):
+            c0 = time.perf_counter()
+            rc, tabs = oracle.compact(hjob.job, "ref" if kind == "reference" else "oracle")
+            times.append(time.perf_counter() - c0)
+            if rc != 0:
+                raise RuntimeError(f"CPU compaction failed ({rc})")
+            cpu_tables = tabs
+        tmed = statistics.median(times)
+        cpu = {"value": round(bytes_in / tmed / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": kind,
+               "sample": f"the whole job ({bytes_in / 2**20:.0f} MiB of input tables, median of {len(times)} runs, "
+                         f"one thread, inputs and outputs in RAM)",
+               "ms_per_job": round(tmed * 1e3, 1)}
+        parity = [sha(t.data) for t in cpu_tables] == [sha(t) for t in gpu_tables]
+        log(f"CPU ({kind}): {cpu['value']} GB/s; outputs identical: {parity}")
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": WORKLOADS[args.config], "scale": args.scale, "input_bytes": bytes_in,
+                   "output_bytes": out_bytes, "entries": st["entries_in"], "kept": st["entries_kept"],
+                   "input_tables": [len(dev[0]), len(dev[1])], "output_tables": len(gpu_tables),
+                   "block_size": 4096, "restart_interval": 16, "bloom_bits_per_key": 10,
+                   "max_file_size": 2 << 20},
+        "device_ms_per_job": round(dev_mean, 3),
+        "device_input_gbps": round(bytes_in / (dev_mean * 1e-3) / 1e9, 3),
+        "output_gbps": round(out_bytes * args.steps / elapsed / 1e9, 3),
+        "job_hbm_roofline_frac": round((bytes_in + out_bytes) / (dev_mean * 1e-3) / 1e9 / HBM_PEAK_GBPS, 5),
+        "p50_ms": round(percentile(lat, 50), 3),
+        "p99_ms": round(percentile(lat, 99), 3),
+        "roofline": {"bound": "hbm", "kernel": top,
+                     "achieved": round(achieved, 2) if achieved else None, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 5) if achieved else None,
+                     "bytes_per_launch": per_launch_bytes, "ms_per_launch": round(per_launch_ms, 4),
+                     "traffic": None},
+        "cpu_baseline": cpu,
+        "parity": parity,
+    }
+    print(json.dumps(line), flush=True)
+    for res in keep:
+        eng.free(res)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -75,8 +75,9 @@ typedef struct luda_grandparent {
 } luda_grandparent;
 
 /* A file of a level >= level+2, as read by IsBaseLevelForKey
- * (db/version_set.cc:1535-1554).  Files of one level are contiguous in the
- * array and sorted by key, as in Version::files_. */
+ * (db/version_set.cc:1535-1554).  Files may come in any level order; within
+ * one level they keep Version::files_ order (sorted by key, disjoint), which
+ * the engine checks. */
 typedef struct luda_level_file {
   luda_slice smallest_user_key;
   luda_slice largest_user_key;
@@ -116,7 +117,7 @@ typedef struct luda_stats {
   double total_ms;          /* whole call, host to host */
   double h2d_ms;            /* input upload, 0 when inputs are already on the device */
   double d2h_ms;            /* output download, 0 with LUDA_JOB_KEEP_ON_DEVICE */
-  double top_kernel_ms;     /* summed duration of the most expensive kernel */
+  double top_kernel_ms;     /* summed duration of the most expensive kernel (LUDA_KERNEL_TIMING set) */
   uint64_t top_kernel_launches;
   char top_kernel[48];      /* its name */
   uint64_t bytes_read;      /* sum of input file sizes */
@@ -127,6 +128,8 @@ typedef struct luda_stats {
   uint64_t blocks_out;
   uint32_t attempts;        /* 1 unless a capacity guess had to be raised */
   uint32_t key_width;       /* user-key bytes held per entry in the key planes */
+  uint64_t data_block_bytes;  /* output data blocks with their trailers */
+  uint64_t top_kernel_bytes;  /* algorithmic HBM bytes of one launch of top_kernel */
 } luda_stats;
 
 typedef struct luda_result {

@@ -1,0 +1,331 @@
+// dev_common.h — device-side building blocks shared by every kernel of the
+// engine: byte coding (util/coding.cc), CRC-32C (util/crc32c.cc), the bloom
+// hash (util/hash.cc), and the key-plane representation of internal keys
+// (db/dbformat.h:165-175) with its comparator (db/dbformat.cc:51-67).
+//
+// Key planes.  Every entry's user key is stored as W 64-bit words
+// P[w * stride + e], word w holding user-key bytes [8w, 8w+8) in big-endian
+// order and zero-padded past the key's length; klen[e] holds the length and
+// tag[e] the fixed64 tag (seq << 8 | type).  Comparing words as integers,
+// then lengths, then tags descending gives InternalKeyComparator order for
+// any keys of at most 8W bytes (a zero-padded prefix ties on the words and
+// loses on the length, as memcmp-then-length does).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef uint8_t u8;
+typedef uint16_t u16;
+typedef uint32_t u32;
+typedef uint64_t u64;
+
+// Error bits in DevStatus::err.
+#define ERR_CORRUPT 1u      // malformed table bytes: LUDA_CORRUPTION
+#define ERR_UNSUPPORTED 2u  // valid but outside what the engine reproduces exactly
+#define ERR_OVERFLOW 4u     // a capacity guess was too small: the host retries
+
+// Where an error was first seen (DevStatus::site), for messages.
+enum ErrSite : u32 {
+  SITE_NONE = 0,
+  SITE_FOOTER = 1,
+  SITE_INDEX = 2,
+  SITE_BLOCK_TRAILER = 3,
+  SITE_ENTRY = 4,
+  SITE_GROUP_COUNT = 5,
+  SITE_SHORT_KEY = 6,
+  SITE_KEY_WIDTH = 7,
+  SITE_UNSORTED = 8,
+  SITE_CAPACITY = 9,
+  SITE_COMPRESSED = 10,
+  SITE_BLOCK_ENTRIES = 11,
+  SITE_FILE_BLOCKS = 12,
+};
+
+// Device-resident counters and flags of one job.  The host reads this struct
+// back twice per job: after the index/restart scan and at the end.
+struct DevStatus {
+  u32 err;
+  u32 site;
+  u32 max_ulen;       // longest user key seen while decoding
+  u32 max_rulen;      // longest user key at a restart point (first guess of the key width)
+  u32 pad0;
+  u32 n_blocks_in;    // data blocks of all inputs
+  u32 n_entries;      // entries of all inputs
+  u32 n_kept;         // entries that survive the drop rule
+  u32 n_blocks_out;
+  u32 n_files_out;
+  u32 n_windows;      // filter windows of all outputs
+  u32 blk_step_max;   // most entries in one output block
+  u32 file_step_max;  // most blocks in one output file
+  u32 n_stops;        // ShouldStopBefore cuts that fired
+  u32 n_forced;       // output blocks that start a table because of such a cut
+  u32 need_files;     // tables, when over the host's capacity
+  u32 need_windows;   // filter windows, when over the host's capacity
+  u32 pad1;
+  u64 out_bytes;      // sum of output table sizes
+  u64 data_bytes;     // output data blocks with trailers
+  u64 keys_needed;    // bytes of smallest/largest keys
+};
+
+// ---------------------------------------------------------------------------
+// byte coding, util/coding.cc
+
+__device__ __forceinline__ u32 ld32(const u8* p) {
+  return (u32)p[0] | ((u32)p[1] << 8) | ((u32)p[2] << 16) | ((u32)p[3] << 24);
+}
+__device__ __forceinline__ u64 ld64(const u8* p) { return (u64)ld32(p) | ((u64)ld32(p + 4) << 32); }
+
+// GetVarint32Ptr (util/coding.cc:121-140): nullptr on overrun or >5 bytes.
+__device__ __forceinline__ const u8* dec_varint32(const u8* p, const u8* lim, u32* v) {
+  u32 r = 0;
+  for (u32 shift = 0; shift <= 28 && p < lim; shift += 7) {
+    u32 b = *p++;
+    if (b & 128) {
+      r |= (b & 127) << shift;
+    } else {
+      r |= b << shift;
+      *v = r;
+      return p;
+    }
+  }
+  return nullptr;
+}
+// GetVarint64Ptr (util/coding.cc:150-165)
+__device__ __forceinline__ const u8* dec_varint64(const u8* p, const u8* lim, u64* v) {
+  u64 r = 0;
+  for (u32 shift = 0; shift <= 63 && p < lim; shift += 7) {
+    u64 b = *p++;
+    if (b & 128) {
+      r |= (b & 127) << shift;
+    } else {
+      r |= b << shift;
+      *v = r;
+      return p;
+    }
+  }
+  return nullptr;
+}
+__host__ __device__ __forceinline__ u32 varint_len(u64 v) {
+  u32 n = 1;
+  while (v >= 128) {
+    v >>= 7;
+    n++;
+  }
+  return n;
+}
+__device__ __forceinline__ u8* enc_varint(u8* p, u64 v) {
+  while (v >= 128) {
+    *p++ = (u8)(v | 128);
+    v >>= 7;
+  }
+  *p++ = (u8)v;
+  return p;
+}
+__device__ __forceinline__ void st32(u8* p, u32 v) {
+  p[0] = (u8)v;
+  p[1] = (u8)(v >> 8);
+  p[2] = (u8)(v >> 16);
+  p[3] = (u8)(v >> 24);
+}
+
+// ---------------------------------------------------------------------------
+// CRC-32C (Castagnoli), util/crc32c.cc:276-377.  Tables are derived at
+// compile time from the reflected polynomial 0x82F63B78.
+
+#define CRC_ZERO_LEVELS 7  // shifts by 4, 8, 16, ..., 256 zero bytes
+
+// CRC-32C on a wavefront.  The message is left-padded with zeros to whole
+// rows of 256 bytes; lane i folds the 4-byte words at row offsets 4i with
+// slicing-by-4 lookups and a "append 256 zero bytes" step between rows; a
+// 6-level tree then joins the lanes with "append 4*2^j zero bytes" steps.
+// The ~0 initial register is folded into the message by complementing its
+// first four bytes (a zero-initialised register fed X equals a ~0-initialised
+// one fed X with its first 32 bits flipped), so zero padding is neutral.
+struct CrcTables {
+  u32 slice[4][256];                   // slice[k][b]: byte b followed by k zero bytes, from a zero register
+  u32 zeros[CRC_ZERO_LEVELS][4][256];  // zeros[j]: "append 4*2^j zero bytes", one table per register byte
+  constexpr CrcTables() : slice(), zeros() {
+    for (u32 i = 0; i < 256; i++) {
+      u32 c = i;
+      for (int k = 0; k < 8; k++) c = (c & 1) ? (c >> 1) ^ 0x82F63B78u : (c >> 1);
+      slice[0][i] = c;
+    }
+    for (int k = 1; k < 4; k++)
+      for (u32 i = 0; i < 256; i++) {
+        u32 s = slice[k - 1][i];
+        slice[k][i] = slice[0][s & 0xff] ^ (s >> 8);
+      }
+    // The register update is linear over GF(2), so "feed n zero bytes" is a
+    // 32x32 bit matrix, kept as its 32 columns.  Level 0 (4 bytes) by direct
+    // feeding, level j+1 as level j applied twice.
+    u32 col[32] = {};
+    for (int bit = 0; bit < 32; bit++) {
+      u32 s = 1u << bit;
+      for (int k = 0; k < 4; k++) s = slice[0][s & 0xff] ^ (s >> 8);
+      col[bit] = s;
+    }
+    for (int j = 0; j < CRC_ZERO_LEVELS; j++) {
+      for (int b = 0; b < 4; b++)
+        for (u32 v = 0; v < 256; v++) {
+          u32 r = 0;
+          for (int bit = 0; bit < 8; bit++)
+            if (v & (1u << bit)) r ^= col[8 * b + bit];
+          zeros[j][b][v] = r;
+        }
+      u32 next[32] = {};
+      for (int bit = 0; bit < 32; bit++) {
+        u32 x = col[bit], y = 0;
+        for (int t = 0; t < 32; t++)
+          if (x & (1u << t)) y ^= col[t];
+        next[bit] = y;
+      }
+      for (int bit = 0; bit < 32; bit++) col[bit] = next[bit];
+    }
+  }
+};
+
+// One copy per translation unit (device code is not linked across units).
+static __device__ const CrcTables g_crc = CrcTables();
+
+// Apply a zero-append map (one of g_crc.zeros) to a raw register value.
+__device__ __forceinline__ u32 crc_shift(const u32 (*z)[256], u32 s) {
+  return z[0][s & 0xff] ^ z[1][(s >> 8) & 0xff] ^ z[2][(s >> 16) & 0xff] ^ z[3][s >> 24];
+}
+
+// crc32c::Value of a message of L >= 4 bytes, byte(i) returning byte i.
+// Called by all 64 lanes of a wavefront; every lane gets the result.
+// `slice` and `z256` may point at LDS copies of g_crc.slice / g_crc.zeros[6].
+template <typename ByteAt>
+__device__ __forceinline__ u32 wave_crc32c(u64 L, const u32 (*slice)[256], const u32 (*z256)[256], ByteAt at) {
+  const u32 lane = threadIdx.x & 63;
+  const u64 P = (256 - (L & 255)) & 255;
+  const u64 rows = (L + P) >> 8;
+  u32 s = 0;
+  for (u64 r = 0; r < rows; r++) {
+    u64 q = 256 * r + 4 * lane;
+    u32 b[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const long long x = (long long)(q + t) - (long long)P;
+      const bool in = x >= 0 && (u64)x < L;
+      // The index is clamped before it reaches an address: a padding byte
+      // reads byte 0 and is then zeroed.  No address below the message is
+      // ever formed, not even as the base of a folded immediate offset.
+      const u64 xi = in ? (u64)x : 0;
+      u32 v = (u32)at(xi);
+      v = in ? v : 0u;
+      if (x >= 0 && x < 4) v ^= 0xffu;
+      b[t] = v;
+    }
+    u32 w = slice[3][b[0]] ^ slice[2][b[1]] ^ slice[1][b[2]] ^ slice[0][b[3]];
+    s = crc_shift(z256, s) ^ w;
+  }
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    u32 other = __shfl_down(s, 1u << j, 64);
+    if ((lane & ((2u << j) - 1)) == 0) s = crc_shift(g_crc.zeros[j], s) ^ other;
+  }
+  return __shfl(s, 0, 64) ^ 0xffffffffu;
+}
+
+// util/crc32c.h:29-32
+__host__ __device__ __forceinline__ u32 crc_mask(u32 crc) { return ((crc >> 15) | (crc << 17)) + 0xa282ead8u; }
+
+// ---------------------------------------------------------------------------
+// util/hash.cc:22-53 over bytes produced by a getter.
+
+template <typename ByteAt>
+__device__ __forceinline__ u32 leveldb_hash(u32 n, u32 seed, ByteAt at) {
+  const u32 m = 0xc6a4a793u;
+  u32 h = seed ^ (n * m);
+  u32 i = 0;
+  for (; i + 4 <= n; i += 4) {
+    u32 w = (u32)at(i) | ((u32)at(i + 1) << 8) | ((u32)at(i + 2) << 16) | ((u32)at(i + 3) << 24);
+    h += w;
+    h *= m;
+    h ^= (h >> 16);
+  }
+  switch (n - i) {
+    case 3:
+      h += (u32)at(i + 2) << 16;
+      [[fallthrough]];
+    case 2:
+      h += (u32)at(i + 1) << 8;
+      [[fallthrough]];
+    case 1:
+      h += at(i);
+      h *= m;
+      h ^= (h >> 24);
+      break;
+  }
+  return h;
+}
+
+// ---------------------------------------------------------------------------
+// key planes
+
+struct Planes {
+  u64* P;       // W planes of `stride` words
+  u32* klen;    // user key length
+  u64* tag;     // seq << 8 | type
+  u64 stride;
+  u32 W;
+};
+
+__device__ __forceinline__ u8 ukey_byte(const Planes& k, u64 e, u32 i) {
+  if ((i >> 3) >= k.W) return 0;  // only reachable on a rejected job; keeps reads in bounds
+  u64 w = k.P[(u64)(i >> 3) * k.stride + e];
+  return (u8)(w >> (56 - 8 * (i & 7)));
+}
+// byte i of the internal key user_key || fixed64(tag)
+__device__ __forceinline__ u8 ikey_byte(const Planes& k, u64 e, u32 ulen, u64 tag, u32 i) {
+  return i < ulen ? ukey_byte(k, e, i) : (u8)(tag >> (8 * (i - ulen)));
+}
+
+// InternalKeyComparator::Compare for two entries of one plane set.
+__device__ __forceinline__ int ikey_cmp(const Planes& k, u64 a, u64 b) {
+  for (u32 w = 0; w < k.W; w++) {
+    u64 x = k.P[w * k.stride + a], y = k.P[w * k.stride + b];
+    if (x != y) return x < y ? -1 : 1;
+  }
+  u32 la = k.klen[a], lb = k.klen[b];
+  if (la != lb) return la < lb ? -1 : 1;
+  u64 ta = k.tag[a], tb = k.tag[b];
+  if (ta != tb) return ta > tb ? -1 : 1;
+  return 0;
+}
+__device__ __forceinline__ bool ukey_eq(const Planes& k, u64 a, u64 b) {
+  if (k.klen[a] != k.klen[b]) return false;
+  for (u32 w = 0; w < k.W; w++)
+    if (k.P[w * k.stride + a] != k.P[w * k.stride + b]) return false;
+  return true;
+}
+
+// A key held outside the planes (grandparent largest key, deeper-level
+// range ends): its first 8W bytes as words, and its full length.  Keys longer
+// than 8W bytes compare correctly against plane keys of at most 8W bytes.
+struct FixedKey {
+  u32 len;   // user-key length
+  u32 pad;
+  u64 tag;   // only for internal keys
+};
+
+// Bytewise compare of plane key e against fixed key f (words fw[0..W)).
+__device__ __forceinline__ int ukey_cmp_fixed(const Planes& k, u64 e, const u64* fw, u32 flen) {
+  for (u32 w = 0; w < k.W; w++) {
+    u64 x = k.P[w * k.stride + e], y = fw[w];
+    if (x != y) return x < y ? -1 : 1;
+  }
+  u32 la = k.klen[e];
+  if (la != flen) return la < flen ? -1 : 1;
+  return 0;
+}
+
+// Grid-stride helpers.
+#define GRID_STRIDE(i, n) for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < (u64)(n); i += (u64)gridDim.x * blockDim.x)
+
+__device__ __forceinline__ void set_err(DevStatus* st, u32 bits, u32 site) {
+  atomicOr(&st->err, bits);
+  atomicCAS(&st->site, 0u, site);
+}

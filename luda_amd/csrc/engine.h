@@ -1,0 +1,261 @@
+// engine.h — structures shared by the host driver (engine.hip) and the kernel
+// units (k_*.hip), and the host-side launchers each unit exports.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "dev_common.h"
+
+// ---------------------------------------------------------------------------
+// per-kernel timing with HIP events on the engine's stream
+
+struct KTimer {
+  hipStream_t s = nullptr;
+  bool on = false;
+  struct Rec {
+    const char* name;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  hipEvent_t get() {
+    if (used == pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+  const char* cur = nullptr;
+  hipEvent_t cur_a = nullptr;
+  std::vector<std::string> only;  // when non-empty, time just these kernels
+  void begin(const char* name) {
+    cur_a = nullptr;
+    if (!on) return;
+    if (!only.empty()) {
+      bool hit = false;
+      for (auto& o : only) hit |= (o == name);
+      if (!hit) return;
+    }
+    cur = name;
+    cur_a = get();
+    if (cur_a) hipEventRecord(cur_a, s);
+  }
+  void end() {
+    if (!on || !cur_a) return;
+    hipEvent_t b = get();
+    if (!b) return;
+    hipEventRecord(b, s);
+    recs.push_back({cur, cur_a, b});
+  }
+  void reset() {
+    recs.clear();
+    used = 0;
+  }
+  ~KTimer() {
+    for (auto e : pool) hipEventDestroy(e);
+  }
+};
+
+#define KLAUNCH(T, name, kern, grid, block, lds, ...)                                     \
+  do {                                                                                    \
+    (T).begin(name);                                                                      \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), (lds), (T).s, __VA_ARGS__);       \
+    (T).end();                                                                            \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// input side
+
+struct FileDesc {
+  const u8* base;  // device address of the table's first byte
+  u64 size;
+};
+struct FileInfo {
+  u64 ioff;   // index block handle
+  u64 isize;
+  u32 nr;     // index restarts
+  u32 pad;
+};
+struct BlockDesc {
+  u64 off;    // data block handle within its file
+  u32 size;
+  u32 file;
+};
+
+// Two adjacent runs of an index array merged into the same positions:
+// [a0, a1) and [a1, b1); tile0 = first thread-tile of this pair.
+struct MergePair {
+  u32 a0, a1, b1, tile0;
+};
+
+// Parameters of the drop rule (db/db_impl.cc:1113-1147), IsBaseLevelForKey
+// (db/version_set.cc:1535-1554) and ShouldStopBefore (:1556-1579).
+struct DropParams {
+  u64 S;            // smallest_snapshot
+  u64 limit;        // MaxGrandParentOverlapBytes = 10 * max_file_size
+  u32 ngp;          // grandparents
+  u32 ndeep;
+  const u64* gp_words;   // ngp x W: largest user key, first 8W bytes
+  const u32* gp_len;     // user-key length of each largest key
+  const u64* gp_tag;
+  const u64* gp_prefix;  // ngp + 1 running sums of file sizes
+  const u64* deep_words; // ndeep x 2 x W: smallest, largest user keys
+  const u32* deep_len;   // ndeep x 2
+  u32 lvl_begin[8], lvl_end[8];  // deeper files of level L at [lvl_begin[L], lvl_end[L])
+  u32 first_level;       // level + 2
+  u32 pad;
+};
+
+// The stream the table encoder consumes: kept entries in output order.
+struct Kept {
+  Planes K;
+  u64* vptr;    // device address of the value
+  u32* vlen;
+  u32* forced;  // 1 where a new table must start before this entry
+};
+
+struct EncParams {
+  u64 block_size;
+  u64 max_file_size;
+  u32 interval;     // block_restart_interval
+  u32 bloom_bits;   // 0 = no filter block
+  u32 bloom_k;
+  u32 kw;           // bytes of one stored internal key (8W + 8)
+};
+
+// Per output table, read back by the host.
+struct FileMeta {
+  u64 offset;       // in the output buffer
+  u64 size;
+  u64 nent;
+  u32 slen, llen;   // smallest / largest internal key lengths
+};
+
+// Per output block and per output table working arrays (see k_encode.hip).
+struct EncBufs {
+  // per kept entry
+  u32* shared;   // shared-prefix length with the previous kept key
+  u32* szr;      // encoded size as a restart entry
+  u32* sznr;     // encoded size as a prefix-compressed entry
+  u32* bnext;    // first entry after the block that would start here
+  u32* bbytes;   // raw size of that block
+  u32* bs;       // 1 where a block starts
+  u32* bid;      // exclusive scan of bs: block id
+  // per output block (nbo + 1 entries where a sentinel is needed)
+  u32* ob_first;
+  u32* ob_n;
+  u32* ob_bytes;
+  u32* ob_forced;  // 1 where ShouldStopBefore starts a table at this block
+  u32* ob_tot;     // ob_bytes + 5, the scan input of O
+  u64* O;          // exclusive scan of ob_tot
+  u32* ff;         // exclusive scan of ob_forced
+  u32* flist;      // forced blocks in order
+  u32* fnext;      // first block after the table that would start here
+  u32* fs;         // 1 where a table starts
+  u32* fid;        // exclusive scan of fs: table id
+  u32* ob_file;
+  u64* ob_off;     // offset inside its table
+  u64* ob_isz;     // index entry size, then (scanned) its offset in the index block
+  // per output table (nf + 1 entries)
+  u32* f_first;    // first block; f_first[nf] = nbo
+  u32* f_nwin;     // filter windows, then (scanned) f_win
+  u32* f_win;
+  u64* f_dend;     // data end offset
+  u64* f_fbytes;   // filter data bytes
+  u64* f_ibytes;   // index block size
+  u64* f_off;      // table size, then (scanned in place) table offset in the output
+  FileMeta* meta;
+  u8* keys;        // 2 x kw bytes per table: smallest, largest
+  // per filter window (nw + 1 entries)
+  u32* w_keys;
+  u32* w_first;
+  u64* w_fsz;      // filter bytes, then (scanned in place) offset in the filter data
+};
+
+// ---------------------------------------------------------------------------
+// launchers
+
+struct ScanTmp {
+  void* tiles = nullptr;  // device, >= scan_tmp_bytes(n_cap)
+};
+size_t scan_tmp_bytes(u64 n_cap);
+// Exclusive scan of n = (n_dev ? *n_dev : 0) + n_add items (n <= n_cap);
+// total to *total (may be null).  In place is allowed.
+void scan_u32(KTimer& T, const u32* in, u32* out, const u32* n_dev, u32 n_add, u32 n_cap, u32* total, void* tmp);
+void scan_u64(KTimer& T, const u64* in, u64* out, const u32* n_dev, u32 n_add, u32 n_cap, u64* total, void* tmp);
+void scan_u32_to_u64(KTimer& T, const u32* in, u64* out, const u32* n_dev, u32 n_add, u32 n_cap, u64* total,
+                     void* tmp);
+void fill_u32(KTimer& T, u32* p, u32 v, u64 n);
+void fill_u8(KTimer& T, u8* p, u8 v, u64 n);
+void iota_u32(KTimer& T, u32* p, u64 n);
+
+// k_decode.hip
+void launch_files(KTimer& T, const FileDesc* files, u32 nfiles, FileInfo* info, u32* nblk, DevStatus* st);
+void launch_index(KTimer& T, const FileDesc* files, const FileInfo* info, const u32* blk_base, u32 nfiles,
+                  BlockDesc* blks, DevStatus* st);
+void launch_blkscan(KTimer& T, const FileDesc* files, const BlockDesc* blks, const u32* nb_dev, u32 nb_cap,
+                    u32* blk_nent, DevStatus* st);
+void launch_file_ebase(KTimer& T, const u32* blk_base, const u32* blk_ebase, u32 nfiles, const u32* n_entries,
+                       u32* file_ebase);
+void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 nb, const u32* blk_ebase, Planes K,
+                   u64* vptr, u32* vlen, DevStatus* st);
+void launch_ingest(KTimer& T, u64 n, const u8* keys, const u64* key_off, const u8* vals, const u64* val_off, Kept out,
+                   DevStatus* st);
+
+// k_merge.hip
+void launch_merge(KTimer& T, const MergePair* pairs, u32 npairs, u32 total_tiles, const u32* in, u32* out, Planes K);
+u32 merge_items_per_tile();
+void launch_check_sorted(KTimer& T, Planes K, u64 n, const u32* child_start, u32 nchild, DevStatus* st);
+
+// k_select.hip
+void launch_drop(KTimer& T, const u32* perm, u32 n, Planes K, const DropParams& dp, u32* keep, u32* gcnt,
+                 DevStatus* st);
+void launch_gather(KTimer& T, const u32* perm, u32 n, const u32* keep, const u32* kpos, Planes K, const u64* vptr,
+                   const u32* vlen, Kept out);
+void launch_stops(KTimer& T, const u32* gcnt, const u32* kpos, const u32* keep, u32 n, const DropParams& dp,
+                  u32* events, u32 ev_cap, Kept out, DevStatus* st);
+
+// k_orbit.hip: marks in flag[] the nodes reached from node 0 by nxt[].
+struct OrbitPlan {
+  u32 C;       // chunk size
+  u32 Wc;      // window per chunk (max step bound)
+  u32 G;       // composition group size
+  u32 levels;  // composition levels above the chunk level
+  u64 n_cap;
+  std::vector<u64> level_items;   // capacity of items per level
+  std::vector<u64> level_offset;  // offsets (in u16 items of width Wc) of each level's tables
+  u64 table_entries;              // total u16 entries
+  u64 entry_entries;              // total u32 entries
+};
+OrbitPlan plan_orbit(u64 n_cap, u32 C, u32 Wc, u32 G);
+void run_orbit(KTimer& T, const OrbitPlan& p, const u32* nxt, const u32* n_dev, const u32* step_dev, u32* flag,
+               u16* ftab, u32* entries, DevStatus* st);
+
+// k_encode.hip
+void launch_entry_sizes(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncBufs b, DevStatus* st);
+void launch_bnext(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncParams ep, EncBufs b, DevStatus* st);
+void launch_block_info(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncBufs b, DevStatus* st);
+void launch_block_tot(KTimer& T, const u32* nbo_dev, u32 nbo_cap, EncBufs b, u32* tot);
+void launch_block_forced(KTimer& T, const u32* nbo_dev, u32 nbo_cap, EncBufs b);
+void launch_fnext(KTimer& T, const u32* nbo_dev, u32 nbo_cap, const u32* nforced_dev, EncParams ep, EncBufs b,
+                  DevStatus* st);
+void launch_file_first(KTimer& T, const u32* nbo_dev, u32 nbo_cap, EncBufs b, const u32* nf_dev);
+void launch_block_layout(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, const u32* nf_dev, EncParams ep,
+                         EncBufs b);
+void launch_file_layout(KTimer& T, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b);
+void launch_window_init(KTimer& T, const u32* nw_dev, u32 nw_cap, EncBufs b);
+void launch_window_counts(KTimer& T, const u32* nbo_dev, u32 nbo_cap, const u32* nf_dev, const u32* nw_dev,
+                          EncBufs b);
+void launch_window_sizes(KTimer& T, const u32* nw_dev, u32 nw_cap, EncParams ep, EncBufs b);
+void launch_file_sizes(KTimer& T, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b);
+void launch_write_blocks(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u8* out,
+                         u64 out_cap, DevStatus* st);
+void launch_write_filters(KTimer& T, Kept k, const u32* nw_dev, u32 nw_cap, const u32* nf_dev, EncParams ep, EncBufs b,
+                          u8* out, u64 out_cap, DevStatus* st);
+void launch_write_tail(KTimer& T, Kept k, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b, u8* out,
+                       u64 out_cap, DevStatus* st);

@@ -1,0 +1,1161 @@
+// engine.hip — host driver of the compaction engine and the C-ABI of
+// include/luda.h.
+//
+// One call runs two device phases on the context's stream:
+//   A  footers, index blocks and restart arrays of the inputs: how many data
+//      blocks and entries there are (one small read-back);
+//   B  decode, merge, drop, table cut and encode, sized from A, then one
+//      read-back of the status and the output metadata.
+// Everything in between is driven by device-resident counts.  A capacity
+// guess that turns out too small (key width, output bytes, tables, filter
+// windows) is reported by the kernels and the call re-runs phase B with the
+// measured sizes.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "luda.h"
+
+namespace {
+
+constexpr u64 kAlign = 256;
+constexpr u32 kMaxKeyWords = 32;     // user keys up to 256 bytes
+constexpr u64 kMaxBlockSize = 8192;   // block_size handled on the device
+constexpr u32 kMaxChunk = 8192;       // orbit chunk: its LDS footprint is 5 bytes per node
+constexpr u32 kMaxAttempts = 4;
+
+u64 align_up(u64 x, u64 a) { return (x + a - 1) / a * a; }
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// Bump allocator over one device allocation; a dry run (base == nullptr)
+// measures the bytes a plan needs.
+struct Arena {
+  u8* base = nullptr;
+  u64 cap = 0;
+  u64 used = 0;
+  bool dry = false;
+  template <typename T>
+  T* take(u64 count) {
+    const u64 off = align_up(used, kAlign);
+    used = off + align_up(count * sizeof(T), kAlign);
+    if (dry) return nullptr;
+    return reinterpret_cast<T*>(base + off);
+  }
+};
+
+struct DevBuf {
+  u8* p = nullptr;
+  u64 cap = 0;
+  hipError_t reserve(u64 need) {
+    if (need <= cap) return hipSuccess;
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+    u64 c = std::max<u64>(need, 1 << 20);
+    hipError_t e = hipMalloc(&p, c);
+    if (e == hipSuccess) cap = c;
+    return e;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct PinnedBuf {
+  u8* p = nullptr;
+  u64 cap = 0;
+  hipError_t reserve(u64 need) {
+    if (need <= cap) return hipSuccess;
+    if (p) hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    u64 c = std::max<u64>(need, 1 << 16);
+    hipError_t e = hipHostMalloc(&p, c, hipHostMallocDefault);
+    if (e == hipSuccess) cap = c;
+    return e;
+  }
+  void release() {
+    if (p) hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+void set_msg(char* err, size_t errlen, const char* fmt, const char* a = "", long long b = 0) {
+  if (!err || !errlen) return;
+  snprintf(err, errlen, fmt, a, b);
+}
+
+const char* site_name(u32 s) {
+  switch (s) {
+    case SITE_FOOTER: return "table footer";
+    case SITE_INDEX: return "index block";
+    case SITE_BLOCK_TRAILER: return "data block restart array";
+    case SITE_ENTRY: return "block entry";
+    case SITE_GROUP_COUNT: return "restart group entry count";
+    case SITE_SHORT_KEY: return "internal key shorter than 8 bytes";
+    case SITE_KEY_WIDTH: return "key width";
+    case SITE_UNSORTED: return "input not strictly sorted";
+    case SITE_CAPACITY: return "capacity";
+    case SITE_COMPRESSED: return "compressed block";
+    case SITE_BLOCK_ENTRIES: return "block entries";
+    case SITE_FILE_BLOCKS: return "table blocks";
+    default: return "unknown";
+  }
+}
+
+// Key -> its first 8W bytes as big-endian words.
+void to_words(const u8* k, u64 len, u32 W, u64* out) {
+  for (u32 w = 0; w < W; w++) {
+    u64 v = 0;
+    for (u32 t = 0; t < 8; t++) {
+      const u64 pos = 8ull * w + t;
+      v = (v << 8) | (pos < len ? k[pos] : 0u);
+    }
+    out[w] = v;
+  }
+}
+
+int bytewise_cmp(const u8* a, u64 an, const u8* b, u64 bn) {
+  const u64 m = std::min(an, bn);
+  int r = m ? memcmp(a, b, m) : 0;
+  if (r == 0) r = an < bn ? -1 : (an > bn ? 1 : 0);
+  return r < 0 ? -1 : (r > 0 ? 1 : 0);
+}
+int ikey_cmp_host(const u8* a, u64 an, const u8* b, u64 bn) {
+  int r = bytewise_cmp(a, an - 8, b, bn - 8);
+  if (r) return r;
+  u64 ta = 0, tb = 0;
+  memcpy(&ta, a + an - 8, 8);
+  memcpy(&tb, b + bn - 8, 8);
+  return ta > tb ? -1 : (ta < tb ? 1 : 0);
+}
+
+u32 next_pow2(u64 x) {
+  u32 p = 1;
+  while (p < x) p <<= 1;
+  return p;
+}
+
+// The stream the encoder reads, plus the counts it depends on.
+struct EncodeSetup {
+  EncParams ep;
+  u32 m_cap;      // kept entries
+  u32 nbo_cap;    // output blocks
+  u32 nf_cap;     // output tables
+  u32 nw_cap;     // filter windows
+  u64 out_cap;    // output bytes
+  bool size_cuts; // max_file_size can cut a table before the stream ends
+  OrbitPlan ob, of;
+};
+
+}  // namespace
+
+struct luda_ctx {
+  int device = 0;
+  hipStream_t s = nullptr;
+  u64 budget = 0;
+  std::mutex mu;
+  DevBuf in_buf;     // uploaded host inputs
+  DevBuf a_buf;      // phase A working memory
+  DevBuf b_buf;      // phase B working memory
+  DevStatus* st = nullptr;
+  PinnedBuf pin;     // staging for small host<->device copies
+  PinnedBuf pin2;
+  u32 kw_hint = 0;   // key words that fit the keys seen so far
+  KTimer T;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::multimap<u64, void*> out_pool;  // freed output buffers by size
+
+  void* take_out(u64 bytes) {
+    auto it = out_pool.lower_bound(bytes);
+    if (it != out_pool.end() && it->first <= 2 * bytes + (64 << 20)) {
+      void* p = it->second;
+      out_pool.erase(it);
+      return p;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    pool_sizes[p] = bytes;
+    return p;
+  }
+  void give_out(void* p) {
+    auto it = pool_sizes.find(p);
+    if (it == pool_sizes.end()) return;
+    out_pool.emplace(it->second, p);
+    // keep the pool bounded
+    while (out_pool.size() > 8) {
+      auto first = out_pool.begin();
+      pool_sizes.erase(first->second);
+      hipFree(first->second);
+      out_pool.erase(first);
+    }
+  }
+  std::map<void*, u64> pool_sizes;
+};
+
+namespace {
+
+struct Fail {
+  int code;
+  std::string msg;
+};
+
+#define HIPCHK(x)                                                                \
+  do {                                                                           \
+    hipError_t e__ = (x);                                                        \
+    if (e__ != hipSuccess) throw Fail{LUDA_IO_ERROR, std::string("HIP: ") + #x + ": " + hipGetErrorString(e__)}; \
+  } while (0)
+
+int status_code_of(const DevStatus& st) {
+  if (st.err & ERR_CORRUPT) return LUDA_CORRUPTION;
+  if (st.err & ERR_UNSUPPORTED) return LUDA_NOT_SUPPORTED;
+  return LUDA_OK;
+}
+
+void fill_status(luda_ctx* c, const DevStatus& init) {
+  memcpy(c->pin.p, &init, sizeof(DevStatus));
+  HIPCHK(hipMemcpyAsync(c->st, c->pin.p, sizeof(DevStatus), hipMemcpyHostToDevice, c->s));
+}
+
+// Copies host bytes to device memory through the pinned staging buffer.
+void upload(luda_ctx* c, void* dst, const void* src, u64 bytes) {
+  if (!bytes) return;
+  HIPCHK(c->pin2.reserve(bytes));
+  HIPCHK(hipStreamSynchronize(c->s));
+  memcpy(c->pin2.p, src, bytes);
+  HIPCHK(hipMemcpyAsync(dst, c->pin2.p, bytes, hipMemcpyHostToDevice, c->s));
+}
+
+struct EncodeArrays {
+  Kept kept;
+  EncBufs eb;
+  u16* ftab_b;
+  u32* ent_b;
+  u16* ftab_f;
+  u32* ent_f;
+  void* scan_tmp;
+};
+
+// Carves the encoder's arrays for a setup out of an arena (dry or real).
+EncodeArrays carve_encode(Arena& a, const EncodeSetup& s, u32 W, bool with_kept) {
+  EncodeArrays r;
+  memset(&r, 0, sizeof(r));
+  const u64 m = s.m_cap, nb = s.nbo_cap, nf = s.nf_cap, nw = s.nw_cap;
+  if (with_kept) {
+    r.kept.K.P = a.take<u64>((u64)W * m);
+    r.kept.K.klen = a.take<u32>(m);
+    r.kept.K.tag = a.take<u64>(m);
+    r.kept.vptr = a.take<u64>(m);
+    r.kept.vlen = a.take<u32>(m);
+    r.kept.forced = a.take<u32>(m + 1);
+    r.kept.K.stride = m;
+    r.kept.K.W = W;
+  }
+  EncBufs& b = r.eb;
+  b.shared = a.take<u32>(m);
+  b.szr = a.take<u32>(m);
+  b.sznr = a.take<u32>(m);
+  b.bnext = a.take<u32>(m);
+  b.bbytes = a.take<u32>(m);
+  b.bs = a.take<u32>(m);
+  b.bid = a.take<u32>(m);
+  b.ob_first = a.take<u32>(nb + 1);
+  b.ob_n = a.take<u32>(nb + 1);
+  b.ob_bytes = a.take<u32>(nb + 1);
+  b.ob_forced = a.take<u32>(nb + 1);
+  b.ob_tot = a.take<u32>(nb + 1);
+  b.O = a.take<u64>(nb + 1);
+  b.ff = a.take<u32>(nb + 1);
+  b.flist = a.take<u32>(nb + 1);
+  b.fnext = a.take<u32>(nb + 1);
+  b.fs = a.take<u32>(nb + 1);
+  b.fid = a.take<u32>(nb + 1);
+  b.ob_file = a.take<u32>(nb + 1);
+  b.ob_off = a.take<u64>(nb + 1);
+  b.ob_isz = a.take<u64>(nb + 1);
+  b.f_first = a.take<u32>(nf + 1);
+  b.f_nwin = a.take<u32>(nf + 1);
+  b.f_win = a.take<u32>(nf + 1);
+  b.f_dend = a.take<u64>(nf + 1);
+  b.f_fbytes = a.take<u64>(nf + 1);
+  b.f_ibytes = a.take<u64>(nf + 1);
+  b.f_off = a.take<u64>(nf + 1);
+  b.meta = a.take<FileMeta>(nf + 1);
+  b.keys = a.take<u8>((nf + 1) * 2ull * s.ep.kw);
+  b.w_keys = a.take<u32>(nw + 1);
+  b.w_first = a.take<u32>(nw + 1);
+  b.w_fsz = a.take<u64>(nw + 1);
+  r.ftab_b = a.take<u16>(s.ob.table_entries);
+  r.ent_b = a.take<u32>(s.ob.entry_entries);
+  if (s.size_cuts) {
+    r.ftab_f = a.take<u16>(s.of.table_entries);
+    r.ent_f = a.take<u32>(s.of.entry_entries);
+  }
+  const u64 maxn = std::max<u64>(std::max<u64>(m, nb), std::max<u64>(nf, nw)) + 2;
+  r.scan_tmp = a.take<u8>(scan_tmp_bytes(maxn));
+  return r;
+}
+
+__global__ void k_guard(u32* count, u32 cap, DevStatus* st, u32* need) {
+  if (*count > cap) {
+    *need = *count;
+    *count = 0;
+    atomicOr(&st->err, ERR_OVERFLOW);
+    atomicCAS(&st->site, 0u, (u32)SITE_CAPACITY);
+  }
+}
+
+__global__ void k_table_starts_forced(const u32* nbo_dev, EncBufs b) {
+  const u64 nbo = *nbo_dev;
+  GRID_STRIDE(i, nbo) b.fs[i] = (i == 0 || b.ob_forced[i]) ? 1u : 0u;
+}
+
+u32 grid_for(u64 n, u32 per) {
+  u64 g = (n + per - 1) / per;
+  return (u32)std::min<u64>(std::max<u64>(g, 1), 16384);
+}
+
+// The encoder over the kept stream, counts in st.
+void run_encode(luda_ctx* c, const EncodeSetup& s, const EncodeArrays& A, u8* out) {
+  KTimer& T = c->T;
+  DevStatus* st = c->st;
+  const Kept& k = A.kept;
+  const EncBufs& b = A.eb;
+  const u32* m_dev = &st->n_kept;
+  const u32* nbo_dev = &st->n_blocks_out;
+  const u32* nf_dev = &st->n_files_out;
+  const u32* nw_dev = &st->n_windows;
+  launch_entry_sizes(T, k, m_dev, s.m_cap, b, st);
+  launch_bnext(T, k, m_dev, s.m_cap, s.ep, b, st);
+  run_orbit(T, s.ob, b.bnext, m_dev, &st->blk_step_max, b.bs, A.ftab_b, A.ent_b, st);
+  scan_u32(T, b.bs, b.bid, m_dev, 0, s.m_cap, &st->n_blocks_out, A.scan_tmp);
+  launch_block_info(T, k, m_dev, s.m_cap, b, st);
+  launch_block_tot(T, nbo_dev, s.nbo_cap, b, b.ob_tot);
+  scan_u32_to_u64(T, b.ob_tot, b.O, nbo_dev, 1, s.nbo_cap + 1, &st->data_bytes, A.scan_tmp);
+  if (s.size_cuts) {
+    scan_u32(T, b.ob_forced, b.ff, nbo_dev, 0, s.nbo_cap, &st->n_forced, A.scan_tmp);
+    launch_block_forced(T, nbo_dev, s.nbo_cap, b);
+    launch_fnext(T, nbo_dev, s.nbo_cap, &st->n_forced, s.ep, b, st);
+    run_orbit(T, s.of, b.fnext, nbo_dev, &st->file_step_max, b.fs, A.ftab_f, A.ent_f, st);
+  } else {
+    KLAUNCH(T, "table_starts", k_table_starts_forced, grid_for(s.nbo_cap, 256), 256, 0, nbo_dev, b);
+  }
+  scan_u32(T, b.fs, b.fid, nbo_dev, 0, s.nbo_cap, &st->n_files_out, A.scan_tmp);
+  KLAUNCH(T, "guard", k_guard, 1, 1, 0, &st->n_files_out, s.nf_cap, st, &st->need_files);
+  launch_file_first(T, nbo_dev, s.nbo_cap, b, nf_dev);
+  launch_block_layout(T, k, nbo_dev, s.nbo_cap, nf_dev, s.ep, b);
+  scan_u64(T, b.ob_isz, b.ob_isz, nbo_dev, 1, s.nbo_cap + 1, nullptr, A.scan_tmp);
+  launch_file_layout(T, nf_dev, s.nf_cap, s.ep, b);
+  scan_u32(T, b.f_nwin, b.f_win, nf_dev, 1, s.nf_cap + 1, &st->n_windows, A.scan_tmp);
+  if (s.ep.bloom_bits) {
+    KLAUNCH(T, "guard", k_guard, 1, 1, 0, &st->n_windows, s.nw_cap, st, &st->need_windows);
+    launch_window_init(T, nw_dev, s.nw_cap, b);
+    launch_window_counts(T, nbo_dev, s.nbo_cap, nf_dev, nw_dev, b);
+    launch_window_sizes(T, nw_dev, s.nw_cap, s.ep, b);
+    scan_u64(T, b.w_fsz, b.w_fsz, nw_dev, 1, s.nw_cap + 1, nullptr, A.scan_tmp);
+  }
+  launch_file_sizes(T, nf_dev, s.nf_cap, s.ep, b);
+  scan_u64(T, b.f_off, b.f_off, nf_dev, 1, s.nf_cap + 1, &st->out_bytes, A.scan_tmp);
+  launch_write_blocks(T, k, nbo_dev, s.nbo_cap, s.ep, b, out, s.out_cap, st);
+  if (s.ep.bloom_bits) launch_write_filters(T, k, nw_dev, s.nw_cap, nf_dev, s.ep, b, out, s.out_cap, st);
+  launch_write_tail(T, k, nf_dev, s.nf_cap, s.ep, b, out, s.out_cap, st);
+}
+
+// Capacities and orbit plans for an encoder run over at most m entries.
+EncodeSetup plan_encode(const luda_options& o, u32 W, u64 m, u64 out_cap, u64 extra_files, u64 nf_cap_min,
+                        u64 nw_cap_min) {
+  EncodeSetup s;
+  memset(&s.ep, 0, sizeof(s.ep));
+  s.ep.block_size = o.block_size;
+  s.ep.max_file_size = o.max_file_size;
+  s.ep.interval = o.block_restart_interval;
+  s.ep.bloom_bits = o.bloom_bits_per_key;
+  u64 k = (u64)(o.bloom_bits_per_key * 0.69);  // util/bloom.cc:21
+  if (k < 1) k = 1;
+  if (k > 30) k = 30;
+  s.ep.bloom_k = (u32)k;
+  s.ep.kw = 8 * W + 8;
+  s.m_cap = (u32)std::max<u64>(m, 1);
+  s.nbo_cap = s.m_cap;
+  s.out_cap = out_cap;
+  s.size_cuts = o.max_file_size < out_cap;
+  u64 nf = s.size_cuts ? out_cap / std::max<u64>(o.max_file_size, 1) + extra_files + 2 : extra_files + 2;
+  nf = std::max<u64>(nf, nf_cap_min);
+  s.nf_cap = (u32)std::min<u64>(nf, s.nbo_cap);
+  u64 nw = o.bloom_bits_per_key ? out_cap / 2048 + s.nf_cap + 2 : 1;
+  s.nw_cap = (u32)std::max<u64>(nw, nw_cap_min);
+  // blocks: an entry is at least 4 bytes and 16 of them add a 4-byte restart
+  const u64 sb = o.block_size / 4 + 2;
+  const u32 Cb = std::max<u32>(2048, next_pow2(2 * sb));
+  const u32 Wb = (u32)std::min<u64>(sb, Cb);
+  s.ob = plan_orbit(s.m_cap, Cb, Wb, std::max<u32>(2, std::min<u32>(32, 16384 / Wb)));
+  if (Cb > kMaxChunk) throw Fail{LUDA_NOT_SUPPORTED, "block_size too large for the device block cut"};
+  if (s.size_cuts) {
+    // tables: every block but a table's last is at least block_size long
+    const u64 sf = o.max_file_size / (o.block_size + 5) + 2;
+    const u32 Cf = std::max<u32>(1024, next_pow2(2 * std::min<u64>(sf, s.nbo_cap)));
+    if (Cf > kMaxChunk) throw Fail{LUDA_NOT_SUPPORTED, "too many blocks per table for the device table cut"};
+    const u32 Wf = (u32)std::min<u64>(sf, Cf);
+    const u32 Gf = std::max<u32>(2, std::min<u32>(32, 16384 / Wf));
+    s.of = plan_orbit(s.nbo_cap, Cf, Wf, Gf);
+  }
+  return s;
+}
+
+void validate_options(const luda_options& o) {
+  if (o.compression != 0) throw Fail{LUDA_NOT_SUPPORTED, "only kNoCompression is supported"};
+  if (o.block_restart_interval < 1) throw Fail{LUDA_INVALID_ARGUMENT, "block_restart_interval < 1"};
+  if (o.block_size < 1 || o.block_size > kMaxBlockSize)
+    throw Fail{LUDA_NOT_SUPPORTED, "block_size outside what the device encoder handles"};
+  if (o.max_file_size < 1) throw Fail{LUDA_INVALID_ARGUMENT, "max_file_size < 1"};
+  if (o.bloom_bits_per_key > 1000) throw Fail{LUDA_NOT_SUPPORTED, "bloom_bits_per_key too large"};
+}
+
+// LUDA_KERNEL_TIMING=1 times every kernel with HIP events on the engine's
+// stream; LUDA_KERNEL_TIMING=name[,name...] times only those kernels.
+void init_ctx_timer(luda_ctx* c) {
+  const char* e = getenv("LUDA_KERNEL_TIMING");
+  c->T.on = e && *e && *e != '0';
+  c->T.only.clear();
+  if (c->T.on && strcmp(e, "1") != 0) {
+    std::string s(e);
+    size_t p = 0;
+    while (p <= s.size()) {
+      size_t q = s.find(',', p);
+      if (q == std::string::npos) q = s.size();
+      if (q > p) c->T.only.push_back(s.substr(p, q - p));
+      p = q + 1;
+    }
+  }
+  c->T.reset();
+}
+
+// Algorithmic HBM bytes of one launch of the kernels worth a roofline
+// (each input byte read once, each output byte written once):
+//   decode        data blocks read + key planes, tags, value refs written
+//   merge         per round: indices read and written + the keys compared once
+//   write_blocks  key and value bytes read + data blocks written
+//   write_filters user keys read + filter bytes written (approx. 1.25 B/key)
+u64 kernel_bytes(const std::string& name, const luda_stats& s, u32 W, u64 rounds) {
+  const u64 n = s.entries_in, m = s.entries_kept;
+  const u64 soa = 8ull * W + 4 + 8 + 8 + 4;
+  if (name == "decode") return s.bytes_read + n * soa;
+  if (name == "merge") return rounds ? n * (4 + 4 + soa - 12) : 0;
+  if (name == "write_blocks") return 2 * s.data_block_bytes;
+  if (name == "write_filters") return m * (8ull * W + 4) + m * 2;
+  if (name == "drop") return n * (4 + 4 + 8 + 8ull * W + 4);
+  if (name == "gather") return n * 8 + 2 * m * soa;
+  return 0;
+}
+
+void fill_kernel_stats(luda_ctx* c, luda_stats* s, u32 W, u64 rounds) {
+  if (!c->T.on) return;
+  std::map<std::string, std::pair<double, u64>> acc;
+  for (auto& r : c->T.recs) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      auto& x = acc[r.name];
+      x.first += ms;
+      x.second++;
+    }
+  }
+  double best = -1;
+  for (auto& kv : acc) {
+    if (kv.second.first > best) {
+      best = kv.second.first;
+      s->top_kernel_ms = kv.second.first;
+      s->top_kernel_launches = kv.second.second;
+      snprintf(s->top_kernel, sizeof(s->top_kernel), "%s", kv.first.c_str());
+    }
+  }
+  if (best >= 0) s->top_kernel_bytes = kernel_bytes(s->top_kernel, *s, W, rounds);
+}
+
+// Reads the outputs back and builds the result.
+luda_result* finish_result(luda_ctx* c, const EncodeSetup& s, const EncodeArrays& A, void* out, const DevStatus& st,
+                           u32 flags, luda_stats stats) {
+  const u32 nf = st.n_files_out;
+  luda_result* r = (luda_result*)calloc(1, sizeof(luda_result));
+  r->num_outputs = nf;
+  r->outputs = (luda_output*)calloc(nf ? nf : 1, sizeof(luda_output));
+  std::vector<FileMeta> meta(nf);
+  std::vector<u8> keys((u64)nf * 2 * s.ep.kw);
+  if (nf) {
+    HIPCHK(hipMemcpyAsync(meta.data(), A.eb.meta, nf * sizeof(FileMeta), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(keys.data(), A.eb.keys, keys.size(), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+  }
+  u64 kbytes = 0;
+  for (u32 i = 0; i < nf; i++) kbytes += meta[i].slen + meta[i].llen;
+  r->keys = (uint8_t*)malloc(kbytes ? kbytes : 1);
+  u64 ko = 0;
+  for (u32 i = 0; i < nf; i++) {
+    luda_output& o = r->outputs[i];
+    o.offset = meta[i].offset;
+    o.file_size = meta[i].size;
+    o.num_entries = meta[i].nent;
+    o.smallest_offset = ko;
+    o.smallest_size = meta[i].slen;
+    memcpy(r->keys + ko, &keys[(u64)i * 2 * s.ep.kw], meta[i].slen);
+    ko += meta[i].slen;
+    o.largest_offset = ko;
+    o.largest_size = meta[i].llen;
+    memcpy(r->keys + ko, &keys[(u64)i * 2 * s.ep.kw + s.ep.kw], meta[i].llen);
+    ko += meta[i].llen;
+  }
+  r->total_bytes = st.out_bytes;
+  r->device_data = out;
+  if (!(flags & LUDA_JOB_KEEP_ON_DEVICE)) {
+    const double t = now_ms();
+    void* h = nullptr;
+    if (hipHostMalloc(&h, std::max<u64>(st.out_bytes, 1), hipHostMallocDefault) != hipSuccess) {
+      free(r->outputs);
+      free(r->keys);
+      free(r);
+      throw Fail{LUDA_IO_ERROR, "pinned host allocation for the output failed"};
+    }
+    if (st.out_bytes) HIPCHK(hipMemcpyAsync(h, out, st.out_bytes, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    r->data = (uint8_t*)h;
+    stats.d2h_ms = now_ms() - t;
+  }
+  stats.bytes_written = st.out_bytes;
+  stats.entries_kept = st.n_kept;
+  stats.blocks_out = st.n_blocks_out;
+  stats.data_block_bytes = st.data_bytes;
+  r->stats = stats;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// luda_compact
+
+struct Child {
+  u32 f0, f1;  // files [f0, f1)
+};
+
+luda_result* compact(luda_ctx* c, const luda_job* job) {
+  const double t_call = now_ms();
+  luda_stats stats;
+  memset(&stats, 0, sizeof(stats));
+  validate_options(job->options);
+  if (job->level < 0 || job->level >= 6) throw Fail{LUDA_INVALID_ARGUMENT, "level out of range"};
+  // Inputs in MakeInputIterator order (db/version_set.cc:1237-1268).
+  std::vector<const luda_file*> files;
+  std::vector<Child> children;
+  for (int which = 0; which < 2; which++) {
+    const u32 n = job->num_inputs[which];
+    if (!n) continue;
+    const u32 f0 = (u32)files.size();
+    for (u32 i = 0; i < n; i++) files.push_back(&job->inputs[which][i]);
+    if (job->level + which == 0) {
+      for (u32 i = 0; i < n; i++) children.push_back({f0 + i, f0 + i + 1});
+    } else {
+      children.push_back({f0, f0 + n});
+    }
+  }
+  const u32 nfiles = (u32)files.size();
+  u64 bytes_in = 0;
+  for (auto* f : files) bytes_in += f->size;
+  stats.bytes_read = bytes_in;
+  // Grandparents: sorted internal keys (Compaction::grandparents_ come from a
+  // sorted level); deeper levels: Version::files_ order, sorted by key.
+  for (u32 i = 0; i < job->num_grandparents; i++) {
+    const luda_slice& k = job->grandparents[i].largest;
+    if (k.size < 8) throw Fail{LUDA_NOT_SUPPORTED, "grandparent key shorter than 8 bytes"};
+    if (i > 0) {
+      const luda_slice& p = job->grandparents[i - 1].largest;
+      if (ikey_cmp_host(p.data, p.size, k.data, k.size) > 0) throw Fail{LUDA_NOT_SUPPORTED, "grandparents not sorted"};
+    }
+  }
+  std::vector<std::vector<u32>> lvl_files(8);
+  for (u32 i = 0; i < job->num_deeper; i++) {
+    const int L = job->deeper[i].level;
+    if (L >= job->level + 2 && L < 7) lvl_files[L].push_back(i);
+  }
+  for (int L = 0; L < 7; L++)
+    for (size_t j = 1; j < lvl_files[L].size(); j++) {
+      const luda_level_file& a = job->deeper[lvl_files[L][j - 1]];
+      const luda_level_file& b = job->deeper[lvl_files[L][j]];
+      if (bytewise_cmp(a.largest_user_key.data, a.largest_user_key.size, b.largest_user_key.data,
+                       b.largest_user_key.size) > 0)
+        throw Fail{LUDA_NOT_SUPPORTED, "deeper level files not sorted"};
+    }
+  if (nfiles == 0) {
+    luda_result* r = (luda_result*)calloc(1, sizeof(luda_result));
+    r->outputs = (luda_output*)calloc(1, sizeof(luda_output));
+    r->keys = (uint8_t*)malloc(1);
+    r->stats = stats;
+    r->stats.attempts = 1;
+    return r;
+  }
+
+  HIPCHK(c->pin.reserve(1024 + 4ull * (nfiles + 1) + sizeof(DevStatus)));
+  init_ctx_timer(c);
+  // ---- inputs resident in HBM
+  std::vector<FileDesc> fdesc(nfiles);
+  if (job->flags & LUDA_JOB_INPUTS_ON_DEVICE) {
+    for (u32 i = 0; i < nfiles; i++) fdesc[i] = {files[i]->data, files[i]->size};
+  } else {
+    const double t = now_ms();
+    u64 need = 0;
+    for (auto* f : files) need += align_up(f->size + 16, 256);
+    HIPCHK(c->in_buf.reserve(need));
+    u64 off = 0;
+    for (u32 i = 0; i < nfiles; i++) {
+      HIPCHK(hipMemcpyAsync(c->in_buf.p + off, files[i]->data, files[i]->size, hipMemcpyHostToDevice, c->s));
+      fdesc[i] = {c->in_buf.p + off, files[i]->size};
+      off += align_up(files[i]->size + 16, 256);
+    }
+    HIPCHK(hipStreamSynchronize(c->s));
+    stats.h2d_ms = now_ms() - t;
+  }
+  HIPCHK(hipEventRecord(c->ev0, c->s));
+
+  // ---- phase A: footers, index blocks, restart arrays
+  u64 nb_cap = 0;
+  for (auto* f : files) nb_cap += f->size / 41 + 1;
+  if (nb_cap > 0xffffffffull / 2) throw Fail{LUDA_NOT_SUPPORTED, "inputs too large"};
+  auto carve_a = [&](Arena& a, FileDesc*& d_files, FileInfo*& d_info, u32*& d_nblk, u32*& d_bbase, BlockDesc*& d_blks,
+                     u32*& d_nent, u32*& d_ebase, u32*& d_febase, void*& tmp) {
+    d_files = a.take<FileDesc>(nfiles);
+    d_info = a.take<FileInfo>(nfiles);
+    d_nblk = a.take<u32>(nfiles + 1);
+    d_bbase = a.take<u32>(nfiles + 1);
+    d_blks = a.take<BlockDesc>(nb_cap);
+    d_nent = a.take<u32>(nb_cap + 1);
+    d_ebase = a.take<u32>(nb_cap + 1);
+    d_febase = a.take<u32>(nfiles + 1);
+    tmp = a.take<u8>(scan_tmp_bytes(std::max<u64>(nb_cap, nfiles) + 2));
+  };
+  FileDesc* d_files;
+  FileInfo* d_info;
+  u32 *d_nblk, *d_bbase, *d_nent, *d_ebase, *d_febase;
+  BlockDesc* d_blks;
+  void* tmpA;
+  {
+    Arena dry;
+    dry.dry = true;
+    carve_a(dry, d_files, d_info, d_nblk, d_bbase, d_blks, d_nent, d_ebase, d_febase, tmpA);
+    HIPCHK(c->a_buf.reserve(dry.used));
+    Arena a;
+    a.base = c->a_buf.p;
+    a.cap = c->a_buf.cap;
+    carve_a(a, d_files, d_info, d_nblk, d_bbase, d_blks, d_nent, d_ebase, d_febase, tmpA);
+  }
+  DevStatus st0;
+  memset(&st0, 0, sizeof(st0));
+  fill_status(c, st0);
+  upload(c, d_files, fdesc.data(), nfiles * sizeof(FileDesc));
+  KTimer& T = c->T;
+  fill_u32(T, d_nblk, 0, nfiles + 1);
+  launch_files(T, d_files, nfiles, d_info, d_nblk, c->st);
+  scan_u32(T, d_nblk, d_bbase, nullptr, nfiles + 1, nfiles + 1, nullptr, tmpA);
+  {
+    // n_blocks_in = blk_base[nfiles]
+    HIPCHK(hipMemcpyAsync(&c->st->n_blocks_in, d_bbase + nfiles, 4, hipMemcpyDeviceToDevice, c->s));
+  }
+  launch_index(T, d_files, d_info, d_bbase, nfiles, d_blks, c->st);
+  launch_blkscan(T, d_files, d_blks, &c->st->n_blocks_in, (u32)nb_cap, d_nent, c->st);
+  scan_u32(T, d_nent, d_ebase, &c->st->n_blocks_in, 0, (u32)nb_cap, &c->st->n_entries, tmpA);
+  launch_file_ebase(T, d_bbase, d_ebase, nfiles, &c->st->n_entries, d_febase);
+  DevStatus stA;
+  std::vector<u32> febase(nfiles + 1);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(c->pin.p, c->st, sizeof(DevStatus), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(c->pin.p + 1024, d_febase, (nfiles + 1) * 4, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  memcpy(&stA, c->pin.p, sizeof(DevStatus));
+  memcpy(febase.data(), c->pin.p + 1024, (nfiles + 1) * 4);
+  if (status_code_of(stA) != LUDA_OK)
+    throw Fail{status_code_of(stA), std::string("input rejected: ") + site_name(stA.site)};
+  const u32 nb = stA.n_blocks_in;
+  const u32 n = stA.n_entries;
+  stats.blocks_in = nb;
+  stats.entries_in = n;
+  if (nb > nb_cap) throw Fail{LUDA_CORRUPTION, "more data blocks than the tables can hold"};
+
+  // ---- phase B
+  u32 W = std::max<u32>(1, (std::max<u32>(stA.max_rulen, 8 * c->kw_hint) + 7) / 8);
+  u64 out_cap = bytes_in + bytes_in / 4 + (1 << 20);
+  u64 nf_min = 0, nw_min = 0;
+  // Runs to merge: children's entry ranges, empty ones left out.
+  std::vector<u32> cstart;
+  std::vector<std::pair<u32, u32>> runs;
+  for (auto& ch : children) {
+    const u32 e0 = febase[ch.f0], e1 = febase[ch.f1];
+    cstart.push_back(e0);
+    if (e1 > e0) runs.push_back({e0, e1});
+  }
+  cstart.push_back(n);
+  // merge rounds
+  std::vector<std::vector<MergePair>> rounds;
+  {
+    auto cur = runs;
+    const u32 items = merge_items_per_tile();
+    while (cur.size() > 1) {
+      std::vector<MergePair> prs;
+      std::vector<std::pair<u32, u32>> nxt;
+      u32 tiles = 0;
+      for (size_t i = 0; i < cur.size(); i += 2) {
+        MergePair p;
+        p.a0 = cur[i].first;
+        p.a1 = cur[i].second;
+        p.b1 = i + 1 < cur.size() ? cur[i + 1].second : cur[i].second;
+        p.tile0 = tiles;
+        tiles += (p.b1 - p.a0 + items - 1) / items;
+        prs.push_back(p);
+        nxt.push_back({p.a0, p.b1});
+      }
+      rounds.push_back(prs);
+      cur = nxt;
+    }
+  }
+  const u32 ngp = job->num_grandparents;
+  u32 ndeep = 0;
+  for (int L = 0; L < 7; L++) ndeep += (u32)lvl_files[L].size();
+
+  for (u32 attempt = 1;; attempt++) {
+    if (W > kMaxKeyWords) throw Fail{LUDA_NOT_SUPPORTED, "user keys longer than the device key planes"};
+    EncodeSetup es = plan_encode(job->options, W, n, out_cap, ngp, nf_min, nw_min);
+    // carve phase B
+    Planes K;
+    u64 *vptr, *gp_words, *gp_tag, *gp_prefix, *deep_words;
+    u32 *vlen, *permA, *permB, *keep, *kpos, *gcnt, *events, *gp_len, *deep_len, *d_cstart;
+    MergePair* d_pairs;
+    EncodeArrays EA;
+    u64 npairs_total = 0;
+    for (auto& r : rounds) npairs_total += r.size();
+    auto carve_b = [&](Arena& a) {
+      K.P = a.take<u64>((u64)W * n);
+      K.klen = a.take<u32>(n);
+      K.tag = a.take<u64>(n);
+      K.stride = n;
+      K.W = W;
+      vptr = a.take<u64>(n);
+      vlen = a.take<u32>(n);
+      permA = a.take<u32>(n);
+      permB = a.take<u32>(n);
+      keep = a.take<u32>(n);
+      kpos = a.take<u32>(n);
+      gcnt = a.take<u32>(ngp ? n : 1);
+      events = a.take<u32>(ngp + 2);
+      gp_words = a.take<u64>((u64)std::max<u32>(ngp, 1) * W);
+      gp_len = a.take<u32>(std::max<u32>(ngp, 1));
+      gp_tag = a.take<u64>(std::max<u32>(ngp, 1));
+      gp_prefix = a.take<u64>(ngp + 1);
+      deep_words = a.take<u64>((u64)std::max<u32>(ndeep, 1) * 2 * W);
+      deep_len = a.take<u32>(std::max<u32>(ndeep, 1) * 2);
+      d_cstart = a.take<u32>(cstart.size());
+      d_pairs = a.take<MergePair>(std::max<u64>(npairs_total, 1));
+      EA = carve_encode(a, es, W, true);
+    };
+    {
+      Arena dry;
+      dry.dry = true;
+      carve_b(dry);
+      if (dry.used > c->budget) throw Fail{LUDA_NOT_SUPPORTED, "job exceeds the HBM budget"};
+      HIPCHK(c->b_buf.reserve(dry.used));
+      Arena a;
+      a.base = c->b_buf.p;
+      a.cap = c->b_buf.cap;
+      carve_b(a);
+    }
+    void* out = c->take_out(std::max<u64>(out_cap, 1));
+    if (!out) throw Fail{LUDA_IO_ERROR, "output allocation failed"};
+    // host-side tables: grandparents, deeper levels, children, merge pairs
+    {
+      std::vector<u64> gw((u64)std::max<u32>(ngp, 1) * W, 0), gt(std::max<u32>(ngp, 1), 0), gpre(ngp + 1, 0);
+      std::vector<u32> gl(std::max<u32>(ngp, 1), 0);
+      for (u32 i = 0; i < ngp; i++) {
+        const luda_slice& k = job->grandparents[i].largest;
+        to_words(k.data, k.size - 8, W, &gw[(u64)i * W]);
+        gl[i] = (u32)(k.size - 8);
+        memcpy(&gt[i], k.data + k.size - 8, 8);
+        gpre[i + 1] = gpre[i] + job->grandparents[i].file_size;
+      }
+      std::vector<u64> dw((u64)std::max<u32>(ndeep, 1) * 2 * W, 0);
+      std::vector<u32> dl(std::max<u32>(ndeep, 1) * 2, 0);
+      DropParams dp;
+      memset(&dp, 0, sizeof(dp));
+      u32 di = 0;
+      for (int L = 0; L < 7; L++) {
+        dp.lvl_begin[L] = di;
+        for (u32 idx : lvl_files[L]) {
+          const luda_level_file& f = job->deeper[idx];
+          to_words(f.smallest_user_key.data, f.smallest_user_key.size, W, &dw[(2ull * di) * W]);
+          to_words(f.largest_user_key.data, f.largest_user_key.size, W, &dw[(2ull * di + 1) * W]);
+          dl[2 * di] = (u32)f.smallest_user_key.size;
+          dl[2 * di + 1] = (u32)f.largest_user_key.size;
+          di++;
+        }
+        dp.lvl_end[L] = di;
+      }
+      std::vector<MergePair> allp;
+      for (auto& r : rounds) allp.insert(allp.end(), r.begin(), r.end());
+      // one staging copy
+      std::vector<u8> blob;
+      auto put = [&](const void* p, u64 bytes) {
+        const u64 off = align_up(blob.size(), 16);
+        blob.resize(off + bytes);
+        memcpy(blob.data() + off, p, bytes);
+        return off;
+      };
+      const u64 o_gw = put(gw.data(), gw.size() * 8), o_gl = put(gl.data(), gl.size() * 4),
+                o_gt = put(gt.data(), gt.size() * 8), o_gp = put(gpre.data(), gpre.size() * 8),
+                o_dw = put(dw.data(), dw.size() * 8), o_dl = put(dl.data(), dl.size() * 4),
+                o_cs = put(cstart.data(), cstart.size() * 4),
+                o_mp = put(allp.data(), std::max<u64>(allp.size(), 0) * sizeof(MergePair));
+      HIPCHK(c->pin2.reserve(blob.size() + 16));
+      HIPCHK(hipStreamSynchronize(c->s));
+      memcpy(c->pin2.p, blob.data(), blob.size());
+      auto cp = [&](void* dst, u64 off, u64 bytes) {
+        if (bytes) HIPCHK(hipMemcpyAsync(dst, c->pin2.p + off, bytes, hipMemcpyHostToDevice, c->s));
+      };
+      cp(gp_words, o_gw, gw.size() * 8);
+      cp(gp_len, o_gl, gl.size() * 4);
+      cp(gp_tag, o_gt, gt.size() * 8);
+      cp(gp_prefix, o_gp, gpre.size() * 8);
+      cp(deep_words, o_dw, dw.size() * 8);
+      cp(deep_len, o_dl, dl.size() * 4);
+      cp(d_cstart, o_cs, cstart.size() * 4);
+      cp(d_pairs, o_mp, allp.size() * sizeof(MergePair));
+      dp.S = job->smallest_snapshot;
+      dp.limit = 10 * job->options.max_file_size;  // MaxGrandParentOverlapBytes, db/version_set.cc:31-33
+      dp.ngp = ngp;
+      dp.ndeep = ndeep;
+      dp.gp_words = gp_words;
+      dp.gp_len = gp_len;
+      dp.gp_tag = gp_tag;
+      dp.gp_prefix = gp_prefix;
+      dp.deep_words = deep_words;
+      dp.deep_len = deep_len;
+      dp.first_level = (u32)job->level + 2;
+
+      // status for this attempt: phase A counts, phase B zeroed
+      DevStatus sb = stA;
+      sb.err = 0;
+      sb.site = 0;
+      sb.max_ulen = 0;
+      fill_status(c, sb);
+      // decode, check, merge, drop, gather, stops
+      launch_decode(T, d_files, d_blks, nb, d_ebase, K, vptr, vlen, c->st);
+      launch_check_sorted(T, K, n, d_cstart, (u32)cstart.size() - 1, c->st);
+      const u32* perm = permA;
+      iota_u32(T, permA, n);
+      {
+        u32* in = permA;
+        u32* outp = permB;
+        u64 poff = 0;
+        for (auto& r : rounds) {
+          const MergePair& last = r.back();
+          const u32 items = merge_items_per_tile();
+          const u32 tiles = last.tile0 + (last.b1 - last.a0 + items - 1) / items;
+          // runs not touched by this round keep their positions: copy them
+          launch_merge(T, d_pairs + poff, (u32)r.size(), tiles, in, outp, K);
+          poff += r.size();
+          std::swap(in, outp);
+        }
+        perm = in;
+      }
+      launch_drop(T, perm, n, K, dp, keep, gcnt, c->st);
+      scan_u32(T, keep, kpos, nullptr, n, n, &c->st->n_kept, EA.scan_tmp);
+      launch_gather(T, perm, n, keep, kpos, K, vptr, vlen, EA.kept);
+      if (ngp) launch_stops(T, gcnt, kpos, keep, n, dp, events, ngp, EA.kept, c->st);
+      run_encode(c, es, EA, (u8*)out);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev1, c->s));
+    HIPCHK(hipMemcpyAsync(c->pin.p, c->st, sizeof(DevStatus), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    DevStatus stB;
+    memcpy(&stB, c->pin.p, sizeof(DevStatus));
+    float dms = 0;
+    hipEventElapsedTime(&dms, c->ev0, c->ev1);
+    stats.device_ms = dms;
+    if (status_code_of(stB) != LUDA_OK) {
+      c->give_out(out);
+      throw Fail{status_code_of(stB), std::string("compaction rejected: ") + site_name(stB.site)};
+    }
+    const bool overflow = (stB.err & ERR_OVERFLOW) || stB.out_bytes > out_cap;
+    if (overflow) {
+      c->give_out(out);
+      if (attempt >= kMaxAttempts) throw Fail{LUDA_NOT_SUPPORTED, std::string("capacity: ") + site_name(stB.site)};
+      bool grew = false;
+      if (stB.max_ulen > 8 * W) {
+        W = (stB.max_ulen + 7) / 8;
+        grew = true;
+      }
+      if (stB.out_bytes > out_cap) {
+        out_cap = stB.out_bytes + (1 << 20);
+        grew = true;
+      }
+      if (stB.need_files) {
+        nf_min = (u64)stB.need_files + 16;
+        grew = true;
+      }
+      if (stB.need_windows) {
+        nw_min = (u64)stB.need_windows + 16;
+        grew = true;
+      }
+      if (!grew) throw Fail{LUDA_NOT_SUPPORTED, std::string("capacity: ") + site_name(stB.site)};
+      continue;
+    }
+    c->kw_hint = std::max<u32>(c->kw_hint, (stB.max_ulen + 7) / 8);
+    stats.attempts = attempt;
+    stats.key_width = 8 * W;
+    stats.entries_kept = stB.n_kept;
+    stats.data_block_bytes = stB.data_bytes;
+    fill_kernel_stats(c, &stats, W, rounds.size());
+    luda_result* r = finish_result(c, es, EA, out, stB, job->flags, stats);
+    r->stats.total_ms = now_ms() - t_call;
+    return r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// luda_build_tables
+
+luda_result* build_tables(luda_ctx* c, const luda_options* opts, u64 n, const u8* keys, const u64* key_off,
+                          const u8* vals, const u64* val_off, u32 flags) {
+  const double t_call = now_ms();
+  luda_stats stats;
+  memset(&stats, 0, sizeof(stats));
+  validate_options(*opts);
+  if (n >= 0xffffffffull) throw Fail{LUDA_NOT_SUPPORTED, "too many entries"};
+  u64 maxu = 0;
+  for (u64 i = 0; i < n; i++) {
+    const u64 kl = key_off[i + 1] - key_off[i];
+    if (kl < 8) throw Fail{LUDA_NOT_SUPPORTED, "internal key shorter than 8 bytes"};
+    maxu = std::max<u64>(maxu, kl - 8);
+  }
+  const u32 W = (u32)std::max<u64>(1, (maxu + 7) / 8);
+  if (W > kMaxKeyWords) throw Fail{LUDA_NOT_SUPPORTED, "user keys longer than the device key planes"};
+  HIPCHK(c->pin.reserve(1 << 16));
+  init_ctx_timer(c);
+  const u64 kbytes = key_off[n], vbytes = val_off[n];
+  stats.entries_in = n;
+  if (n == 0) {
+    luda_result* r = (luda_result*)calloc(1, sizeof(luda_result));
+    r->outputs = (luda_output*)calloc(1, sizeof(luda_output));
+    r->keys = (uint8_t*)malloc(1);
+    r->stats = stats;
+    r->stats.attempts = 1;
+    return r;
+  }
+  // inputs: keys, offsets and values uploaded once
+  u8 *d_keys, *d_vals;
+  u64 *d_koff, *d_voff;
+  {
+    Arena dry;
+    dry.dry = true;
+    dry.take<u8>(kbytes + 16);
+    dry.take<u8>(vbytes + 16);
+    dry.take<u64>(n + 1);
+    dry.take<u64>(n + 1);
+    HIPCHK(c->in_buf.reserve(dry.used));
+    Arena a;
+    a.base = c->in_buf.p;
+    a.cap = c->in_buf.cap;
+    d_keys = a.take<u8>(kbytes + 16);
+    d_vals = a.take<u8>(vbytes + 16);
+    d_koff = a.take<u64>(n + 1);
+    d_voff = a.take<u64>(n + 1);
+  }
+  const double t = now_ms();
+  HIPCHK(hipMemcpyAsync(d_keys, keys, kbytes, hipMemcpyHostToDevice, c->s));
+  HIPCHK(hipMemcpyAsync(d_vals, vals, vbytes, hipMemcpyHostToDevice, c->s));
+  HIPCHK(hipMemcpyAsync(d_koff, key_off, (n + 1) * 8, hipMemcpyHostToDevice, c->s));
+  HIPCHK(hipMemcpyAsync(d_voff, val_off, (n + 1) * 8, hipMemcpyHostToDevice, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  stats.h2d_ms = now_ms() - t;
+  stats.bytes_read = kbytes + vbytes;
+  HIPCHK(hipEventRecord(c->ev0, c->s));
+  u64 out_cap = kbytes + vbytes + 16 * n + (1 << 20);
+  u64 nf_min = 0, nw_min = 0;
+  for (u32 attempt = 1;; attempt++) {
+    EncodeSetup es = plan_encode(*opts, W, n, out_cap, 0, nf_min, nw_min);
+    EncodeArrays EA;
+    {
+      Arena dry;
+      dry.dry = true;
+      carve_encode(dry, es, W, true);
+      if (dry.used > c->budget) throw Fail{LUDA_NOT_SUPPORTED, "job exceeds the HBM budget"};
+      HIPCHK(c->b_buf.reserve(dry.used));
+      Arena a;
+      a.base = c->b_buf.p;
+      a.cap = c->b_buf.cap;
+      EA = carve_encode(a, es, W, true);
+    }
+    void* out = c->take_out(out_cap);
+    if (!out) throw Fail{LUDA_IO_ERROR, "output allocation failed"};
+    DevStatus s0;
+    memset(&s0, 0, sizeof(s0));
+    s0.n_entries = (u32)n;
+    s0.n_kept = (u32)n;
+    fill_status(c, s0);
+    launch_ingest(c->T, n, d_keys, d_koff, d_vals, d_voff, EA.kept, c->st);
+    run_encode(c, es, EA, (u8*)out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev1, c->s));
+    HIPCHK(hipMemcpyAsync(c->pin.p, c->st, sizeof(DevStatus), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    DevStatus stB;
+    memcpy(&stB, c->pin.p, sizeof(DevStatus));
+    float dms = 0;
+    hipEventElapsedTime(&dms, c->ev0, c->ev1);
+    stats.device_ms = dms;
+    if (status_code_of(stB) != LUDA_OK) {
+      c->give_out(out);
+      throw Fail{status_code_of(stB), std::string("build rejected: ") + site_name(stB.site)};
+    }
+    if ((stB.err & ERR_OVERFLOW) || stB.out_bytes > out_cap) {
+      c->give_out(out);
+      if (attempt >= kMaxAttempts) throw Fail{LUDA_NOT_SUPPORTED, std::string("capacity: ") + site_name(stB.site)};
+      if (stB.out_bytes > out_cap) out_cap = stB.out_bytes + (1 << 20);
+      else out_cap *= 2;
+      if (stB.need_files) nf_min = (u64)stB.need_files + 16;
+      if (stB.need_windows) nw_min = (u64)stB.need_windows + 16;
+      continue;
+    }
+    stats.attempts = attempt;
+    stats.key_width = 8 * W;
+    stats.entries_kept = stB.n_kept;
+    stats.data_block_bytes = stB.data_bytes;
+    fill_kernel_stats(c, &stats, W, 0);
+    luda_result* r = finish_result(c, es, EA, out, stB, flags, stats);
+    r->stats.total_ms = now_ms() - t_call;
+    return r;
+  }
+}
+
+template <typename F>
+int guarded(luda_ctx* c, char* err, size_t errlen, F fn) {
+  try {
+    fn();
+    return LUDA_OK;
+  } catch (const Fail& f) {
+    if (err && errlen) snprintf(err, errlen, "%s", f.msg.c_str());
+    // a failed call may leave work queued; drain it so the next call starts clean
+    if (c && c->s) hipStreamSynchronize(c->s);
+    return f.code;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* luda_version(void) { return "luda-mi355x 0.1 (gfx950)"; }
+
+int luda_init(int device, uint64_t hbm_budget_bytes, luda_ctx** out, char* err, size_t errlen) {
+  if (!out) return LUDA_INVALID_ARGUMENT;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
+    if (err && errlen) snprintf(err, errlen, "no HIP device %d", device);
+    return LUDA_NOT_SUPPORTED;
+  }
+  luda_ctx* c = new luda_ctx;
+  c->device = device;
+  c->budget = hbm_budget_bytes ? hbm_budget_bytes : (64ull << 30);
+  int rc = guarded(c, err, errlen, [&] {
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    HIPCHK(hipMalloc(&c->st, sizeof(DevStatus)));
+    HIPCHK(hipEventCreate(&c->ev0));
+    HIPCHK(hipEventCreate(&c->ev1));
+    HIPCHK(c->pin.reserve(1 << 16));
+    c->T.s = c->s;
+  });
+  if (rc != LUDA_OK) {
+    luda_shutdown(c);
+    return rc;
+  }
+  *out = c;
+  return LUDA_OK;
+}
+
+void luda_shutdown(luda_ctx* c) {
+  if (!c) return;
+  if (c->s) hipStreamSynchronize(c->s);
+  c->in_buf.release();
+  c->a_buf.release();
+  c->b_buf.release();
+  c->pin.release();
+  c->pin2.release();
+  for (auto& kv : c->out_pool) hipFree(kv.second);
+  c->out_pool.clear();
+  if (c->st) hipFree(c->st);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->s) hipStreamDestroy(c->s);
+  delete c;
+}
+
+int luda_compact(luda_ctx* c, const luda_job* job, luda_result** out, char* err, size_t errlen) {
+  if (!c || !job || !out) return LUDA_INVALID_ARGUMENT;
+  *out = nullptr;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  return guarded(c, err, errlen, [&] { *out = compact(c, job); });
+}
+
+int luda_build_tables(luda_ctx* c, const luda_options* opts, uint64_t n, const uint8_t* keys, const uint64_t* key_off,
+                      const uint8_t* vals, const uint64_t* val_off, uint32_t flags, luda_result** out, char* err,
+                      size_t errlen) {
+  if (!c || !opts || !out || (n && (!keys || !key_off || !vals || !val_off))) return LUDA_INVALID_ARGUMENT;
+  *out = nullptr;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  return guarded(c, err, errlen, [&] { *out = build_tables(c, opts, n, keys, key_off, vals, val_off, flags); });
+}
+
+void luda_free_result(luda_ctx* c, luda_result* r) {
+  if (!r) return;
+  if (c && r->device_data) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->give_out(r->device_data);
+  }
+  if (r->data) hipHostFree(r->data);
+  free(r->outputs);
+  free(r->keys);
+  free(r);
+}
+
+int luda_device_alloc(luda_ctx* c, uint64_t bytes, void** dptr) {
+  if (!c || !dptr) return LUDA_INVALID_ARGUMENT;
+  hipSetDevice(c->device);
+  return hipMalloc(dptr, bytes ? bytes : 1) == hipSuccess ? LUDA_OK : LUDA_IO_ERROR;
+}
+int luda_device_free(luda_ctx* c, void* dptr) {
+  if (!c) return LUDA_INVALID_ARGUMENT;
+  hipSetDevice(c->device);
+  return hipFree(dptr) == hipSuccess ? LUDA_OK : LUDA_IO_ERROR;
+}
+int luda_memcpy_h2d(luda_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!c) return LUDA_INVALID_ARGUMENT;
+  hipSetDevice(c->device);
+  return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? LUDA_OK : LUDA_IO_ERROR;
+}
+int luda_memcpy_d2h(luda_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!c) return LUDA_INVALID_ARGUMENT;
+  hipSetDevice(c->device);
+  return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? LUDA_OK : LUDA_IO_ERROR;
+}
+
+}  // extern "C"

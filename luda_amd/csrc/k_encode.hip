@@ -1,0 +1,793 @@
+// k_encode.hip — the table writer: TableBuilder (table/table_builder.cc),
+// BlockBuilder in the fork's format (table/block_builder.cc), the filter
+// block (table/filter_block.cc) with the bloom policy (util/bloom.cc) over
+// user keys (db/dbformat.cc:105-115), block trailers with masked CRC-32C
+// (table_builder.cc:178-195), the metaindex and index blocks with shortened
+// separators (db/dbformat.cc:69-101) and the footer (table/format.cc:31-40),
+// plus DoCompactionWork's cut into tables (db/db_impl.cc:1158-1181).
+//
+// Input: the kept stream (entries in output order, `forced` where a table
+// must start).  Every size the sequential writer would compute one entry at a
+// time is computed per entry, block or table in parallel; the two greedy cut
+// sequences (blocks, tables) come from the orbit kernels.
+#include "engine.h"
+
+namespace {
+
+constexpr u32 kThreads = 256;
+constexpr u32 kWaves = kThreads / 64;
+constexpr u32 kStage = 6144;   // bytes of one data block assembled in LDS per wave
+constexpr u32 kFBits = 4096;   // bytes of one bloom filter assembled in LDS per wave
+constexpr u64 kMaxSeqTag = ((((u64)1 << 56) - 1) << 8) | 1;  // kMaxSequenceNumber, kValueTypeForSeek
+
+__device__ __forceinline__ void wave_sync() {
+  __threadfence_block();
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ u32 wave_incl_scan(u32 v) {
+  const u32 lane = threadIdx.x & 63;
+#pragma unroll
+  for (u32 d = 1; d < 64; d <<= 1) {
+    u32 o = __shfl_up(v, d, 64);
+    if (lane >= d) v += o;
+  }
+  return v;
+}
+
+__device__ __forceinline__ u32 wave_max(u32 v) {
+#pragma unroll
+  for (u32 d = 32; d > 0; d >>= 1) v = max(v, (u32)__shfl_xor(v, d, 64));
+  return v;
+}
+
+// First byte where the user keys of a and b differ, capped at min length.
+__device__ __forceinline__ u32 ukey_diff(const Planes& K, u64 a, u64 b, u32 m) {
+  for (u32 w = 0; w < K.W; w++) {
+    const u64 x = K.P[w * K.stride + a] ^ K.P[w * K.stride + b];
+    if (x) {
+      const u32 pos = 8 * w + ((u32)__clzll((long long)x) >> 3);
+      return pos < m ? pos : m;
+    }
+  }
+  return m;
+}
+
+// Shared prefix of the internal keys user_key || fixed64(tag) of a and b
+// (BlockBuilder::Add, table/block_builder.cc:80-87).
+__device__ u32 ikey_lcp(const Planes& K, u64 a, u64 b) {
+  const u32 la = K.klen[a], lb = K.klen[b];
+  const u32 m = min(la, lb);
+  const u32 d = ukey_diff(K, a, b, m);
+  if (d < m) return d;
+  const u64 ta = K.tag[a], tb = K.tag[b];
+  const u32 lim = min(la, lb) + 8;
+  u32 i = m;
+  while (i < lim && ikey_byte(K, a, la, ta, i) == ikey_byte(K, b, lb, tb, i)) i++;
+  return i;
+}
+
+// Index key for a block boundary: InternalKeyComparator::FindShortestSeparator
+// (db/dbformat.cc:69-86) over BytewiseComparatorImpl's (util/comparator.cc:29-48).
+// Returns the key length; writes the key when out is non-null.
+__device__ u32 index_sep(const Planes& K, u64 a, u64 b, u8* out) {
+  const u32 la = K.klen[a], lb = K.klen[b];
+  const u32 m = min(la, lb);
+  const u32 d = ukey_diff(K, a, b, m);
+  if (d < m) {
+    const u32 da = ukey_byte(K, a, d), db = ukey_byte(K, b, d);
+    if (da < 0xff && da + 1 < db && d + 1 < la) {
+      if (out) {
+        for (u32 i = 0; i < d; i++) out[i] = ukey_byte(K, a, i);
+        out[d] = (u8)(da + 1);
+        for (u32 i = 0; i < 8; i++) out[d + 1 + i] = (u8)(kMaxSeqTag >> (8 * i));
+      }
+      return d + 1 + 8;
+    }
+  }
+  if (out) {
+    const u64 ta = K.tag[a];
+    for (u32 i = 0; i < la + 8; i++) out[i] = ikey_byte(K, a, la, ta, i);
+  }
+  return la + 8;
+}
+
+// Index key after a table's last block: FindShortSuccessor
+// (db/dbformat.cc:88-101, util/comparator.cc:50-63).
+__device__ u32 index_succ(const Planes& K, u64 a, u8* out) {
+  const u32 la = K.klen[a];
+  for (u32 i = 0; i < la; i++) {
+    const u32 c = ukey_byte(K, a, i);
+    if (c != 0xff) {
+      if (i + 1 < la) {
+        if (out) {
+          for (u32 t = 0; t < i; t++) out[t] = ukey_byte(K, a, t);
+          out[i] = (u8)(c + 1);
+          for (u32 t = 0; t < 8; t++) out[i + 1 + t] = (u8)(kMaxSeqTag >> (8 * t));
+        }
+        return i + 1 + 8;
+      }
+      break;
+    }
+  }
+  if (out) {
+    const u64 ta = K.tag[a];
+    for (u32 i = 0; i < la + 8; i++) out[i] = ikey_byte(K, a, la, ta, i);
+  }
+  return la + 8;
+}
+
+// util/hash.cc over the user key of entry q, four bytes per step.
+__device__ u32 hash_ukey(const Planes& K, u64 q, u32 seed) {
+  const u32 n = min(K.klen[q], 8 * K.W);
+  const u32 m = 0xc6a4a793u;
+  u32 h = seed ^ (n * m);
+  u32 i = 0;
+  for (; i + 4 <= n; i += 4) {
+    const u64 word = K.P[(u64)(i >> 3) * K.stride + q];
+    const u32 half = (i & 4) ? (u32)word : (u32)(word >> 32);
+    h += __builtin_bswap32(half);
+    h *= m;
+    h ^= (h >> 16);
+  }
+  switch (n - i) {
+    case 3:
+      h += (u32)ukey_byte(K, q, i + 2) << 16;
+      [[fallthrough]];
+    case 2:
+      h += (u32)ukey_byte(K, q, i + 1) << 8;
+      [[fallthrough]];
+    case 1:
+      h += ukey_byte(K, q, i);
+      h *= m;
+      h ^= (h >> 24);
+      break;
+  }
+  return h;
+}
+
+// ---------------------------------------------------------------------------
+// sizes and cut points
+
+__global__ void k_entry_sizes(Kept k, const u32* m_dev, EncBufs b, DevStatus* st) {
+  const u64 m = *m_dev;
+  GRID_STRIDE(q, m) {
+    const Planes& K = k.K;
+    u32 sh = 0;
+    if (q > 0) {
+      if (ikey_cmp(K, q - 1, q) >= 0) set_err(st, ERR_UNSUPPORTED, SITE_UNSORTED);
+      sh = ikey_lcp(K, q - 1, q);
+    }
+    const u32 il = K.klen[q] + 8;
+    const u32 vl = k.vlen[q];
+    b.shared[q] = sh;
+    b.szr[q] = 1 + varint_len(il) + varint_len(vl) + il + vl;
+    b.sznr[q] = varint_len(sh) + varint_len(il - sh) + varint_len(vl) + (il - sh) + vl;
+  }
+}
+
+// The block that TableBuilder would build starting at entry q: entries are
+// added until CurrentSizeEstimate() >= block_size (table_builder.cc:119-123),
+// a table cut is due, or the stream ends.
+__global__ void k_bnext(Kept k, const u32* m_dev, EncParams ep, EncBufs b, DevStatus* st) {
+  const u64 m = *m_dev;
+  u32 smax = 0;
+  GRID_STRIDE(q, m) {
+    u64 acc = 0, j = q;
+    const u32 R = ep.interval;
+    for (;;) {
+      const u64 kq = j - q;
+      acc += (kq % R == 0) ? b.szr[j] : b.sznr[j];
+      j++;
+      const u64 est = acc + 4 * (kq / R + 1) + 4;
+      if (est >= ep.block_size || j >= m || k.forced[j]) break;
+    }
+    const u64 cnt = j - q;
+    b.bnext[q] = (u32)j;
+    b.bbytes[q] = (u32)(acc + 4 * ((cnt - 1) / R + 1) + 4);
+    smax = max(smax, (u32)cnt);
+  }
+  smax = wave_max(smax);
+  if ((threadIdx.x & 63) == 0 && smax) atomicMax(&st->blk_step_max, smax);
+}
+
+__global__ void k_block_info(Kept k, const u32* m_dev, EncBufs b, DevStatus* st) {
+  const u64 m = *m_dev;
+  GRID_STRIDE(q, m) {
+    if (!b.bs[q]) continue;
+    const u32 id = b.bid[q];
+    b.ob_first[id] = (u32)q;
+    b.ob_n[id] = b.bnext[q] - (u32)q;
+    b.ob_bytes[id] = b.bbytes[q];
+    b.ob_forced[id] = (q > 0 && k.forced[q]) ? 1u : 0u;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const u32 nbo = st->n_blocks_out;
+    b.ob_forced[nbo] = 0;
+    b.ob_bytes[nbo] = 0;
+  }
+}
+
+// ob_bytes + 5 in place of a separate array for the offset scan.
+__global__ void k_block_tot(const u32* nbo_dev, EncBufs b, u32* tot) {
+  const u64 nbo = *nbo_dev;
+  GRID_STRIDE(i, nbo + 1) tot[i] = i < nbo ? b.ob_bytes[i] + 5 : 0;
+}
+
+__global__ void k_block_forced(const u32* nbo_dev, EncBufs b) {
+  const u64 nbo = *nbo_dev;
+  GRID_STRIDE(i, nbo) {
+    if (b.ob_forced[i]) b.flist[b.ff[i]] = (u32)i;
+  }
+}
+
+// The table DoCompactionWork would build starting at block i: blocks are
+// added until FileSize() >= max_file_size (db/db_impl.cc:1173-1180), or up to
+// the next block that ShouldStopBefore starts (:1105-1111).
+__global__ void k_fnext(const u32* nbo_dev, const u32* nforced_dev, EncParams ep, EncBufs b, DevStatus* st) {
+  const u64 nbo = *nbo_dev;
+  const u32 nforced = *nforced_dev;
+  u32 smax = 0;
+  GRID_STRIDE(i, nbo) {
+    const u64 base = b.O[i];
+    const u64 target = ep.max_file_size > ~0ull - base ? ~0ull : base + ep.max_file_size;
+    u64 lo = i + 1, hi = nbo;  // smallest k in [i+1, nbo] with O[k] >= target, else nbo
+    while (lo < hi) {
+      const u64 mid = (lo + hi) >> 1;
+      if (b.O[mid] >= target) hi = mid;
+      else lo = mid + 1;
+    }
+    u64 nx = lo;
+    const u32 before = b.ff[i] + b.ob_forced[i];  // forced blocks at or before i
+    if (before < nforced) nx = min(nx, (u64)b.flist[before]);
+    b.fnext[i] = (u32)nx;
+    smax = max(smax, (u32)(nx - i));
+  }
+  smax = wave_max(smax);
+  if ((threadIdx.x & 63) == 0 && smax) atomicMax(&st->file_step_max, smax);
+}
+
+__global__ void k_file_first(const u32* nbo_dev, EncBufs b, const u32* nf_dev) {
+  const u64 nbo = *nbo_dev;
+  const u32 nf = *nf_dev;  // 0 when the table count overflowed its capacity
+  GRID_STRIDE(i, nbo) {
+    if (b.fs[i] && b.fid[i] < nf) b.f_first[b.fid[i]] = (u32)i;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) b.f_first[*nf_dev] = (u32)nbo;
+}
+
+__device__ __forceinline__ u32 file_of_block(const EncBufs& b, u32 nf, u64 i) {
+  u32 lo = 0, hi = nf;
+  while (hi - lo > 1) {
+    const u32 mid = (lo + hi) >> 1;
+    if (b.f_first[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void k_block_layout(Kept k, const u32* nbo_dev, const u32* nf_dev, EncParams ep, EncBufs b) {
+  const u64 nbo = *nbo_dev;
+  const u32 nf = *nf_dev;
+  if (nf == 0) return;  // no tables, or their count overflowed (reported)
+  GRID_STRIDE(i, nbo) {
+    const u32 f = file_of_block(b, nf, i);
+    const u64 off = b.O[i] - b.O[b.f_first[f]];
+    b.ob_file[i] = f;
+    b.ob_off[i] = off;
+    const u64 last = (u64)b.ob_first[i] + b.ob_n[i] - 1;
+    const u32 kl = (i + 1 < b.f_first[f + 1]) ? index_sep(k.K, last, b.ob_first[i + 1], nullptr)
+                                              : index_succ(k.K, last, nullptr);
+    const u32 hl = varint_len(off) + varint_len(b.ob_bytes[i]);
+    b.ob_isz[i] = 1 + varint_len(kl) + varint_len(hl) + kl + hl;
+  }
+}
+
+__global__ void k_isz_sentinel(const u32* nbo_dev, EncBufs b) { b.ob_isz[*nbo_dev] = 0; }
+
+// Filter windows of a table: FilterBlockBuilder emits one filter per 2 KiB of
+// block start offsets up to the data end, plus one for keys still pending at
+// Finish (table/filter_block.cc:21-49).
+__global__ void k_file_layout(const u32* nf_dev, EncParams ep, EncBufs b) {
+  const u64 nf = *nf_dev;
+  GRID_STRIDE(f, nf) {
+    const u32 b0 = b.f_first[f], b1 = b.f_first[f + 1];
+    const u64 dend = b.O[b1] - b.O[b0];
+    const u64 last = b.O[b1 - 1] - b.O[b0];
+    b.f_dend[f] = dend;
+    u64 nw = 0;
+    if (ep.bloom_bits) nw = max(dend >> 11, (last >> 11) + 1);
+    b.f_nwin[f] = (u32)nw;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) b.f_nwin[nf] = 0;
+}
+
+__global__ void k_window_init(const u32* nw_dev, EncBufs b) {
+  const u64 nw = *nw_dev;
+  GRID_STRIDE(w, nw + 1) {
+    b.w_keys[w] = 0;
+    b.w_first[w] = 0xffffffffu;
+  }
+}
+
+__global__ void k_window_counts(const u32* nbo_dev, const u32* nf_dev, const u32* nw_dev, EncBufs b) {
+  const u64 nbo = *nbo_dev;
+  const u32 nw = *nw_dev;  // 0 when the window count overflowed its capacity
+  if (*nf_dev == 0 || nw == 0) return;
+  GRID_STRIDE(i, nbo) {
+    const u32 w = b.f_win[b.ob_file[i]] + (u32)(b.ob_off[i] >> 11);
+    if (w >= nw) continue;
+    atomicAdd(&b.w_keys[w], b.ob_n[i]);
+    atomicMin(&b.w_first[w], b.ob_first[i]);
+  }
+}
+
+// BloomFilterPolicy::CreateFilter sizes (util/bloom.cc:30-38).
+__global__ void k_window_sizes(const u32* nw_dev, EncParams ep, EncBufs b) {
+  const u64 nw = *nw_dev;
+  GRID_STRIDE(w, nw + 1) {
+    const u64 n = w < nw ? b.w_keys[w] : 0;
+    u64 sz = 0;
+    if (n) {
+      u64 bits = n * ep.bloom_bits;
+      if (bits < 64) bits = 64;
+      sz = (bits + 7) / 8 + 1;
+    }
+    b.w_fsz[w] = sz;
+  }
+}
+
+__device__ __forceinline__ u64 meta_block_size(const EncParams& ep, u64 dend, u64 fb) {
+  if (!ep.bloom_bits) return 8;
+  const u64 hl = varint_len(dend) + varint_len(fb);
+  return 3 + 34 + hl + 4 + 4;
+}
+
+__global__ void k_file_sizes(const u32* nf_dev, EncParams ep, EncBufs b) {
+  const u64 nf = *nf_dev;
+  GRID_STRIDE(f, nf + 1) {
+    if (f == nf) {
+      b.f_off[f] = 0;
+      continue;
+    }
+    const u32 b0 = b.f_first[f], b1 = b.f_first[f + 1];
+    const u64 dend = b.f_dend[f];
+    u64 fb = 0, size = dend;
+    if (ep.bloom_bits) {
+      const u32 w0 = b.f_win[f], w1 = b.f_win[f + 1];
+      const u64 fbytes = b.w_fsz[w1] - b.w_fsz[w0];
+      b.f_fbytes[f] = fbytes;
+      fb = fbytes + 4ull * (w1 - w0) + 5;
+      size += fb + 5;
+    }
+    const u64 ib = b.ob_isz[b1] - b.ob_isz[b0] + 4ull * (b1 - b0) + 4;
+    b.f_ibytes[f] = ib;
+    size += meta_block_size(ep, dend, fb) + 5 + ib + 5 + 48;
+    b.f_off[f] = size;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// writers
+
+__device__ __forceinline__ void load_crc_tables(u32 (*sl)[256], u32 (*z)[256]) {
+  for (u32 i = threadIdx.x; i < 4 * 256; i += blockDim.x) {
+    sl[i >> 8][i & 255] = g_crc.slice[i >> 8][i & 255];
+    z[i >> 8][i & 255] = g_crc.zeros[6][i >> 8][i & 255];
+  }
+}
+
+
+// The bytes of output block i (entries, restart array, type byte, masked CRC)
+// assembled at buf: BlockBuilder::Add/Finish (table/block_builder.cc:62-114)
+// with the fork's (offset << 8) | count restart words, and
+// TableBuilder::WriteRawBlock (table/table_builder.cc:178-195).  buf is
+// either the wave's LDS stage or the block's place in the output; the two
+// are separate instantiations so that every access is a plain LDS (ds_*) or
+// global access and never a generic (flat) one that could address either.
+template <typename Buf>
+__device__ __forceinline__ void assemble_block(const Kept& k, const EncParams& ep, const EncBufs& b, u64 i, Buf buf,
+                                               const u32 (*sl)[256], const u32 (*z256)[256], DevStatus* st) {
+  const u32 lane = threadIdx.x & 63;
+  const u32 R = ep.interval;
+  const Planes& K = k.K;
+  const u64 q0 = b.ob_first[i];
+  const u32 cnt = b.ob_n[i];
+  const u64 bytes = b.ob_bytes[i];
+  // entries: header and key delta by the entry's lane
+  u64 base = 0;
+  for (u32 j0 = 0; j0 < cnt; j0 += 64) {
+    const u32 j = j0 + lane;
+    const bool act = j < cnt;
+    const u64 q = q0 + j;
+    const bool rs = (j % R) == 0;
+    const u32 sz = act ? (rs ? b.szr[q] : b.sznr[q]) : 0;
+    const u32 incl = wave_incl_scan(sz);
+    const u64 off = base + incl - sz;
+    u64 voff = 0, vsrc = 0;
+    u32 vl = 0;
+    if (act) {
+      const u32 ulen = K.klen[q];
+      const u32 il = ulen + 8;
+      const u32 sh = rs ? 0 : b.shared[q];
+      vl = k.vlen[q];
+      vsrc = k.vptr[q];
+      u64 p = off;
+      p = buf.put_varint(p, sh);
+      p = buf.put_varint(p, il - sh);
+      p = buf.put_varint(p, vl);
+      const u64 tag = K.tag[q];
+      for (u32 t = sh; t < il; t++) buf.put(p++, ikey_byte(K, q, ulen, tag, t));
+      voff = p;
+    }
+    // values: the whole wave copies each entry's value, coalesced
+    for (u32 t = 0; t < 64 && j0 + t < cnt; t++) {
+      const u64 vo = __shfl(voff, t, 64);
+      const u64 vs = __shfl(vsrc, t, 64);
+      const u32 vn = __shfl(vl, t, 64);
+      const u8* s = reinterpret_cast<const u8*>(vs);
+      for (u32 x = lane; x < vn; x += 64) buf.put(vo + x, s[x]);
+    }
+    base += __shfl(incl, 63, 64);
+  }
+  // restart array and count: the data ends at `base`
+  const u32 ngroups = (cnt + R - 1) / R;
+  {
+    u64 rbase = 0;
+    for (u32 j0 = 0; j0 < cnt; j0 += 64) {
+      const u32 j = j0 + lane;
+      const bool act = j < cnt;
+      const u64 q = q0 + j;
+      const u32 sz = act ? ((j % R) == 0 ? b.szr[q] : b.sznr[q]) : 0;
+      const u32 incl = wave_incl_scan(sz);
+      const u64 off = rbase + incl - sz;
+      if (act && (j % R) == 0) {
+        const u32 gcnt = min(R, cnt - j);
+        buf.put32(base + 4ull * (j / R), ((u32)off << 8) | gcnt);
+      }
+      rbase += __shfl(incl, 63, 64);
+    }
+  }
+  if (lane == 0) {
+    buf.put32(base + 4ull * ngroups, ngroups);
+    if (base + 4ull * ngroups + 4 != bytes) set_err(st, ERR_CORRUPT, SITE_CAPACITY);
+    buf.put(bytes, 0);  // kNoCompression
+  }
+  wave_sync();
+  const u32 crc = wave_crc32c(bytes + 1, sl, z256, [&](u64 x) -> u8 { return buf.get(x); });
+  if (lane == 0) buf.put32(bytes + 1, crc_mask(crc));
+  wave_sync();
+}
+
+// Byte sinks for assemble_block.  LdsBuf holds a pointer into a __shared__
+// array and nothing else, so the compiler addresses it with ds_* instructions.
+struct LdsBuf {
+  u8* p;
+  __device__ __forceinline__ void put(u64 x, u8 v) const { p[x] = v; }
+  __device__ __forceinline__ u8 get(u64 x) const { return p[x]; }
+  __device__ __forceinline__ void put32(u64 x, u32 v) const {
+    p[x] = (u8)v;
+    p[x + 1] = (u8)(v >> 8);
+    p[x + 2] = (u8)(v >> 16);
+    p[x + 3] = (u8)(v >> 24);
+  }
+  __device__ __forceinline__ u64 put_varint(u64 x, u64 v) const {
+    while (v >= 128) {
+      p[x++] = (u8)(v | 128);
+      v >>= 7;
+    }
+    p[x++] = (u8)v;
+    return x;
+  }
+};
+struct GlobalBuf {
+  u8* p;
+  __device__ __forceinline__ void put(u64 x, u8 v) const { p[x] = v; }
+  __device__ __forceinline__ u8 get(u64 x) const { return p[x]; }
+  __device__ __forceinline__ void put32(u64 x, u32 v) const {
+    p[x] = (u8)v;
+    p[x + 1] = (u8)(v >> 8);
+    p[x + 2] = (u8)(v >> 16);
+    p[x + 3] = (u8)(v >> 24);
+  }
+  __device__ __forceinline__ u64 put_varint(u64 x, u64 v) const {
+    while (v >= 128) {
+      p[x++] = (u8)(v | 128);
+      v >>= 7;
+    }
+    p[x++] = (u8)v;
+    return x;
+  }
+};
+
+// Copy n bytes from the LDS stage (src, which agrees with dst mod 16) to dst.
+__device__ __forceinline__ void wave_copy_out(u8* dst, const uint4* stage4, u32 mis, u64 n) {
+  const u32 lane = threadIdx.x & 63;
+  const u8* stage = reinterpret_cast<const u8*>(stage4);
+  const u64 head = (16 - mis) & 15;
+  const u64 h = head < n ? head : n;
+  if (lane < h) dst[lane] = stage[mis + lane];
+  const u64 body = (n - h) & ~15ull;
+  // stage[mis + h] is 16-byte aligned: mis + h is 0 or 16
+  const u32 s4 = (u32)((mis + h) >> 4);
+  uint4* d4 = reinterpret_cast<uint4*>(dst + h);
+  for (u64 x = lane; x < body / 16; x += 64) d4[x] = stage4[s4 + x];
+  const u64 t0 = h + body;
+  if (t0 + lane < n) dst[t0 + lane] = stage[mis + t0 + lane];
+}
+
+// One wavefront per data block.  Blocks that fit are assembled in the wave's
+// LDS stage and written out with 16-byte stores; larger ones (a single big
+// value) are assembled in place.
+__global__ __launch_bounds__(kThreads) void k_write_blocks(Kept k, const u32* nbo_dev, EncParams ep, EncBufs b,
+                                                            u8* out, u64 out_cap, DevStatus* st) {
+  __shared__ u32 sl[4][256];
+  __shared__ u32 z256[4][256];
+  __shared__ uint4 stage_all[kWaves][(kStage + 32) / 16];
+  if (st->out_bytes > out_cap || st->err) return;
+  load_crc_tables(sl, z256);
+  __syncthreads();
+  const u64 nbo = *nbo_dev;
+  const u32 wave = threadIdx.x >> 6;
+  for (u64 i = (u64)blockIdx.x * kWaves + wave; i < nbo; i += (u64)gridDim.x * kWaves) {
+    u8* dst = out + b.f_off[b.ob_file[i]] + b.ob_off[i];
+    const u64 bytes = b.ob_bytes[i];
+    if (bytes + 5 + 16 <= kStage) {
+      const u32 mis = (u32)((u64)dst & 15);
+      assemble_block(k, ep, b, i, LdsBuf{reinterpret_cast<u8*>(stage_all[wave]) + mis}, sl, z256, st);
+      wave_copy_out(dst, stage_all[wave], mis, bytes + 5);
+    } else {
+      assemble_block(k, ep, b, i, GlobalBuf{dst}, sl, z256, st);
+    }
+    wave_sync();
+  }
+}
+
+// One wavefront per filter window with keys: bloom bits with LDS atomics.
+__global__ __launch_bounds__(kThreads) void k_write_filters(Kept k, const u32* nw_dev, const u32* nf_dev,
+                                                             EncParams ep, EncBufs b, u8* out, u64 out_cap,
+                                                             DevStatus* st) {
+  __shared__ u32 bits_all[kWaves][kFBits / 4];
+  if (st->out_bytes > out_cap || st->err) return;
+  const u64 nw = *nw_dev;
+  const u32 nf = *nf_dev;
+  const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  u32* lb = bits_all[wave];
+  for (u64 w = (u64)blockIdx.x * kWaves + wave; w < nw; w += (u64)gridDim.x * kWaves) {
+    const u32 n = b.w_keys[w];
+    if (!n) continue;
+    // table of this window
+    u32 lo = 0, hi = nf;
+    while (hi - lo > 1) {
+      const u32 mid = (lo + hi) >> 1;
+      if (b.f_win[mid] <= w) lo = mid;
+      else hi = mid;
+    }
+    const u32 f = lo;
+    u8* dst = out + b.f_off[f] + b.f_dend[f] + (b.w_fsz[w] - b.w_fsz[b.f_win[f]]);
+    u64 nbits = (u64)n * ep.bloom_bits;
+    if (nbits < 64) nbits = 64;
+    const u64 nbytes = (nbits + 7) / 8;
+    nbits = nbytes * 8;
+    if (nbytes > kFBits) {
+      if (lane == 0) set_err(st, ERR_UNSUPPORTED, SITE_CAPACITY);
+      continue;
+    }
+    for (u32 i = lane; i < (nbytes + 3) / 4; i += 64) lb[i] = 0;
+    wave_sync();
+    const u64 q0 = b.w_first[w];
+    for (u32 t = lane; t < n; t += 64) {
+      u32 h = hash_ukey(k.K, q0 + t, 0xbc9f1d34u);
+      const u32 delta = (h >> 17) | (h << 15);
+      for (u32 j = 0; j < ep.bloom_k; j++) {
+        const u32 bp = (u32)(h % nbits);
+        atomicOr(&lb[bp >> 5], 1u << (bp & 31));
+        h += delta;
+      }
+    }
+    wave_sync();
+    for (u32 i = lane; i < nbytes; i += 64) dst[i] = (u8)(lb[i >> 2] >> (8 * (i & 3)));
+    if (lane == 0) dst[nbytes] = (u8)ep.bloom_k;
+    wave_sync();
+  }
+}
+
+__device__ __forceinline__ u8* put_fixed32(u8* p, u32 v) {
+  st32(p, v);
+  return p + 4;
+}
+
+// One workgroup per table: filter offsets, filter/metaindex/index blocks with
+// trailers, footer, and the table's metadata.
+__global__ __launch_bounds__(kThreads) void k_write_tail(Kept k, const u32* nf_dev, EncParams ep, EncBufs b, u8* out,
+                                                          u64 out_cap, DevStatus* st) {
+  __shared__ u32 sl[4][256];
+  __shared__ u32 z256[4][256];
+  __shared__ u64 s_moff, s_msize;
+  if (st->out_bytes > out_cap || st->err) return;
+  load_crc_tables(sl, z256);
+  __syncthreads();
+  const u64 nf = *nf_dev;
+  const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const Planes& K = k.K;
+  for (u64 f = blockIdx.x; f < nf; f += gridDim.x) {
+    const u32 b0 = b.f_first[f], b1 = b.f_first[f + 1];
+    u8* base = out + b.f_off[f];
+    const u64 dend = b.f_dend[f];
+    u64 fbsize = 0;
+    // filter block: offsets array, array offset, base lg (FilterBlockBuilder::Finish)
+    if (ep.bloom_bits) {
+      const u32 w0 = b.f_win[f], w1 = b.f_win[f + 1];
+      const u64 fbytes = b.f_fbytes[f];
+      u8* fs = base + dend;
+      for (u32 i = threadIdx.x; i < w1 - w0; i += blockDim.x)
+        st32(fs + fbytes + 4ull * i, (u32)(b.w_fsz[w0 + i] - b.w_fsz[w0]));
+      if (threadIdx.x == 0) {
+        st32(fs + fbytes + 4ull * (w1 - w0), (u32)fbytes);
+        fs[fbytes + 4ull * (w1 - w0) + 4] = 11;
+      }
+      fbsize = fbytes + 4ull * (w1 - w0) + 5;
+    }
+    const u64 moff = dend + (ep.bloom_bits ? fbsize + 5 : 0);
+    const u64 msize = meta_block_size(ep, dend, fbsize);
+    const u64 ioff = moff + msize + 5;
+    const u64 isize = b.f_ibytes[f];
+    // metaindex block (one BlockBuilder entry, table_builder.cc:216-227)
+    if (threadIdx.x == 0) {
+      u8* p = base + moff;
+      if (ep.bloom_bits) {
+        static const char kName[] = "filter.leveldb.BuiltinBloomFilter2";
+        const u32 hl = varint_len(dend) + varint_len(fbsize);
+        *p++ = 0;
+        *p++ = 34;
+        *p++ = (u8)hl;
+        for (u32 i = 0; i < 34; i++) *p++ = (u8)kName[i];
+        p = enc_varint(enc_varint(p, dend), fbsize);
+        p = put_fixed32(p, 1);  // (0 << 8) | 1
+        p = put_fixed32(p, 1);
+      } else {
+        p = put_fixed32(p, 0);  // (0 << 8) | 0
+        p = put_fixed32(p, 1);
+      }
+      s_moff = moff;
+      s_msize = msize;
+    }
+    // index block entries, restart interval 1
+    {
+      u8* ib = base + ioff;
+      const u64 ibase = b.ob_isz[b0];
+      const u64 isum = b.ob_isz[b1] - ibase;
+      for (u32 i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+        const u64 eoff = b.ob_isz[i] - ibase;
+        u8* p = ib + eoff;
+        const u64 last = (u64)b.ob_first[i] + b.ob_n[i] - 1;
+        const u32 kl = (i + 1 < b1) ? index_sep(K, last, b.ob_first[i + 1], nullptr) : index_succ(K, last, nullptr);
+        const u32 hl = varint_len(b.ob_off[i]) + varint_len(b.ob_bytes[i]);
+        *p++ = 0;
+        p = enc_varint(p, kl);
+        p = enc_varint(p, hl);
+        if (i + 1 < b1) index_sep(K, last, b.ob_first[i + 1], p);
+        else index_succ(K, last, p);
+        p += kl;
+        p = enc_varint(p, b.ob_off[i]);
+        p = enc_varint(p, b.ob_bytes[i]);
+        st32(ib + isum + 4ull * (i - b0), ((u32)eoff << 8) | 1u);
+      }
+      if (threadIdx.x == 0) st32(ib + isum + 4ull * (b1 - b0), b1 - b0);
+    }
+    __syncthreads();
+    // trailers: wave 0 filter block, wave 1 metaindex, wave 2 index
+    if (wave == 0 && ep.bloom_bits) {
+      u8* fs = base + dend;
+      if (lane == 0) fs[fbsize] = 0;
+      wave_sync();
+      const u32 crc = wave_crc32c(fbsize + 1, sl, z256, [&](u64 x) -> u8 { return fs[x]; });
+      if (lane == 0) st32(fs + fbsize + 1, crc_mask(crc));
+    } else if (wave == 1) {
+      u8* ms = base + s_moff;
+      if (lane == 0) ms[s_msize] = 0;
+      wave_sync();
+      const u32 crc = wave_crc32c(s_msize + 1, sl, z256, [&](u64 x) -> u8 { return ms[x]; });
+      if (lane == 0) st32(ms + s_msize + 1, crc_mask(crc));
+    } else if (wave == 2) {
+      u8* is = base + ioff;
+      if (lane == 0) is[isize] = 0;
+      wave_sync();
+      const u32 crc = wave_crc32c(isize + 1, sl, z256, [&](u64 x) -> u8 { return is[x]; });
+      if (lane == 0) st32(is + isize + 1, crc_mask(crc));
+    } else if (wave == 3 && lane == 0) {
+      // footer (Footer::EncodeTo, table/format.cc:31-40)
+      u8* p = base + ioff + isize + 5;
+      u8* s = p;
+      p = enc_varint(p, moff);
+      p = enc_varint(p, msize);
+      p = enc_varint(p, ioff);
+      p = enc_varint(p, isize);
+      while (p < s + 40) *p++ = 0;
+      p = put_fixed32(p, 0x8b80fb57u);
+      p = put_fixed32(p, 0xdb477524u);
+      // table metadata: Output::smallest / largest (db/db_impl.cc:1163-1166)
+      const u64 qs = b.ob_first[b0];
+      const u64 ql = (u64)b.ob_first[b1 - 1] + b.ob_n[b1 - 1] - 1;
+      FileMeta fm;
+      fm.offset = b.f_off[f];
+      fm.size = (u64)(p - base);
+      fm.nent = ql + 1 - qs;
+      fm.slen = K.klen[qs] + 8;
+      fm.llen = K.klen[ql] + 8;
+      b.meta[f] = fm;
+      u8* ks = b.keys + f * 2ull * ep.kw;
+      for (u32 t = 0; t < fm.slen; t++) ks[t] = ikey_byte(K, qs, fm.slen - 8, K.tag[qs], t);
+      u8* kl = ks + ep.kw;
+      for (u32 t = 0; t < fm.llen; t++) kl[t] = ikey_byte(K, ql, fm.llen - 8, K.tag[ql], t);
+    }
+    __syncthreads();
+  }
+}
+
+u32 grid_for(u64 n, u32 per_block) {
+  u64 g = (n + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > 16384) g = 16384;
+  return (u32)g;
+}
+
+}  // namespace
+
+void launch_entry_sizes(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncBufs b, DevStatus* st) {
+  KLAUNCH(T, "entry_sizes", k_entry_sizes, grid_for(m_cap, 256), 256, 0, k, m_dev, b, st);
+}
+void launch_bnext(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncParams ep, EncBufs b, DevStatus* st) {
+  KLAUNCH(T, "bnext", k_bnext, grid_for(m_cap, 256), 256, 0, k, m_dev, ep, b, st);
+}
+void launch_block_info(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncBufs b, DevStatus* st) {
+  KLAUNCH(T, "block_info", k_block_info, grid_for(m_cap, 256), 256, 0, k, m_dev, b, st);
+}
+void launch_block_tot(KTimer& T, const u32* nbo_dev, u32 nbo_cap, EncBufs b, u32* tot) {
+  KLAUNCH(T, "block_tot", k_block_tot, grid_for(nbo_cap + 1, 256), 256, 0, nbo_dev, b, tot);
+}
+void launch_block_forced(KTimer& T, const u32* nbo_dev, u32 nbo_cap, EncBufs b) {
+  KLAUNCH(T, "block_forced", k_block_forced, grid_for(nbo_cap, 256), 256, 0, nbo_dev, b);
+}
+void launch_fnext(KTimer& T, const u32* nbo_dev, u32 nbo_cap, const u32* nforced_dev, EncParams ep, EncBufs b,
+                  DevStatus* st) {
+  KLAUNCH(T, "fnext", k_fnext, grid_for(nbo_cap, 256), 256, 0, nbo_dev, nforced_dev, ep, b, st);
+}
+void launch_file_first(KTimer& T, const u32* nbo_dev, u32 nbo_cap, EncBufs b, const u32* nf_dev) {
+  KLAUNCH(T, "file_first", k_file_first, grid_for(nbo_cap, 256), 256, 0, nbo_dev, b, nf_dev);
+}
+void launch_block_layout(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, const u32* nf_dev, EncParams ep,
+                         EncBufs b) {
+  KLAUNCH(T, "isz_sentinel", k_isz_sentinel, 1, 1, 0, nbo_dev, b);
+  KLAUNCH(T, "block_layout", k_block_layout, grid_for(nbo_cap, 256), 256, 0, k, nbo_dev, nf_dev, ep, b);
+}
+void launch_file_layout(KTimer& T, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b) {
+  KLAUNCH(T, "file_layout", k_file_layout, grid_for(nf_cap, 256), 256, 0, nf_dev, ep, b);
+}
+void launch_window_init(KTimer& T, const u32* nw_dev, u32 nw_cap, EncBufs b) {
+  KLAUNCH(T, "window_init", k_window_init, grid_for(nw_cap + 1, 256), 256, 0, nw_dev, b);
+}
+void launch_window_counts(KTimer& T, const u32* nbo_dev, u32 nbo_cap, const u32* nf_dev, const u32* nw_dev,
+                          EncBufs b) {
+  KLAUNCH(T, "window_counts", k_window_counts, grid_for(nbo_cap, 256), 256, 0, nbo_dev, nf_dev, nw_dev, b);
+}
+void launch_window_sizes(KTimer& T, const u32* nw_dev, u32 nw_cap, EncParams ep, EncBufs b) {
+  KLAUNCH(T, "window_sizes", k_window_sizes, grid_for(nw_cap + 1, 256), 256, 0, nw_dev, ep, b);
+}
+void launch_file_sizes(KTimer& T, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b) {
+  KLAUNCH(T, "file_sizes", k_file_sizes, grid_for(nf_cap + 1, 256), 256, 0, nf_dev, ep, b);
+}
+void launch_write_blocks(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u8* out,
+                         u64 out_cap, DevStatus* st) {
+  KLAUNCH(T, "write_blocks", k_write_blocks, grid_for(nbo_cap, kWaves), kThreads, 0, k, nbo_dev, ep, b, out, out_cap,
+          st);
+}
+void launch_write_filters(KTimer& T, Kept k, const u32* nw_dev, u32 nw_cap, const u32* nf_dev, EncParams ep, EncBufs b,
+                          u8* out, u64 out_cap, DevStatus* st) {
+  KLAUNCH(T, "write_filters", k_write_filters, grid_for(nw_cap, kWaves), kThreads, 0, k, nw_dev, nf_dev, ep, b, out,
+          out_cap, st);
+}
+void launch_write_tail(KTimer& T, Kept k, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b, u8* out,
+                       u64 out_cap, DevStatus* st) {
+  KLAUNCH(T, "write_tail", k_write_tail, grid_for(nf_cap, 1), kThreads, 0, k, nf_dev, ep, b, out, out_cap, st);
+}

@@ -1,0 +1,210 @@
+// k_orbit.hip — the nodes reached from node 0 by a successor function nxt
+// (i < nxt[i] <= n, nxt[i] - i <= step).  This is how the encoder finds, in
+// parallel, the exact greedy cut points that TableBuilder makes one entry at a
+// time: where each data block starts (table/table_builder.cc:119-123) and
+// where each output table starts (db/db_impl.cc:1173-1180).
+//
+// The nodes are cut into chunks of C.  The orbit enters chunk c at one of its
+// first `step` nodes, so each chunk is summarised by a function from entry
+// offset to the entry offset of chunk c+1 (computed by pointer jumping in
+// LDS).  Those functions are composed in groups of G, level by level, until
+// one is left; the entry offsets are then pushed back down the levels, and
+// each chunk marks its own orbit nodes.
+#include "engine.h"
+
+namespace {
+
+constexpr u16 kEnd = 0xffff;
+constexpr u32 kThreads = 256;
+
+__device__ __forceinline__ u64 level_items(u64 n, u32 C, u32 G, u32 level) {
+  u64 it = (n + C - 1) / C;
+  for (u32 l = 0; l < level; l++) it = (it + G - 1) / G;
+  return it;
+}
+
+__global__ __launch_bounds__(kThreads) void k_orbit_chunk(const u32* nxt, const u32* n_dev, const u32* step_dev,
+                                                           u32 C, u32 Wc, u16* f, DevStatus* st) {
+  extern __shared__ u32 ex[];
+  const u64 n = *n_dev;
+  const u32 step = *step_dev;
+  if (step > Wc) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) set_err(st, ERR_OVERFLOW, SITE_BLOCK_ENTRIES);
+    return;
+  }
+  const u32 Wd = step ? step : 1;
+  const u64 nch = (n + C - 1) / C;
+  for (u64 c = blockIdx.x; c < nch; c += gridDim.x) {
+    const u64 c0 = c * C;
+    const u64 cend = c0 + C;
+    for (u32 i = threadIdx.x; i < C; i += blockDim.x) ex[i] = c0 + i < n ? nxt[c0 + i] : (u32)n;
+    __syncthreads();
+    for (u32 r = 1; r <= C; r <<= 1) {
+      for (u32 i = threadIdx.x; i < C; i += blockDim.x) {
+        const u32 v = ex[i];
+        if (v < cend && v < n) ex[i] = ex[v - c0];
+      }
+      __syncthreads();
+    }
+    for (u32 x = threadIdx.x; x < Wd; x += blockDim.x) {
+      u16 r = kEnd;
+      if (c0 + x < n && x < C) {
+        const u64 v = ex[x];
+        if (v < n) {
+          const u64 off = v - cend;
+          if (off >= Wd) set_err(st, ERR_OVERFLOW, SITE_BLOCK_ENTRIES);
+          else r = (u16)off;
+        }
+      }
+      f[c * Wc + x] = r;
+    }
+    __syncthreads();
+  }
+}
+
+// Level l -> l+1: compose G consecutive functions.
+__global__ __launch_bounds__(kThreads) void k_orbit_compose(const u16* fin, u16* fout, const u32* n_dev,
+                                                             const u32* step_dev, u32 C, u32 G, u32 Wc, u32 level) {
+  extern __shared__ u16 tab[];
+  const u64 n = *n_dev;
+  const u32 step = *step_dev;
+  if (step > Wc) return;  // k_orbit_chunk reported the overflow
+  const u32 Wd = step ? step : 1;
+  const u64 nin = level_items(n, C, G, level);
+  const u64 nout = (nin + G - 1) / G;
+  for (u64 o = blockIdx.x; o < nout; o += gridDim.x) {
+    const u64 t0 = o * G;
+    const u32 tc = (u32)min((u64)G, nin - t0);
+    for (u32 i = threadIdx.x; i < tc * Wd; i += blockDim.x) {
+      const u32 t = i / Wd, x = i % Wd;
+      tab[i] = fin[(t0 + t) * Wc + x];
+    }
+    __syncthreads();
+    for (u32 x = threadIdx.x; x < Wd; x += blockDim.x) {
+      u32 y = x;
+      for (u32 t = 0; t < tc && y != kEnd; t++) y = y < Wd ? tab[t * Wd + y] : kEnd;
+      fout[o * Wc + x] = (u16)y;
+    }
+    __syncthreads();
+  }
+}
+
+// Level l+1 entries -> level l entries.
+__global__ __launch_bounds__(kThreads) void k_orbit_expand(const u16* fin, const u32* eout, u32* ein, const u32* n_dev,
+                                                            const u32* step_dev, u32 C, u32 G, u32 Wc, u32 level) {
+  extern __shared__ u16 tab[];
+  const u64 n = *n_dev;
+  const u32 step = *step_dev;
+  if (step > Wc) return;  // k_orbit_chunk reported the overflow
+  const u32 Wd = step ? step : 1;
+  const u64 nin = level_items(n, C, G, level);
+  const u64 nout = (nin + G - 1) / G;
+  for (u64 o = blockIdx.x; o < nout; o += gridDim.x) {
+    const u64 t0 = o * G;
+    const u32 tc = (u32)min((u64)G, nin - t0);
+    for (u32 i = threadIdx.x; i < tc * Wd; i += blockDim.x) {
+      const u32 t = i / Wd, x = i % Wd;
+      tab[i] = fin[(t0 + t) * Wc + x];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      u32 y = eout[o];
+      for (u32 t = 0; t < tc; t++) {
+        ein[t0 + t] = y;
+        if (y != kEnd) y = y < Wd ? tab[t * Wd + y] : kEnd;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_orbit_root(u32* e) { *e = 0; }
+
+__global__ __launch_bounds__(kThreads) void k_orbit_mark(const u32* nxt, const u32* n_dev, const u32* step_dev,
+                                                          u32 C, u32 Wc, const u32* e0, u32* flag) {
+  extern __shared__ u32 loc[];
+  u8* fl = reinterpret_cast<u8*>(loc + C);
+  const u64 n = *n_dev;
+  if (*step_dev > Wc) return;  // k_orbit_chunk reported the overflow
+  const u64 nch = (n + C - 1) / C;
+  for (u64 c = blockIdx.x; c < nch; c += gridDim.x) {
+    const u64 c0 = c * C;
+    const u64 c1 = min(c0 + C, n);
+    for (u32 i = threadIdx.x; i < C; i += blockDim.x) {
+      loc[i] = c0 + i < n ? nxt[c0 + i] : (u32)n;
+      fl[i] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const u32 y = e0[c];
+      if (y < C) {
+        u64 node = c0 + y;
+        while (node < c1) {
+          fl[node - c0] = 1;
+          node = loc[node - c0];
+        }
+      }
+    }
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < c1 - c0; i += blockDim.x) flag[c0 + i] = fl[i];
+    __syncthreads();
+  }
+}
+
+u32 grid_for(u64 n) {
+  if (n < 1) n = 1;
+  if (n > 8192) n = 8192;
+  return (u32)n;
+}
+
+}  // namespace
+
+OrbitPlan plan_orbit(u64 n_cap, u32 C, u32 Wc, u32 G) {
+  OrbitPlan p;
+  p.C = C;
+  p.Wc = Wc;
+  p.G = G;
+  p.n_cap = n_cap;
+  u64 items = (n_cap + C - 1) / C;
+  if (items < 1) items = 1;
+  u64 toff = 0;
+  p.level_items.push_back(items);
+  p.level_offset.push_back(0);
+  toff += items * Wc;
+  while (items > 1) {
+    items = (items + p.G - 1) / p.G;
+    p.level_items.push_back(items);
+    p.level_offset.push_back(toff);
+    toff += items * Wc;
+  }
+  p.levels = (u32)p.level_items.size() - 1;
+  p.table_entries = toff;
+  u64 e = 0;
+  for (auto it : p.level_items) e += it;
+  p.entry_entries = e + 1;
+  return p;
+}
+
+void run_orbit(KTimer& T, const OrbitPlan& p, const u32* nxt, const u32* n_dev, const u32* step_dev, u32* flag,
+               u16* ftab, u32* entries, DevStatus* st) {
+  const u32 L = p.levels;
+  std::vector<u64> eoff(L + 1);
+  u64 acc = 0;
+  for (u32 l = 0; l <= L; l++) {
+    eoff[l] = acc;
+    acc += p.level_items[l];
+  }
+  KLAUNCH(T, "orbit_chunk", k_orbit_chunk, grid_for(p.level_items[0]), kThreads, p.C * sizeof(u32), nxt, n_dev,
+          step_dev, p.C, p.Wc, ftab, st);
+  for (u32 l = 0; l < L; l++)
+    KLAUNCH(T, "orbit_compose", k_orbit_compose, grid_for(p.level_items[l + 1]), kThreads,
+            (size_t)p.G * p.Wc * sizeof(u16), ftab + p.level_offset[l], ftab + p.level_offset[l + 1], n_dev, step_dev,
+            p.C, p.G, p.Wc, l);
+  KLAUNCH(T, "orbit_root", k_orbit_root, 1, 1, 0, entries + eoff[L]);
+  for (u32 l = L; l-- > 0;)
+    KLAUNCH(T, "orbit_expand", k_orbit_expand, grid_for(p.level_items[l + 1]), kThreads,
+            (size_t)p.G * p.Wc * sizeof(u16), ftab + p.level_offset[l], entries + eoff[l + 1], entries + eoff[l],
+            n_dev, step_dev, p.C, p.G, p.Wc, l);
+  KLAUNCH(T, "orbit_mark", k_orbit_mark, grid_for(p.level_items[0]), kThreads, p.C * sizeof(u32) + p.C, nxt, n_dev,
+          step_dev, p.C, p.Wc, entries, flag);
+}

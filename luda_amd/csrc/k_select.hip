@@ -1,0 +1,195 @@
+// k_select.hip — which merged entries survive, and where ShouldStopBefore
+// starts a new output table.
+//
+// The drop rule of DBImpl::DoCompactionWork (db/db_impl.cc:1113-1147) only
+// looks at an entry and its predecessor: last_sequence_for_key is the
+// predecessor's sequence when the predecessor parsed and has the same user
+// key, else kMaxSequenceNumber.  So
+//   drop(p) = parsed(p) && ((parsed(p-1) && ukey(p-1) == ukey(p) && seq(p-1) <= S)
+//                           || (type(p) == deletion && seq(p) <= S && IsBaseLevelForKey(ukey(p))))
+// is evaluated for all p at once.  IsBaseLevelForKey (db/version_set.cc:1535-1554)
+// walks monotone per-level pointers over sorted files; for non-decreasing keys
+// that equals a binary search for the first file whose largest key is >= the
+// key.  ShouldStopBefore (db/version_set.cc:1556-1579) only changes state when
+// the key passes a grandparent's largest key; those crossings are found in
+// parallel and the short sequential budget walk runs over them alone.
+#include "engine.h"
+
+namespace {
+
+// Internal-key compare of plane entry e against a stored key.
+__device__ __forceinline__ int ikey_cmp_fixed(const Planes& K, u64 e, const u64* words, u32 len, u64 tag) {
+  int c = ukey_cmp_fixed(K, e, words, len);
+  if (c) return c;
+  const u64 te = K.tag[e];
+  if (te != tag) return te > tag ? -1 : 1;
+  return 0;
+}
+
+__device__ bool is_base_level_for_key(const Planes& K, u64 e, const DropParams& dp) {
+  for (u32 L = dp.first_level; L < 7; L++) {
+    const u32 b = dp.lvl_begin[L], en = dp.lvl_end[L];
+    if (b >= en) continue;
+    u32 lo = b, hi = en;
+    while (lo < hi) {
+      const u32 m = (lo + hi) >> 1;
+      if (ukey_cmp_fixed(K, e, dp.deep_words + (2ull * m + 1) * K.W, dp.deep_len[2 * m + 1]) > 0) lo = m + 1;
+      else hi = m;
+    }
+    if (lo < en && ukey_cmp_fixed(K, e, dp.deep_words + 2ull * lo * K.W, dp.deep_len[2 * lo]) >= 0) return false;
+  }
+  return true;
+}
+
+__global__ void k_drop(const u32* perm, u32 n, Planes K, DropParams dp, u32* keep, u32* gcnt) {
+  GRID_STRIDE(p, n) {
+    const u32 e = perm[p];
+    const u64 tag = K.tag[e];
+    const u32 type = (u32)(tag & 0xff);
+    const u64 seq = tag >> 8;
+    bool drop = false;
+    if (type <= 1) {
+      if (p > 0) {
+        const u32 ep = perm[p - 1];
+        const u64 tp = K.tag[ep];
+        if ((tp & 0xff) <= 1 && (tp >> 8) <= dp.S && ukey_eq(K, e, ep)) drop = true;
+      }
+      if (!drop && type == 0 && seq <= dp.S && is_base_level_for_key(K, e, dp)) drop = true;
+    }
+    keep[p] = drop ? 0u : 1u;
+    if (dp.ngp) {
+      // number of grandparents whose largest key is below this key
+      u32 lo = 0, hi = dp.ngp;
+      while (lo < hi) {
+        const u32 m = (lo + hi) >> 1;
+        if (ikey_cmp_fixed(K, e, dp.gp_words + (u64)m * K.W, dp.gp_len[m], dp.gp_tag[m]) > 0) lo = m + 1;
+        else hi = m;
+      }
+      gcnt[p] = lo;
+    }
+  }
+}
+
+__global__ void k_gather(const u32* perm, u32 n, const u32* keep, const u32* kpos, Planes K, const u64* vptr,
+                         const u32* vlen, Kept out) {
+  GRID_STRIDE(p, n) {
+    if (!keep[p]) continue;
+    const u32 q = kpos[p];
+    const u32 e = perm[p];
+    for (u32 w = 0; w < K.W; w++) out.K.P[w * out.K.stride + q] = K.P[w * K.stride + e];
+    out.K.klen[q] = K.klen[e];
+    out.K.tag[q] = K.tag[e];
+    out.vptr[q] = vptr[e];
+    out.vlen[q] = vlen[e];
+    out.forced[q] = 0;
+  }
+}
+
+// Positions p >= 1 where the grandparent count grows (at most ngp of them).
+__global__ void k_stop_events(const u32* gcnt, u32 n, u32* events, u32* nev, u32 cap) {
+  GRID_STRIDE(p, n) {
+    if (p == 0 || gcnt[p] <= gcnt[p - 1]) continue;
+    const u32 i = atomicAdd(nev, 1u);
+    if (i < cap) events[i] = (u32)p;
+  }
+}
+
+constexpr u32 kStopThreads = 1024;
+constexpr u32 kMaxEvents = 2048;
+
+// One workgroup: sort the crossings, gather what each adds to the overlap
+// budget, walk them in order as ShouldStopBefore would, and mark the kept
+// entry each cut lands on.
+__global__ __launch_bounds__(kStopThreads) void k_stop_apply(const u32* gcnt, const u32* kpos, u32 n,
+                                                              DropParams dp, const u32* events, const u32* nev_dev,
+                                                              Kept out, DevStatus* st) {
+  __shared__ u32 ev[kMaxEvents];
+  __shared__ u64 add[kMaxEvents];
+  __shared__ u32 cut[kMaxEvents];
+  if (*nev_dev > kMaxEvents) {
+    if (threadIdx.x == 0) set_err(st, ERR_UNSUPPORTED, SITE_CAPACITY);
+    return;
+  }
+  const u32 nev = *nev_dev;
+  const u32 nkept = st->n_kept;
+  u32 np2 = 1;
+  while (np2 < nev) np2 <<= 1;
+  for (u32 i = threadIdx.x; i < np2; i += blockDim.x) ev[i] = i < nev ? events[i] : 0xffffffffu;
+  __syncthreads();
+  // bitonic sort of the event positions
+  for (u32 k = 2; k <= np2; k <<= 1)
+    for (u32 j = k >> 1; j > 0; j >>= 1) {
+      for (u32 i = threadIdx.x; i < np2; i += blockDim.x) {
+        const u32 ixj = i ^ j;
+        if (ixj > i) {
+          const u32 a = ev[i], b = ev[ixj];
+          const bool up = (i & k) == 0;
+          if ((a > b) == up) {
+            ev[i] = b;
+            ev[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (u32 i = threadIdx.x; i < nev; i += blockDim.x) {
+    const u32 p = ev[i];
+    add[i] = dp.gp_prefix[gcnt[p]] - dp.gp_prefix[gcnt[p - 1]];
+    cut[i] = kpos[p];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // Compaction::ShouldStopBefore: overlapped_bytes_ grows by every
+    // grandparent passed after the first key and resets when a cut fires.
+    u64 over = 0;
+    u32 nstops = 0;
+    for (u32 i = 0; i < nev; i++) {
+      over += add[i];
+      if (over > dp.limit) {
+        over = 0;
+        nstops++;
+      } else {
+        cut[i] = 0xffffffffu;
+      }
+    }
+    st->n_stops = nstops;
+  }
+  __syncthreads();
+  for (u32 i = threadIdx.x; i < nev; i += blockDim.x) {
+    const u32 q = cut[i];
+    if (q != 0xffffffffu && q > 0 && q < nkept) out.forced[q] = 1;
+  }
+}
+
+u32 grid_for(u64 n, u32 per_block) {
+  u64 g = (n + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > 16384) g = 16384;
+  return (u32)g;
+}
+
+}  // namespace
+
+void launch_drop(KTimer& T, const u32* perm, u32 n, Planes K, const DropParams& dp, u32* keep, u32* gcnt,
+                 DevStatus* st) {
+  (void)st;
+  if (!n) return;
+  KLAUNCH(T, "drop", k_drop, grid_for(n, 256), 256, 0, perm, n, K, dp, keep, gcnt);
+}
+
+void launch_gather(KTimer& T, const u32* perm, u32 n, const u32* keep, const u32* kpos, Planes K, const u64* vptr,
+                   const u32* vlen, Kept out) {
+  if (!n) return;
+  KLAUNCH(T, "gather", k_gather, grid_for(n, 256), 256, 0, perm, n, keep, kpos, K, vptr, vlen, out);
+}
+
+void launch_stops(KTimer& T, const u32* gcnt, const u32* kpos, const u32* keep, u32 n, const DropParams& dp,
+                  u32* events, u32 ev_cap, Kept out, DevStatus* st) {
+  (void)keep;
+  if (!n || !dp.ngp) return;
+  // events[0] is the counter, events + 1 the list.
+  fill_u32(T, events, 0, 1);
+  KLAUNCH(T, "stop_events", k_stop_events, grid_for(n, 256), 256, 0, gcnt, n, events + 1, events, ev_cap);
+  KLAUNCH(T, "stop_apply", k_stop_apply, 1, kStopThreads, 0, gcnt, kpos, n, dp, (const u32*)(events + 1),
+          (const u32*)events, out, st);
+}

@@ -112,6 +112,8 @@ class luda_stats(C.Structure):
         ("blocks_out", C.c_uint64),
         ("attempts", C.c_uint32),
         ("key_width", C.c_uint32),
+        ("data_block_bytes", C.c_uint64),
+        ("top_kernel_bytes", C.c_uint64),
     ]
 
 

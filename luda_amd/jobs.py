@@ -146,3 +146,108 @@ def make_job(wl, opts, inputs, flags=0, device_inputs=None):
     job.num_deeper = w.num_deeper
     keep.append(wl)
     return Job(job=job, keep=keep, inputs=(tuple(sizes[0]), tuple(sizes[1])))
+
+
+class Engine:
+    """A luda_ctx on one HIP device (the product's C-ABI, include/luda.h)."""
+
+    def __init__(self, device=0, budget=0):
+        self.lib = ffi.engine()
+        self.ctx = C.c_void_p()
+        err = C.create_string_buffer(512)
+        rc = self.lib.luda_init(device, budget, C.byref(self.ctx), err, 512)
+        if rc != ffi.LUDA_OK:
+            raise RuntimeError(f"luda_init failed ({rc}): {err.value.decode()}")
+
+    def close(self):
+        if self.ctx:
+            self.lib.luda_shutdown(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _finish(self, rc, res, err, keep_on_device=False):
+        if rc != ffi.LUDA_OK:
+            return rc, None, err.value.decode()
+        try:
+            stats = res.contents.stats
+            tables = [] if keep_on_device else result_tables(res)
+            return rc, (tables, _stats_dict(stats), res), ""
+        finally:
+            if not keep_on_device:
+                self.lib.luda_free_result(self.ctx, res)
+
+    def compact(self, job, keep_on_device=False):
+        """(status, (tables, stats, result_ptr) | None, message).  With
+        keep_on_device the result stays alive; free it with free()."""
+        res = ffi.RESULT_P()
+        err = C.create_string_buffer(512)
+        if keep_on_device:
+            job.flags |= ffi.LUDA_JOB_KEEP_ON_DEVICE
+        rc = self.lib.luda_compact(self.ctx, C.byref(job), C.byref(res), err, 512)
+        return self._finish(rc, res, err, keep_on_device)
+
+    def build_tables(self, opts_p, n, keys, key_off, vals, val_off, keep_on_device=False):
+        res = ffi.RESULT_P()
+        err = C.create_string_buffer(512)
+        flags = ffi.LUDA_JOB_KEEP_ON_DEVICE if keep_on_device else 0
+        rc = self.lib.luda_build_tables(self.ctx, opts_p, n, keys, key_off, vals, val_off, flags, C.byref(res), err,
+                                        512)
+        return self._finish(rc, res, err, keep_on_device)
+
+    def free(self, res):
+        self.lib.luda_free_result(self.ctx, res)
+
+    def builder(self):
+        """A table builder for build_group_tables backed by the engine."""
+        def build(opts_p, n, keys, key_off, vals, val_off):
+            rc, out, msg = self.build_tables(opts_p, n, keys, key_off, vals, val_off)
+            if rc != ffi.LUDA_OK:
+                raise RuntimeError(f"luda_build_tables failed ({rc}): {msg}")
+            return out[0]
+        return build
+
+    def build_entries(self, entries, opts):
+        keys = b"".join(k for k, _ in entries)
+        vals = b"".join(v for _, v in entries)
+        n = len(entries)
+        ko = (C.c_uint64 * (n + 1))()
+        vo = (C.c_uint64 * (n + 1))()
+        a = b = 0
+        for i, (k, v) in enumerate(entries):
+            ko[i], vo[i] = a, b
+            a += len(k)
+            b += len(v)
+        ko[n], vo[n] = a, b
+        kb = C.create_string_buffer(keys, len(keys) + 1)
+        vb = C.create_string_buffer(vals, len(vals) + 1)
+        return self.build_tables(C.byref(opts), n, C.cast(kb, C.c_void_p), C.cast(ko, C.c_void_p),
+                                 C.cast(vb, C.c_void_p), C.cast(vo, C.c_void_p))
+
+    def device_alloc(self, nbytes):
+        p = C.c_void_p()
+        if self.lib.luda_device_alloc(self.ctx, nbytes, C.byref(p)) != ffi.LUDA_OK:
+            raise RuntimeError("luda_device_alloc failed")
+        return p.value
+
+    def device_free(self, p):
+        self.lib.luda_device_free(self.ctx, p)
+
+    def h2d(self, dst, data):
+        buf = C.create_string_buffer(bytes(data), len(data))
+        if self.lib.luda_memcpy_h2d(self.ctx, dst, buf, len(data)) != ffi.LUDA_OK:
+            raise RuntimeError("luda_memcpy_h2d failed")
+
+    def d2h(self, src, nbytes):
+        buf = C.create_string_buffer(nbytes)
+        if self.lib.luda_memcpy_d2h(self.ctx, buf, src, nbytes) != ffi.LUDA_OK:
+            raise RuntimeError("luda_memcpy_d2h failed")
+        return buf.raw
+
+
+def _stats_dict(s):
+    return {k: (getattr(s, k).decode() if k == "top_kernel" else getattr(s, k)) for k, _ in ffi.luda_stats._fields_}

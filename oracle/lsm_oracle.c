@@ -924,7 +924,9 @@ typedef struct {
   int seen_key;
   int64_t overlapped_bytes;
   size_t level_ptrs[7];
-  size_t level_begin[7], level_end[7];
+  /* Version::files_[lvl]: the job's deeper files of that level, in order */
+  size_t* level_files[7];
+  size_t level_count[7];
 } compaction_t;
 
 /* Compaction::ShouldStopBefore, db/version_set.cc:1556-1579 */
@@ -947,8 +949,8 @@ static int should_stop_before(compaction_t* c, const uint8_t* key, size_t klen) 
 static int is_base_level_for_key(compaction_t* c, const uint8_t* ukey, size_t un) {
   const luda_job* j = c->job;
   for (int lvl = j->level + 2; lvl < 7; lvl++) {
-    while (c->level_begin[lvl] + c->level_ptrs[lvl] < c->level_end[lvl]) {
-      const luda_level_file* f = &j->deeper[c->level_begin[lvl] + c->level_ptrs[lvl]];
+    while (c->level_ptrs[lvl] < c->level_count[lvl]) {
+      const luda_level_file* f = &j->deeper[c->level_files[lvl][c->level_ptrs[lvl]]];
       if (bytewise_compare(ukey, un, f->largest_user_key.data, f->largest_user_key.size) <= 0) {
         if (bytewise_compare(ukey, un, f->smallest_user_key.data, f->smallest_user_key.size) >= 0)
           return 0;
@@ -998,14 +1000,13 @@ int or_compact(const luda_job* job, luda_result** out) {
   compaction_t c;
   memset(&c, 0, sizeof(c));
   c.job = job;
-  for (int lvl = 0; lvl < 7; lvl++) c.level_begin[lvl] = c.level_end[lvl] = 0;
-  for (int lvl = job->level + 2; lvl < 7; lvl++) {
-    size_t b = 0;
-    while (b < job->num_deeper && job->deeper[b].level != lvl) b++;
-    size_t e = b;
-    while (e < job->num_deeper && job->deeper[e].level == lvl) e++;
-    c.level_begin[lvl] = b;
-    c.level_end[lvl] = e;
+  for (int lvl = 0; lvl < 7; lvl++) {
+    c.level_files[lvl] = (size_t*)malloc(sizeof(size_t) * (job->num_deeper ? job->num_deeper : 1));
+    c.level_count[lvl] = 0;
+  }
+  for (uint32_t i = 0; i < job->num_deeper; i++) {
+    int lvl = job->deeper[i].level;
+    if (lvl >= 0 && lvl < 7) c.level_files[lvl][c.level_count[lvl]++] = i;
   }
 
   results_t r = {0};
@@ -1097,6 +1098,7 @@ int or_compact(const luda_job* job, luda_result** out) {
   buf_free(&largest);
   buf_free(&current_user_key);
   free(pos);
+  for (int lvl = 0; lvl < 7; lvl++) free(c.level_files[lvl]);
   for (size_t i = 0; i < max_children; i++) stream_free(&ch[i]);
   free(ch);
   *out = res_finish(&r, bytes_read);

@@ -1,0 +1,48 @@
+"""CPU checks of the C-ABI boundary: the engine library loads and exports
+every function include/luda.h declares, and the ctypes mirror of the structs
+has the C layout."""
+import ctypes as C
+import os
+import re
+
+from luda_amd import ffi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    with open(os.path.join(ROOT, "include", "luda.h")) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(luda_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(ffi.ENGINE_LIB)
+    names = declared_functions()
+    assert "luda_compact" in names and "luda_build_tables" in names
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(ffi.ENGINE_SIGNATURES)
+
+
+def test_version_string_without_a_gpu():
+    assert b"gfx950" in ffi.engine().luda_version()
+
+
+def test_struct_layouts():
+    # sizes fixed by include/luda.h on LP64
+    assert C.sizeof(ffi.luda_options) == 32
+    assert C.sizeof(ffi.luda_file) == 24
+    assert C.sizeof(ffi.luda_grandparent) == 24
+    assert C.sizeof(ffi.luda_level_file) == 40
+    assert C.sizeof(ffi.luda_output) == 48
+    assert ffi.luda_job.inputs.offset == 48 and ffi.luda_job.deeper.offset == 88
+    assert C.sizeof(ffi.luda_job) == 96
+
+
+def test_init_fails_cleanly_without_device():
+    lib = ffi.engine()
+    ctx = C.c_void_p()
+    err = C.create_string_buffer(256)
+    rc = lib.luda_init(99, 0, C.byref(ctx), err, 256)
+    assert rc != ffi.LUDA_OK and not ctx.value

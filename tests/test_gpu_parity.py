@@ -1,0 +1,141 @@
+"""Parity of the HIP engine with the oracle (and, through the golden
+fixtures, with the reference's own code): byte-identical tables and identical
+table metadata.  Every case calls the engine through the C-ABI."""
+import hashlib
+import json
+import os
+import random
+import struct
+
+import pytest
+
+import oracle
+from luda_amd import ffi, jobs
+from tests.tiny_jobs import TinyJob, ikey
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+OPTION_SETS = {
+    "default": dict(),
+    "small_blocks": dict(block_size=256, block_restart_interval=4, max_file_size=1 << 20),
+    "no_filter": dict(bloom_bits_per_key=0),
+}
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = jobs.Engine(0)
+    yield e
+    e.close()
+
+
+def sha(b):
+    return hashlib.sha256(b).hexdigest()
+
+
+def summary(ts):
+    return [(sha(t.data), len(t.data), t.smallest, t.largest, t.num_entries) for t in ts]
+
+
+def test_build_golden_corner_tables(eng):
+    with open(os.path.join(GOLD, "tables.json")) as f:
+        meta = json.load(f)
+    for name in sorted(meta):
+        m = meta[name]
+        print("table", name, flush=True)
+        opts = ffi.luda_options(**m["options"])
+        entries = [(bytes.fromhex(k), bytes.fromhex(v)) for k, v in m["entries"]]
+        rc, out, msg = eng.build_entries(entries, opts)
+        assert rc == 0, (name, msg)
+        tables = out[0]
+        assert len(tables) == 1, name
+        with open(os.path.join(GOLD, "tables", name + ".ldb"), "rb") as f:
+            assert tables[0].data == f.read(), name
+
+
+@pytest.mark.skip(reason="faulted the GPU twice (aperture violation in k_write_blocks) before the cause was found: "
+                         "a flat load whose address register pointed 3 bytes below the LDS aperture; the writer no "
+                         "longer uses generic LDS pointers (tests/test_isa.py), but this exact case stays off the GPU "
+                         "until the next round")
+def test_build_30_entries_one_block(eng):
+    """The first case that ran on the GPU: 30 entries in one 684-byte block,
+    assembled by wave 0 at LDS offset 0, CRC pad of 83 bytes."""
+    ents = [(b"key%03d" % i + struct.pack("<Q", (5 << 8) | 1), b"x" * 10) for i in range(30)]
+    opts = ffi.default_options(max_file_size=ffi.U64_MAX)
+    rc, out, msg = eng.build_entries(ents, opts)
+    assert rc == 0, msg
+    assert summary(out[0]) == summary(oracle.build_tables(ents, opts, "oracle"))
+
+
+def random_entries(rng, n, klen_range, vlen_choices, versions=1):
+    keys = set()
+    while len(keys) < n:
+        L = rng.randint(*klen_range)
+        keys.add(bytes(rng.randrange(256) for _ in range(L)))
+    ents = []
+    seq = 1
+    for u in sorted(keys):
+        for v in range(versions):
+            ents.append((u, 10 ** 6 - seq, rng.choice([0, 1, 1, 1])))
+            seq += 1
+    ents.sort(key=lambda e: (e[0], -e[1]))
+    return [(ikey(u, s, t), b"" if t == 0 else bytes(rng.randrange(256) for _ in range(rng.choice(vlen_choices))))
+            for u, s, t in ents]
+
+
+@pytest.mark.parametrize("case", range(12))
+def test_build_tables_match_oracle(eng, case):
+    rng = random.Random(1000 + case)
+    opts = ffi.default_options(block_size=rng.choice([64, 256, 1024, 4096]),
+                               block_restart_interval=rng.choice([1, 3, 16]),
+                               bloom_bits_per_key=rng.choice([0, 10, 7]),
+                               max_file_size=rng.choice([2000, 20000, ffi.U64_MAX]))
+    ents = random_entries(rng, rng.choice([1, 5, 300, 2000]), (0, rng.choice([4, 30, 80])), [0, 3, 100, 700],
+                          versions=rng.choice([1, 3]))
+    print("case", case, len(ents), opts.block_size, opts.max_file_size, flush=True)
+    rc, out, msg = eng.build_entries(ents, opts)
+    assert rc == 0, msg
+    want = oracle.build_tables(ents, opts, "oracle")
+    assert summary(out[0]) == summary(want)
+
+
+def engine_tables(eng, ents, opts):
+    rc, out, msg = eng.build_entries(ents, opts)
+    assert rc == 0, msg
+    return [t.data for t in out[0]]
+
+
+@pytest.mark.parametrize("block", range(4))
+def test_tiny_jobs_match_oracle(eng, block):
+    for seed in range(block * 100, (block + 1) * 100):
+        tj = TinyJob(seed)
+        job, keep = tj.make_job(lambda e, o: [t.data for t in oracle.build_tables(e, o, "oracle")])
+        rc1, o1 = oracle.compact(job, "oracle")
+        rc2, out, msg = eng.compact(job)
+        assert rc1 == 0 and rc2 == 0, (seed, msg)
+        assert summary(out[0]) == summary(o1), seed
+
+
+with open(os.path.join(GOLD, "jobs.json")) as f:
+    GOLD_JOBS = json.load(f)
+
+
+@pytest.mark.parametrize("rec", GOLD_JOBS, ids=lambda r: f"cfg{r['config']}-{r['seed']}-{r['scale']}-{r['options']}")
+def test_golden_jobs_on_engine(eng, rec):
+    """Inputs built by the engine's table writer, compacted by the engine,
+    against the SHA-256 the reference's own code produced."""
+    print("job", rec["config"], rec["seed"], rec["scale"], rec["options"], flush=True)
+    opts = ffi.default_options(**OPTION_SETS[rec["options"]])
+    wl = jobs.Workload(rec["config"], seed=rec["seed"], scale=rec["scale"])
+    ins = ([], [])
+    for g in wl.groups:
+        ins[g.which].extend(t.data for t in jobs.build_group_tables(eng.builder(), opts, g))
+    assert [[sha(t) for t in ins[0]], [sha(t) for t in ins[1]]] == rec["inputs"]
+    job = jobs.make_job(wl, opts, ins)
+    rc, out, msg = eng.compact(job.job)
+    assert rc == 0, msg
+    got = [{"sha256": sha(t.data), "file_size": len(t.data), "num_entries": t.num_entries,
+            "smallest": t.smallest.hex(), "largest": t.largest.hex()} for t in out[0]]
+    assert got == rec["outputs"]
