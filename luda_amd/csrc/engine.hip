@@ -144,8 +144,8 @@ int ikey_cmp_host(const u8* a, u64 an, const u8* b, u64 bn) {
   return ta > tb ? -1 : (ta < tb ? 1 : 0);
 }
 
-u32 next_pow2(u64 x) {
-  u32 p = 1;
+u64 next_pow2(u64 x) {
+  u64 p = 1;
   while (p < x) p <<= 1;
   return p;
 }
@@ -375,6 +375,16 @@ void run_encode(luda_ctx* c, const EncodeSetup& s, const EncodeArrays& A, u8* ou
   launch_write_tail(T, k, nf_dev, s.nf_cap, s.ep, b, out, s.out_cap, st);
 }
 
+// Orbit chunk for successor steps of at most `step` nodes: a chunk must hold
+// one whole step (an orbit cannot jump over a chunk); twice that when it fits
+// keeps the per-chunk functions small relative to the chunk.
+u32 orbit_chunk(u64 step, u32 min_chunk) {
+  const u64 two = next_pow2(2 * step);
+  if (two <= kMaxChunk) return std::max<u32>(min_chunk, (u32)two);
+  const u64 one = next_pow2(step);
+  return one > kMaxChunk ? (u32)std::min<u64>(one, 1u << 31) : std::max<u32>(min_chunk, (u32)one);
+}
+
 // Capacities and orbit plans for an encoder run over at most m entries.
 EncodeSetup plan_encode(const luda_options& o, u32 W, u64 m, u64 out_cap, u64 extra_files, u64 nf_cap_min,
                         u64 nw_cap_min) {
@@ -400,14 +410,14 @@ EncodeSetup plan_encode(const luda_options& o, u32 W, u64 m, u64 out_cap, u64 ex
   s.nw_cap = (u32)std::max<u64>(nw, nw_cap_min);
   // blocks: an entry is at least 4 bytes and 16 of them add a 4-byte restart
   const u64 sb = o.block_size / 4 + 2;
-  const u32 Cb = std::max<u32>(2048, next_pow2(2 * sb));
+  const u32 Cb = orbit_chunk(sb, 2048);
   const u32 Wb = (u32)std::min<u64>(sb, Cb);
   s.ob = plan_orbit(s.m_cap, Cb, Wb, std::max<u32>(2, std::min<u32>(32, 16384 / Wb)));
   if (Cb > kMaxChunk) throw Fail{LUDA_NOT_SUPPORTED, "block_size too large for the device block cut"};
   if (s.size_cuts) {
     // tables: every block but a table's last is at least block_size long
     const u64 sf = o.max_file_size / (o.block_size + 5) + 2;
-    const u32 Cf = std::max<u32>(1024, next_pow2(2 * std::min<u64>(sf, s.nbo_cap)));
+    const u32 Cf = orbit_chunk(std::min<u64>(sf, s.nbo_cap), 1024);
     if (Cf > kMaxChunk) throw Fail{LUDA_NOT_SUPPORTED, "too many blocks per table for the device table cut"};
     const u32 Wf = (u32)std::min<u64>(sf, Cf);
     const u32 Gf = std::max<u32>(2, std::min<u32>(32, 16384 / Wf));
