@@ -49,7 +49,7 @@ struct DevStatus {
   u32 site;
   u32 max_ulen;       // longest user key seen while decoding
   u32 max_rulen;      // longest user key at a restart point (first guess of the key width)
-  u32 pad0;
+  u32 max_blk;        // largest input data block, bytes
   u32 n_blocks_in;    // data blocks of all inputs
   u32 n_entries;      // entries of all inputs
   u32 n_kept;         // entries that survive the drop rule
@@ -62,7 +62,7 @@ struct DevStatus {
   u32 n_forced;       // output blocks that start a table because of such a cut
   u32 need_files;     // tables, when over the host's capacity
   u32 need_windows;   // filter windows, when over the host's capacity
-  u32 pad1;
+  u32 max_nr;         // most restart points in one input data block
   u64 out_bytes;      // sum of output table sizes
   u64 data_bytes;     // output data blocks with trailers
   u64 keys_needed;    // bytes of smallest/largest keys

@@ -61,10 +61,17 @@ struct KTimer {
   }
 };
 
+// Throws the engine's error for a launch the runtime refused (bad shape or
+// LDS size): nothing after it is enqueued, so no later kernel reads buffers
+// the refused one should have written.
+[[noreturn]] void launch_refused(const char* name, hipError_t e);
+
 #define KLAUNCH(T, name, kern, grid, block, lds, ...)                                     \
   do {                                                                                    \
     (T).begin(name);                                                                      \
     hipLaunchKernelGGL(kern, dim3(grid), dim3(block), (lds), (T).s, __VA_ARGS__);       \
+    const hipError_t klaunch_e = hipGetLastError();                                       \
+    if (klaunch_e != hipSuccess) launch_refused(name, klaunch_e);                         \
     (T).end();                                                                            \
   } while (0)
 
@@ -203,7 +210,7 @@ void launch_blkscan(KTimer& T, const FileDesc* files, const BlockDesc* blks, con
 void launch_file_ebase(KTimer& T, const u32* blk_base, const u32* blk_ebase, u32 nfiles, const u32* n_entries,
                        u32* file_ebase);
 void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 nb, const u32* blk_ebase, Planes K,
-                   u64* vptr, u32* vlen, DevStatus* st);
+                   u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr);
 void launch_ingest(KTimer& T, u64 n, const u8* keys, const u64* key_off, const u8* vals, const u64* val_off, Kept out,
                    DevStatus* st);
 

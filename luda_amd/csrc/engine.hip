@@ -220,6 +220,14 @@ struct Fail {
     if (e__ != hipSuccess) throw Fail{LUDA_IO_ERROR, std::string("HIP: ") + #x + ": " + hipGetErrorString(e__)}; \
   } while (0)
 
+}  // namespace
+
+void launch_refused(const char* name, hipError_t e) {
+  throw Fail{LUDA_IO_ERROR, std::string("HIP refused the launch of ") + name + ": " + hipGetErrorString(e)};
+}
+
+namespace {
+
 int status_code_of(const DevStatus& st) {
   if (st.err & ERR_CORRUPT) return LUDA_CORRUPTION;
   if (st.err & ERR_UNSUPPORTED) return LUDA_NOT_SUPPORTED;
@@ -863,7 +871,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
       sb.max_ulen = 0;
       fill_status(c, sb);
       // decode, check, merge, drop, gather, stops
-      launch_decode(T, d_files, d_blks, nb, d_ebase, K, vptr, vlen, c->st);
+      launch_decode(T, d_files, d_blks, nb, d_ebase, K, vptr, vlen, c->st, stA.max_blk, stA.max_nr);
       launch_check_sorted(T, K, n, d_cstart, (u32)cstart.size() - 1, c->st);
       const u32* perm = permA;
       iota_u32(T, permA, n);

@@ -9,6 +9,8 @@
 // (offset << 8) | entries_in_group (table/block_builder.cc:62-74), which gives
 // every restart group's entry count without parsing: that is what lets one
 // lane decode one group with its output position known up front.
+#include <algorithm>
+
 #include "engine.h"
 
 namespace {
@@ -124,7 +126,7 @@ __global__ void k_index(const FileDesc* files, const FileInfo* info, const u32* 
 __global__ void k_blkscan(const FileDesc* files, const BlockDesc* blks, const u32* nb_dev, u32* blk_nent,
                           DevStatus* st) {
   const u32 nb = *nb_dev;
-  u32 maxr = 0;
+  u32 maxr = 0, maxn = 0, maxg = 0;
   GRID_STRIDE(b, nb) {
     blk_nent[b] = 0;
     const BlockDesc bd = blks[b];
@@ -143,6 +145,9 @@ __global__ void k_blkscan(const FileDesc* files, const BlockDesc* blks, const u3
       set_err(st, ERR_CORRUPT, SITE_BLOCK_TRAILER);
       continue;
     }
+    // sizes of the decoder's LDS: every block that it will look at
+    maxn = max(maxn, n);
+    maxg = max(maxg, nr);
     if (nr == 0) continue;  // Block::NewIterator -> NewEmptyIterator
     const u32 roff = n - 4 - 4 * nr;
     u32 total = 0, prev = 0;
@@ -169,6 +174,8 @@ __global__ void k_blkscan(const FileDesc* files, const BlockDesc* blks, const u3
     blk_nent[b] = total;
   }
   if (maxr) atomicMax(&st->max_rulen, maxr);
+  if (maxn) atomicMax(&st->max_blk, maxn);
+  if (maxg) atomicMax(&st->max_nr, maxg);
 }
 
 __global__ void k_file_ebase(const u32* blk_base, const u32* blk_ebase, u32 nfiles, const u32* n_entries,
@@ -298,12 +305,244 @@ __device__ __forceinline__ void decode_block(const Src& src, u32 n, u64 gp, u64 
   }
 }
 
-// One wavefront per data block.  The block is staged in LDS with 16-byte
-// loads; lane g then walks restart group g (entries known from the count
-// byte, output position from a wave prefix sum of the counts), rebuilding
-// each prefix-compressed key in its own LDS buffer and writing its planes.
+// ---------------------------------------------------------------------------
+// Entry-parallel decoder.
+//
+// One wavefront per data block, one lane per entry.  Lane e finds its
+// restart group from the groups' entry prefix (LDS), parses the headers from
+// the group's restart point up to its own entry, and loads the bytes its
+// entry stores (key bytes [shared, shared + non_shared)) into NW 64-bit
+// big-endian words with a byte mask.  The full key of every entry then comes
+// from one inclusive scan across the wave with the associative operator
+//   (m1, v1) . (m2, v2) = (m1 | m2, (v1 & ~m2) | v2),
+// i.e. every byte takes the value of the latest entry that stored it.  That
+// is exactly BlockBuilder's prefix compression read back: byte j of key i is
+// key i-1's byte j when j < shared_i, so it comes from the latest entry k <= i
+// with shared_k <= j (table/block.cc:224-251).  Chunks of 64 entries carry
+// the last lane's words into the next chunk.
+
+// Byte windows over one data block: `win8(x)` is bytes [x, x+8) as a
+// big-endian word (bytes past the block are don't-cares).
+struct LdsWin {
+  const u64* w64;  // the LDS stage as 8-byte words
+  u32 mis;         // offset of the block's first byte in the stage
+  __device__ __forceinline__ u64 win8(u32 x) const {
+    const u32 o = x + mis, a = o >> 3, sft = (o & 7) * 8;
+    const u64 lo = __builtin_bswap64(w64[a]);
+    const u64 hi = __builtin_bswap64(w64[a + 1]);
+    return sft ? (lo << sft) | (hi >> (64 - sft)) : lo;
+  }
+  __device__ __forceinline__ u32 at(u32 x) const { return (u32)(win8(x) >> 56); }
+};
+struct GlobalWin {
+  const u8* p;
+  u32 n;  // block bytes: nothing at or past n is read
+  __device__ __forceinline__ u64 win8(u32 x) const {
+    u64 v = 0;
+    for (u32 t = 0; t < 8; t++) v = (v << 8) | (x + t < n ? p[x + t] : 0u);
+    return v;
+  }
+  __device__ __forceinline__ u32 at(u32 x) const { return x < n ? p[x] : 0u; }
+};
+template <typename Src>
+__device__ __forceinline__ u32 win_ld32(const Src& s, u32 x) {
+  return __builtin_bswap32((u32)(s.win8(x) >> 32));
+}
+// DecodeEntry (table/block.cc:54-74) on [off, lim): false if malformed.
+template <typename Src>
+__device__ __forceinline__ bool win_entry_header(const Src& s, u32& off, u32 lim, u32& sh, u32& ns, u32& vl) {
+  if (off > lim || lim - off < 3) return false;
+  const u64 w = s.win8(off);
+  const u32 b0 = (u32)(w >> 56), b1 = (u32)(w >> 48) & 0xff, b2 = (u32)(w >> 40) & 0xff;
+  if (((b0 | b1 | b2) & 0x80) == 0) {
+    sh = b0;
+    ns = b1;
+    vl = b2;
+    off += 3;
+  } else if (!src_varint32(s, off, lim, &sh) || !src_varint32(s, off, lim, &ns) ||
+             !src_varint32(s, off, lim, &vl)) {
+    return false;
+  }
+  return (u64)(lim - off) >= (u64)ns + vl;
+}
+
+template <u32 NWMAX, typename Src>
+__device__ __forceinline__ void decode_entries(const Src& src, u32 n, u64 gp, u64 ebase, u32* gpre, u32 max_nr,
+                                               const Planes& K, u64* vptr, u32* vlen, DevStatus* st, u32& maxu) {
+  const u32 lane = threadIdx.x & 63;
+  const u32 NW = K.W + 1;  // words of the internal key: user key words + the tag
+  const u32 nr = n >= 4 ? win_ld32(src, n - 4) : 0;
+  if (nr == 0 || nr > (n - 4) / 4) return;  // k_blkscan reported it
+  if (nr > max_nr) {  // not what k_blkscan saw: gpre holds max_nr + 1 entries
+    if (lane == 0) set_err(st, ERR_CORRUPT, SITE_BLOCK_TRAILER);
+    return;
+  }
+  const u32 roff = n - 4 - 4 * nr;
+  // entry prefix of the restart groups
+  u32 total = 0;
+  for (u32 g0 = 0; g0 < nr; g0 += 64) {
+    const u32 g = g0 + lane;
+    const u32 cnt = (g < nr && roff > 0) ? (win_ld32(src, roff + 4 * g) & 0xff) : 0;
+    const u32 incl = wave_incl_scan_u32(cnt);
+    if (g < nr) gpre[g] = total + incl - cnt;
+    total += __shfl(incl, 63, 64);
+  }
+  if (lane == 0) gpre[nr] = total;
+  __syncthreads();
+  u64 cm[NWMAX], cv[NWMAX];  // words of the entry before the chunk
+#pragma unroll
+  for (u32 w = 0; w < NWMAX; w++) cm[w] = cv[w] = 0;
+  u32 cilen = 0;
+  bool bad = false;
+  for (u32 e0 = 0; e0 < total; e0 += 64) {
+    const u32 e = e0 + lane;
+    const bool act = e < total;
+    u32 lo = 0, hi = nr;
+    while (hi - lo > 1) {
+      const u32 mid = (lo + hi) >> 1;
+      if (gpre[mid] <= e) lo = mid;
+      else hi = mid;
+    }
+    const u32 g = lo;
+    const u32 r = e - gpre[g], cnt = gpre[g + 1] - gpre[g];
+    const u32 start = win_ld32(src, roff + 4 * g) >> 8;
+    const u32 end = g + 1 < nr ? (win_ld32(src, roff + 4 * (g + 1)) >> 8) : roff;
+    const u32 lim = end <= roff ? end : roff;
+    bool ok = act && start < end && end <= roff;
+    u32 off = start, sh = 0, ns = 0, vl = 0;
+    for (u32 i = 0; ok && i < r; i++) {
+      ok = win_entry_header(src, off, lim, sh, ns, vl);
+      off += ns + vl;
+    }
+    if (ok) ok = win_entry_header(src, off, lim, sh, ns, vl);
+    const u32 koff = off, voff = off + ns;
+    const u32 ilen = sh + ns;
+    if (ok && r == 0 && sh != 0) ok = false;
+    if (ok && r + 1 == cnt && voff + vl != lim) ok = false;
+    // this entry's bytes, positioned in the words of the internal key
+    u64 m[NWMAX], v[NWMAX];
+#pragma unroll
+    for (u32 w = 0; w < NWMAX; w++) {
+      m[w] = v[w] = 0;
+      const u32 w0 = 8 * w;
+      const u32 a = sh > w0 ? sh : w0;
+      const u32 b = ilen < w0 + 8 ? ilen : w0 + 8;
+      if (ok && w < NW && a < b) {
+        const u32 lead = a - w0, tail = w0 + 8 - b;
+        const u64 mask = (~0ull >> (8 * lead)) & (~0ull << (8 * tail));
+        m[w] = mask;
+        v[w] = (src.win8(koff + (a - sh)) >> (8 * lead)) & mask;
+      }
+    }
+    // every byte from the latest entry that stored it
+#pragma unroll
+    for (u32 d = 1; d < 64; d <<= 1) {
+#pragma unroll
+      for (u32 w = 0; w < NWMAX; w++) {
+        const u64 om = __shfl_up(m[w], d, 64), ov = __shfl_up(v[w], d, 64);
+        if (lane >= d) {
+          v[w] = (ov & ~m[w]) | v[w];
+          m[w] |= om;
+        }
+      }
+    }
+#pragma unroll
+    for (u32 w = 0; w < NWMAX; w++) {
+      v[w] = (cv[w] & ~m[w]) | v[w];
+      m[w] |= cm[w];
+    }
+    // shared may not exceed the previous key (DecodeEntry's key_.size() >= shared)
+    const u32 up = __shfl_up(ilen, 1, 64);
+    const u32 prev_ilen = lane == 0 ? cilen : up;
+    if (ok && r > 0 && sh > prev_ilen) ok = false;
+    if (ok && ilen < 8) {
+      set_err(st, ERR_UNSUPPORTED, SITE_SHORT_KEY);
+      ok = false;
+    }
+    if (act) {
+      const u64 q = ebase + e;
+      if (ok && ilen <= 8 * NW) {
+        const u32 ulen = ilen - 8;
+        maxu = max(maxu, ulen);
+        const u32 wi = ulen >> 3, sft = (ulen & 7) * 8;
+        u64 x0 = 0, x1 = 0;
+#pragma unroll
+        for (u32 w = 0; w < NWMAX; w++) {
+          if (w == wi) x0 = v[w];
+          if (w == wi + 1) x1 = v[w];
+        }
+        const u64 be = sft ? (x0 << sft) | (x1 >> (64 - sft)) : x0;
+#pragma unroll
+        for (u32 w = 0; w + 1 < NWMAX; w++) {
+          if (w < K.W) {
+            const u32 nb = ulen > 8 * w ? min(ulen - 8 * w, 8u) : 0u;
+            const u64 keep = nb ? (~0ull << (64 - 8 * nb)) : 0ull;
+            K.P[w * K.stride + q] = v[w] & keep;
+          }
+        }
+        K.klen[q] = ulen;
+        K.tag[q] = __builtin_bswap64(be);
+      } else {
+        if (ok) set_err(st, ERR_OVERFLOW, SITE_KEY_WIDTH);  // the host retries with wider planes
+#pragma unroll
+        for (u32 w = 0; w + 1 < NWMAX; w++)
+          if (w < K.W) K.P[w * K.stride + q] = 0;
+        K.klen[q] = 8 * K.W;
+        K.tag[q] = 0;
+      }
+      vptr[q] = ok ? gp + voff : gp;
+      vlen[q] = ok ? vl : 0;
+    }
+    bad |= act && !ok;
+    // carry the chunk's last entry
+#pragma unroll
+    for (u32 w = 0; w < NWMAX; w++) {
+      cm[w] = __shfl(m[w], 63, 64);
+      cv[w] = __shfl(v[w], 63, 64);
+    }
+    cilen = __shfl(ilen, 63, 64);
+  }
+  if (bad) set_err(st, ERR_CORRUPT, SITE_GROUP_COUNT);
+  __syncthreads();
+}
+
+// One wavefront per data block (see decode_entries).  The block is staged in
+// LDS with 16-byte loads when it fits `stage` bytes, else read in place; the
+// restart groups' entry prefix follows the stage.
+template <u32 NWMAX>
+__global__ __launch_bounds__(64) void k_decode_entries(const FileDesc* files, const BlockDesc* blks, u32 nb,
+                                                       const u32* blk_ebase, Planes K, u64* vptr, u32* vlen,
+                                                       DevStatus* st, u32 stage, u32 max_nr) {
+  extern __shared__ uint4 smem[];
+  u32* gpre = reinterpret_cast<u32*>(smem + stage / 16);
+  const u32 lane = threadIdx.x;
+  u32 maxu = 0;
+  for (u32 b = blockIdx.x; b < nb; b += gridDim.x) {
+    const BlockDesc bd = blks[b];
+    const u8* gp = files[bd.file].base + bd.off;
+    const u32 n = bd.size;
+    const u64 ebase = blk_ebase[b];
+    const u64 a0 = (u64)gp & ~15ull;
+    const u64 a1 = ((u64)gp + n + 15) & ~15ull;
+    if (a1 - a0 + 16 <= stage) {
+      const uint4* g4 = reinterpret_cast<const uint4*>(a0);
+      for (u32 x = lane; x < (a1 - a0) / 16; x += 64) smem[x] = g4[x];
+      __syncthreads();
+      const LdsWin src{reinterpret_cast<const u64*>(smem), (u32)((u64)gp - a0)};
+      decode_entries<NWMAX>(src, n, (u64)gp, ebase, gpre, max_nr, K, vptr, vlen, st, maxu);
+    } else {
+      decode_entries<NWMAX>(GlobalWin{gp, n}, n, (u64)gp, ebase, gpre, max_nr, K, vptr, vlen, st, maxu);
+    }
+  }
+  if (maxu) atomicMax(&st->max_ulen, maxu);
+}
+
+// Decoder for key planes wider than k_decode_entries handles (W > 8): one
+// wavefront per data block, staged in LDS; lane g walks restart group g
+// (entries known from the count byte, output position from a wave prefix sum
+// of the counts), rebuilding each prefix-compressed key in its own LDS buffer.
 // A block too large for the stage is read in place.
-__global__ __launch_bounds__(64) void k_decode(const FileDesc* files, const BlockDesc* blks, u32 nb,
+__global__ __launch_bounds__(64) void k_decode_groups(const FileDesc* files, const BlockDesc* blks, u32 nb,
                                                const u32* blk_ebase, Planes K, u64* vptr, u32* vlen,
                                                DevStatus* st) {
   extern __shared__ uint4 smem[];
@@ -401,11 +640,28 @@ void launch_file_ebase(KTimer& T, const u32* blk_base, const u32* blk_ebase, u32
           file_ebase);
 }
 void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 nb, const u32* blk_ebase, Planes K,
-                   u64* vptr, u32* vlen, DevStatus* st) {
+                   u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr) {
   if (!nb) return;
+  const u32 grid = nb < 65536 ? nb : 65536;
+  if (K.W <= 8) {
+    // stage: the largest block, its misalignment, and the 16 bytes an 8-byte
+    // window may read past it; blocks over 16 KiB are read in place
+    u32 stage = (u32)((std::min<u64>(max_blk, 16384) + 15 + 16 + 15) & ~15ull);
+    const size_t lds = stage + 4ull * (max_nr + 1);
+    if (lds > 65536) launch_refused("decode (restart arrays too long for LDS)", hipErrorInvalidValue);
+    if (K.W <= 2)
+      KLAUNCH(T, "decode", k_decode_entries<3>, grid, 64, lds, files, blks, nb, blk_ebase, K, vptr, vlen, st, stage,
+              max_nr);
+    else if (K.W <= 4)
+      KLAUNCH(T, "decode", k_decode_entries<5>, grid, 64, lds, files, blks, nb, blk_ebase, K, vptr, vlen, st, stage,
+              max_nr);
+    else
+      KLAUNCH(T, "decode", k_decode_entries<9>, grid, 64, lds, files, blks, nb, blk_ebase, K, vptr, vlen, st, stage,
+              max_nr);
+    return;
+  }
   const size_t lds = kStage + 16 + 64 * (8 * K.W + 8);
-  u32 grid = nb < 65536 ? nb : 65536;
-  KLAUNCH(T, "decode", k_decode, grid, 64, lds, files, blks, nb, blk_ebase, K, vptr, vlen, st);
+  KLAUNCH(T, "decode", k_decode_groups, grid, 64, lds, files, blks, nb, blk_ebase, K, vptr, vlen, st);
 }
 void launch_ingest(KTimer& T, u64 n, const u8* keys, const u64* key_off, const u8* vals, const u64* val_off, Kept out,
                    DevStatus* st) {

@@ -378,6 +378,21 @@ __device__ __forceinline__ void load_crc_tables(u32 (*sl)[256], u32 (*z)[256]) {
 }
 
 
+// The internal key user_key || fixed64(tag) of entry q as NWMAX big-endian
+// words (bytes past the key are zero); the planes hold the user key's words.
+template <u32 NWMAX>
+__device__ __forceinline__ void ikey_words(const Planes& K, u64 q, u32 ulen, u64 tag, u64 (&ik)[NWMAX]) {
+  const u64 tb = __builtin_bswap64(tag);
+  const u32 wi = ulen >> 3, sft = (ulen & 7) * 8;
+#pragma unroll
+  for (u32 w = 0; w < NWMAX; w++) {
+    u64 x = w < K.W ? K.P[(u64)w * K.stride + q] : 0;
+    if (w == wi) x |= tb >> sft;
+    if (sft && w == wi + 1) x |= tb << (64 - sft);
+    ik[w] = x;
+  }
+}
+
 // The bytes of output block i (entries, restart array, type byte, masked CRC)
 // assembled at buf: BlockBuilder::Add/Finish (table/block_builder.cc:62-114)
 // with the fork's (offset << 8) | count restart words, and
@@ -385,7 +400,10 @@ __device__ __forceinline__ void load_crc_tables(u32 (*sl)[256], u32 (*z)[256]) {
 // either the wave's LDS stage or the block's place in the output; the two
 // are separate instantiations so that every access is a plain LDS (ds_*) or
 // global access and never a generic (flat) one that could address either.
-template <typename Buf>
+// One lane per entry: header, key delta from the key words in registers,
+// restart word, and the value copied with up to 16 loads in flight.
+// NWMAX = 0 reads key bytes from the planes one at a time (any key width).
+template <u32 NWMAX, typename Buf>
 __device__ __forceinline__ void assemble_block(const Kept& k, const EncParams& ep, const EncBufs& b, u64 i, Buf buf,
                                                const u32 (*sl)[256], const u32 (*z256)[256], DevStatus* st) {
   const u32 lane = threadIdx.x & 63;
@@ -394,7 +412,8 @@ __device__ __forceinline__ void assemble_block(const Kept& k, const EncParams& e
   const u64 q0 = b.ob_first[i];
   const u32 cnt = b.ob_n[i];
   const u64 bytes = b.ob_bytes[i];
-  // entries: header and key delta by the entry's lane
+  const u32 ngroups = (cnt + R - 1) / R;
+  const u64 dend = bytes - 4ull * ngroups - 4;  // where the restart array starts
   u64 base = 0;
   for (u32 j0 = 0; j0 < cnt; j0 += 64) {
     const u32 j = j0 + lane;
@@ -404,53 +423,48 @@ __device__ __forceinline__ void assemble_block(const Kept& k, const EncParams& e
     const u32 sz = act ? (rs ? b.szr[q] : b.sznr[q]) : 0;
     const u32 incl = wave_incl_scan(sz);
     const u64 off = base + incl - sz;
-    u64 voff = 0, vsrc = 0;
-    u32 vl = 0;
     if (act) {
       const u32 ulen = K.klen[q];
       const u32 il = ulen + 8;
       const u32 sh = rs ? 0 : b.shared[q];
-      vl = k.vlen[q];
-      vsrc = k.vptr[q];
+      const u32 vl = k.vlen[q];
+      const u8* vs = reinterpret_cast<const u8*>(k.vptr[q]);
+      const u64 tag = K.tag[q];
       u64 p = off;
       p = buf.put_varint(p, sh);
       p = buf.put_varint(p, il - sh);
       p = buf.put_varint(p, vl);
-      const u64 tag = K.tag[q];
-      for (u32 t = sh; t < il; t++) buf.put(p++, ikey_byte(K, q, ulen, tag, t));
-      voff = p;
-    }
-    // values: the whole wave copies each entry's value, coalesced
-    for (u32 t = 0; t < 64 && j0 + t < cnt; t++) {
-      const u64 vo = __shfl(voff, t, 64);
-      const u64 vs = __shfl(vsrc, t, 64);
-      const u32 vn = __shfl(vl, t, 64);
-      const u8* s = reinterpret_cast<const u8*>(vs);
-      for (u32 x = lane; x < vn; x += 64) buf.put(vo + x, s[x]);
+      if constexpr (NWMAX > 0) {
+        u64 ik[NWMAX];
+        ikey_words<NWMAX>(K, q, ulen, tag, ik);
+#pragma unroll
+        for (u32 w = 0; w < NWMAX; w++)
+#pragma unroll
+          for (u32 t = 0; t < 8; t++) {
+            const u32 x = 8 * w + t;
+            if (x >= sh && x < il) buf.put(p + (x - sh), (u8)(ik[w] >> (56 - 8 * t)));
+          }
+      } else {
+        for (u32 t = sh; t < il; t++) buf.put(p + (t - sh), ikey_byte(K, q, ulen, tag, t));
+      }
+      p += il - sh;
+      if (rs) buf.put32(dend + 4ull * (j / R), ((u32)off << 8) | min(R, cnt - j));
+      for (u32 x = 0; x < vl; x += 64) {
+        u32 v4[16];
+#pragma unroll
+        for (u32 t = 0; t < 16; t++)
+          if (x + 4 * t + 4 <= vl) __builtin_memcpy(&v4[t], vs + x + 4 * t, 4);
+#pragma unroll
+        for (u32 t = 0; t < 16; t++)
+          if (x + 4 * t + 4 <= vl) buf.put32(p + x + 4 * t, v4[t]);
+      }
+      for (u32 x = vl & ~3u; x < vl; x++) buf.put(p + x, vs[x]);
     }
     base += __shfl(incl, 63, 64);
   }
-  // restart array and count: the data ends at `base`
-  const u32 ngroups = (cnt + R - 1) / R;
-  {
-    u64 rbase = 0;
-    for (u32 j0 = 0; j0 < cnt; j0 += 64) {
-      const u32 j = j0 + lane;
-      const bool act = j < cnt;
-      const u64 q = q0 + j;
-      const u32 sz = act ? ((j % R) == 0 ? b.szr[q] : b.sznr[q]) : 0;
-      const u32 incl = wave_incl_scan(sz);
-      const u64 off = rbase + incl - sz;
-      if (act && (j % R) == 0) {
-        const u32 gcnt = min(R, cnt - j);
-        buf.put32(base + 4ull * (j / R), ((u32)off << 8) | gcnt);
-      }
-      rbase += __shfl(incl, 63, 64);
-    }
-  }
   if (lane == 0) {
-    buf.put32(base + 4ull * ngroups, ngroups);
-    if (base + 4ull * ngroups + 4 != bytes) set_err(st, ERR_CORRUPT, SITE_CAPACITY);
+    buf.put32(dend + 4ull * ngroups, ngroups);
+    if (base != dend) set_err(st, ERR_CORRUPT, SITE_CAPACITY);
     buf.put(bytes, 0);  // kNoCompression
   }
   wave_sync();
@@ -519,6 +533,7 @@ __device__ __forceinline__ void wave_copy_out(u8* dst, const uint4* stage4, u32 
 // One wavefront per data block.  Blocks that fit are assembled in the wave's
 // LDS stage and written out with 16-byte stores; larger ones (a single big
 // value) are assembled in place.
+template <u32 NWMAX>
 __global__ __launch_bounds__(kThreads) void k_write_blocks(Kept k, const u32* nbo_dev, EncParams ep, EncBufs b,
                                                             u8* out, u64 out_cap, DevStatus* st) {
   __shared__ u32 sl[4][256];
@@ -534,10 +549,10 @@ __global__ __launch_bounds__(kThreads) void k_write_blocks(Kept k, const u32* nb
     const u64 bytes = b.ob_bytes[i];
     if (bytes + 5 + 16 <= kStage) {
       const u32 mis = (u32)((u64)dst & 15);
-      assemble_block(k, ep, b, i, LdsBuf{reinterpret_cast<u8*>(stage_all[wave]) + mis}, sl, z256, st);
+      assemble_block<NWMAX>(k, ep, b, i, LdsBuf{reinterpret_cast<u8*>(stage_all[wave]) + mis}, sl, z256, st);
       wave_copy_out(dst, stage_all[wave], mis, bytes + 5);
     } else {
-      assemble_block(k, ep, b, i, GlobalBuf{dst}, sl, z256, st);
+      assemble_block<NWMAX>(k, ep, b, i, GlobalBuf{dst}, sl, z256, st);
     }
     wave_sync();
   }
@@ -779,8 +794,15 @@ void launch_file_sizes(KTimer& T, const u32* nf_dev, u32 nf_cap, EncParams ep, E
 }
 void launch_write_blocks(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u8* out,
                          u64 out_cap, DevStatus* st) {
-  KLAUNCH(T, "write_blocks", k_write_blocks, grid_for(nbo_cap, kWaves), kThreads, 0, k, nbo_dev, ep, b, out, out_cap,
-          st);
+  const u32 g = grid_for(nbo_cap, kWaves);
+  if (k.K.W <= 2)
+    KLAUNCH(T, "write_blocks", k_write_blocks<3>, g, kThreads, 0, k, nbo_dev, ep, b, out, out_cap, st);
+  else if (k.K.W <= 4)
+    KLAUNCH(T, "write_blocks", k_write_blocks<5>, g, kThreads, 0, k, nbo_dev, ep, b, out, out_cap, st);
+  else if (k.K.W <= 8)
+    KLAUNCH(T, "write_blocks", k_write_blocks<9>, g, kThreads, 0, k, nbo_dev, ep, b, out, out_cap, st);
+  else
+    KLAUNCH(T, "write_blocks", k_write_blocks<0>, g, kThreads, 0, k, nbo_dev, ep, b, out, out_cap, st);
 }
 void launch_write_filters(KTimer& T, Kept k, const u32* nw_dev, u32 nw_cap, const u32* nf_dev, EncParams ep, EncBufs b,
                           u8* out, u64 out_cap, DevStatus* st) {
