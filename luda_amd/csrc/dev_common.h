@@ -202,24 +202,34 @@ __device__ __forceinline__ u32 wave_crc32c(u64 L, const u32 (*slice)[256], const
   const u64 P = (256 - (L & 255)) & 255;
   const u64 rows = (L + P) >> 8;
   u32 s = 0;
-  for (u64 r = 0; r < rows; r++) {
-    u64 q = 256 * r + 4 * lane;
-    u32 b[4];
+  // Rows go in batches whose bytes are all fetched before any is folded, so
+  // a message in global memory costs one load round trip per batch.
+  constexpr u32 kRows = 8;
+  for (u64 r0 = 0; r0 < rows; r0 += kRows) {
+    u32 b[kRows][4];
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-      const long long x = (long long)(q + t) - (long long)P;
-      const bool in = x >= 0 && (u64)x < L;
-      // The index is clamped before it reaches an address: a padding byte
-      // reads byte 0 and is then zeroed.  No address below the message is
-      // ever formed, not even as the base of a folded immediate offset.
-      const u64 xi = in ? (u64)x : 0;
-      u32 v = (u32)at(xi);
-      v = in ? v : 0u;
-      if (x >= 0 && x < 4) v ^= 0xffu;
-      b[t] = v;
+    for (u32 rr = 0; rr < kRows; rr++) {
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const long long x = (long long)(256 * (r0 + rr) + 4 * lane + t) - (long long)P;
+        const bool in = r0 + rr < rows && x >= 0 && (u64)x < L;
+        // The index is clamped before it reaches an address: a padding byte
+        // reads byte 0 and is then zeroed.  No address below the message is
+        // ever formed, not even as the base of a folded immediate offset.
+        const u64 xi = in ? (u64)x : 0;
+        u32 v = (u32)at(xi);
+        v = in ? v : 0u;
+        if (x >= 0 && x < 4) v ^= 0xffu;
+        b[rr][t] = v;
+      }
     }
-    u32 w = slice[3][b[0]] ^ slice[2][b[1]] ^ slice[1][b[2]] ^ slice[0][b[3]];
-    s = crc_shift(z256, s) ^ w;
+#pragma unroll
+    for (u32 rr = 0; rr < kRows; rr++) {
+      if (r0 + rr < rows) {
+        const u32 w = slice[3][b[rr][0]] ^ slice[2][b[rr][1]] ^ slice[1][b[rr][2]] ^ slice[0][b[rr][3]];
+        s = crc_shift(z256, s) ^ w;
+      }
+    }
   }
 #pragma unroll
   for (int j = 0; j < 6; j++) {

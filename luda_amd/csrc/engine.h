@@ -89,8 +89,8 @@ struct FileInfo {
   u32 pad;
 };
 struct BlockDesc {
-  u64 off;    // data block handle within its file
-  u32 size;
+  u64 addr;   // device address of the data block's first byte
+  u32 size;   // BlockHandle::size (without the 5-byte trailer)
   u32 file;
 };
 

@@ -113,7 +113,7 @@ __global__ void k_index(const FileDesc* files, const FileInfo* info, const u32* 
         set_err(st, ERR_CORRUPT, SITE_INDEX);
         continue;
       }
-      bd.off = boff;
+      bd.addr = (u64)(base + boff);
       bd.size = (u32)bsz;
       blks[blk_base[f] + j] = bd;
     }
@@ -130,7 +130,7 @@ __global__ void k_blkscan(const FileDesc* files, const BlockDesc* blks, const u3
   GRID_STRIDE(b, nb) {
     blk_nent[b] = 0;
     const BlockDesc bd = blks[b];
-    const u8* p = files[bd.file].base + bd.off;
+    const u8* p = reinterpret_cast<const u8*>(bd.addr);
     const u32 n = bd.size;
     if (p[n] == 1) {
       set_err(st, ERR_UNSUPPORTED, SITE_COMPRESSED);
@@ -510,29 +510,67 @@ __device__ __forceinline__ void decode_entries(const Src& src, u32 n, u64 gp, u6
 // LDS with 16-byte loads when it fits `stage` bytes, else read in place; the
 // restart groups' entry prefix follows the stage.
 template <u32 NWMAX>
-__global__ __launch_bounds__(64) void k_decode_entries(const FileDesc* files, const BlockDesc* blks, u32 nb,
-                                                       const u32* blk_ebase, Planes K, u64* vptr, u32* vlen,
-                                                       DevStatus* st, u32 stage, u32 max_nr) {
+__global__ __launch_bounds__(64) void k_decode_entries(const BlockDesc* blks, u32 nb, const u32* blk_ebase, Planes K,
+                                                       u64* vptr, u32* vlen, DevStatus* st, u32 stage, u32 max_nr) {
   extern __shared__ uint4 smem[];
   u32* gpre = reinterpret_cast<u32*>(smem + stage / 16);
   const u32 lane = threadIdx.x;
   u32 maxu = 0;
-  for (u32 b = blockIdx.x; b < nb; b += gridDim.x) {
-    const BlockDesc bd = blks[b];
-    const u8* gp = files[bd.file].base + bd.off;
+  u32 b = blockIdx.x;
+  // the next block's descriptor is loaded one block ahead
+  BlockDesc nd = {0, 0, 0};
+  u64 ne = 0;
+  if (b < nb) {
+    nd = blks[b];
+    ne = blk_ebase[b];
+  }
+  for (; b < nb; b += gridDim.x) {
+    const BlockDesc bd = nd;
+    const u64 ebase = ne;
+    const u8* gp = reinterpret_cast<const u8*>(bd.addr);
     const u32 n = bd.size;
-    const u64 ebase = blk_ebase[b];
     const u64 a0 = (u64)gp & ~15ull;
     const u64 a1 = ((u64)gp + n + 15) & ~15ull;
-    if (a1 - a0 + 16 <= stage) {
-      const uint4* g4 = reinterpret_cast<const uint4*>(a0);
-      for (u32 x = lane; x < (a1 - a0) / 16; x += 64) smem[x] = g4[x];
-      __syncthreads();
+    const bool staged = a1 - a0 + 16 <= stage;
+    const u32 n16 = staged ? (u32)((a1 - a0) / 16) : 0;
+    const u64* g8 = reinterpret_cast<const u64*>(a0);
+    u64* s8 = reinterpret_cast<u64*>(smem);
+    // every staging load of a batch in flight before the first LDS store
+    constexpr u32 kBatch = 8;
+    for (u32 x0 = 0; x0 < n16; x0 += 64 * kBatch) {
+      u64 lo[kBatch], hi[kBatch];
+#pragma unroll
+      for (u32 t = 0; t < kBatch; t++) {
+        const u32 x = x0 + 64 * t + lane;
+        const u32 xc = x < n16 ? x : 0;  // in-bounds index for idle lanes
+        lo[t] = g8[2 * xc];
+        hi[t] = g8[2 * xc + 1];
+      }
+      if (x0 == 0 && b + gridDim.x < nb) {
+        nd = blks[b + gridDim.x];
+        ne = blk_ebase[b + gridDim.x];
+      }
+#pragma unroll
+      for (u32 t = 0; t < kBatch; t++) {
+        const u32 x = x0 + 64 * t + lane;
+        if (x < n16) {
+          s8[2 * x] = lo[t];
+          s8[2 * x + 1] = hi[t];
+        }
+      }
+    }
+    if (n16 == 0 && b + gridDim.x < nb) {
+      nd = blks[b + gridDim.x];
+      ne = blk_ebase[b + gridDim.x];
+    }
+    __syncthreads();
+    if (staged) {
       const LdsWin src{reinterpret_cast<const u64*>(smem), (u32)((u64)gp - a0)};
       decode_entries<NWMAX>(src, n, (u64)gp, ebase, gpre, max_nr, K, vptr, vlen, st, maxu);
     } else {
       decode_entries<NWMAX>(GlobalWin{gp, n}, n, (u64)gp, ebase, gpre, max_nr, K, vptr, vlen, st, maxu);
     }
+    __syncthreads();
   }
   if (maxu) atomicMax(&st->max_ulen, maxu);
 }
@@ -552,7 +590,7 @@ __global__ __launch_bounds__(64) void k_decode_groups(const FileDesc* files, con
   u32 maxu = 0;
   for (u32 b = blockIdx.x; b < nb; b += gridDim.x) {
     const BlockDesc bd = blks[b];
-    const u8* gp = files[bd.file].base + bd.off;
+    const u8* gp = reinterpret_cast<const u8*>(bd.addr);
     const u32 n = bd.size;
     const u64 ebase = blk_ebase[b];
     const u64 a0 = (u64)gp & ~15ull;
@@ -642,24 +680,23 @@ void launch_file_ebase(KTimer& T, const u32* blk_base, const u32* blk_ebase, u32
 void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 nb, const u32* blk_ebase, Planes K,
                    u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr) {
   if (!nb) return;
-  const u32 grid = nb < 65536 ? nb : 65536;
   if (K.W <= 8) {
     // stage: the largest block, its misalignment, and the 16 bytes an 8-byte
     // window may read past it; blocks over 16 KiB are read in place
-    u32 stage = (u32)((std::min<u64>(max_blk, 16384) + 15 + 16 + 15) & ~15ull);
+    const u32 stage = (u32)((std::min<u64>(max_blk, 16384) + 15 + 16 + 15) & ~15ull);
     const size_t lds = stage + 4ull * (max_nr + 1);
     if (lds > 65536) launch_refused("decode (restart arrays too long for LDS)", hipErrorInvalidValue);
+    // a few blocks per wavefront, so that the next descriptor's load overlaps
+    const u32 grid = std::max<u32>(1, std::min<u32>((nb + 3) / 4, 16384));
     if (K.W <= 2)
-      KLAUNCH(T, "decode", k_decode_entries<3>, grid, 64, lds, files, blks, nb, blk_ebase, K, vptr, vlen, st, stage,
-              max_nr);
+      KLAUNCH(T, "decode", k_decode_entries<3>, grid, 64, lds, blks, nb, blk_ebase, K, vptr, vlen, st, stage, max_nr);
     else if (K.W <= 4)
-      KLAUNCH(T, "decode", k_decode_entries<5>, grid, 64, lds, files, blks, nb, blk_ebase, K, vptr, vlen, st, stage,
-              max_nr);
+      KLAUNCH(T, "decode", k_decode_entries<5>, grid, 64, lds, blks, nb, blk_ebase, K, vptr, vlen, st, stage, max_nr);
     else
-      KLAUNCH(T, "decode", k_decode_entries<9>, grid, 64, lds, files, blks, nb, blk_ebase, K, vptr, vlen, st, stage,
-              max_nr);
+      KLAUNCH(T, "decode", k_decode_entries<9>, grid, 64, lds, blks, nb, blk_ebase, K, vptr, vlen, st, stage, max_nr);
     return;
   }
+  const u32 grid = nb < 65536 ? nb : 65536;
   const size_t lds = kStage + 16 + 64 * (8 * K.W + 8);
   KLAUNCH(T, "decode", k_decode_groups, grid, 64, lds, files, blks, nb, blk_ebase, K, vptr, vlen, st);
 }

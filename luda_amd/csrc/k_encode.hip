@@ -168,24 +168,55 @@ __global__ void k_entry_sizes(Kept k, const u32* m_dev, EncBufs b, DevStatus* st
 
 // The block that TableBuilder would build starting at entry q: entries are
 // added until CurrentSizeEstimate() >= block_size (table_builder.cc:119-123),
-// a table cut is due, or the stream ends.
-__global__ void k_bnext(Kept k, const u32* m_dev, EncParams ep, EncBufs b, DevStatus* st) {
+// a table cut is due, or the stream ends.  A workgroup takes 256 starts and
+// stages the sizes of the entries they can reach (kBnextSpan past the last
+// start) in LDS; a walk that runs past the staged span reads on from global
+// memory.
+constexpr u32 kBnextSpan = 768;
+__global__ __launch_bounds__(256) void k_bnext(Kept k, const u32* m_dev, EncParams ep, EncBufs b, DevStatus* st) {
+  __shared__ u32 s_r[256 + kBnextSpan], s_nr[256 + kBnextSpan];
+  __shared__ u8 s_f[256 + kBnextSpan + 1];
   const u64 m = *m_dev;
+  const u32 R = ep.interval;
   u32 smax = 0;
-  GRID_STRIDE(q, m) {
-    u64 acc = 0, j = q;
-    const u32 R = ep.interval;
-    for (;;) {
-      const u64 kq = j - q;
-      acc += (kq % R == 0) ? b.szr[j] : b.sznr[j];
-      j++;
-      const u64 est = acc + 4 * (kq / R + 1) + 4;
-      if (est >= ep.block_size || j >= m || k.forced[j]) break;
+  for (u64 q0 = (u64)blockIdx.x * 256; q0 < m; q0 += (u64)gridDim.x * 256) {
+    const u64 tend = min(q0 + 256 + kBnextSpan, m);  // staged entries [q0, tend)
+    for (u32 x = threadIdx.x; x < 256 + kBnextSpan + 1; x += 256) {
+      const u64 j = q0 + x;
+      if (j < tend) {
+        s_r[x] = b.szr[j];
+        s_nr[x] = b.sznr[j];
+      }
+      s_f[x] = (j < m && k.forced[j]) ? 1 : 0;
     }
-    const u64 cnt = j - q;
-    b.bnext[q] = (u32)j;
-    b.bbytes[q] = (u32)(acc + 4 * ((cnt - 1) / R + 1) + 4);
-    smax = max(smax, (u32)cnt);
+    __syncthreads();
+    const u64 q = q0 + threadIdx.x;
+    if (q < m) {
+      u64 acc = 0, j = q;
+      bool done = false;
+      // within the staged span (LDS only) ...
+      while (!done && j < tend) {
+        const u64 kq = j - q;
+        const u32 x = (u32)(j - q0);
+        acc += kq % R == 0 ? s_r[x] : s_nr[x];
+        j++;
+        const u64 est = acc + 4 * (kq / R + 1) + 4;
+        done = est >= ep.block_size || j >= m || s_f[x + 1] != 0;
+      }
+      // ... then past it (global memory only): the two never share a pointer
+      while (!done) {
+        const u64 kq = j - q;
+        acc += kq % R == 0 ? b.szr[j] : b.sznr[j];
+        j++;
+        const u64 est = acc + 4 * (kq / R + 1) + 4;
+        done = est >= ep.block_size || j >= m || k.forced[j] != 0;
+      }
+      const u64 cnt = j - q;
+      b.bnext[q] = (u32)j;
+      b.bbytes[q] = (u32)(acc + 4 * ((cnt - 1) / R + 1) + 4);
+      smax = max(smax, (u32)cnt);
+    }
+    __syncthreads();
   }
   smax = wave_max(smax);
   if ((threadIdx.x & 63) == 0 && smax) atomicMax(&st->blk_step_max, smax);
