@@ -129,7 +129,7 @@ typedef struct luda_stats {
   uint32_t attempts;        /* 1 unless a capacity guess had to be raised */
   uint32_t key_width;       /* user-key bytes held per entry in the key planes */
   uint64_t data_block_bytes;  /* output data blocks with their trailers */
-  uint64_t top_kernel_bytes;  /* algorithmic HBM bytes of one launch of top_kernel */
+  uint64_t top_kernel_bytes;  /* algorithmic HBM bytes of top_kernel's launches in this call */
 } luda_stats;
 
 typedef struct luda_result {

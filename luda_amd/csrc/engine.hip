@@ -468,15 +468,23 @@ void init_ctx_timer(luda_ctx* c) {
 //   merge         per round: indices read and written + the keys compared once
 //   write_blocks  key and value bytes read + data blocks written
 //   write_filters user keys read + filter bytes written (approx. 1.25 B/key)
+// Algorithmic HBM bytes of all launches of a kernel in one call: what it has
+// to read and write at least, from the job's counts (n entries in, m kept, W
+// key words, soa = bytes of one entry's planes and value reference).
 u64 kernel_bytes(const std::string& name, const luda_stats& s, u32 W, u64 rounds) {
   const u64 n = s.entries_in, m = s.entries_kept;
   const u64 soa = 8ull * W + 4 + 8 + 8 + 4;
-  if (name == "decode") return s.bytes_read + n * soa;
-  if (name == "merge") return rounds ? n * (4 + 4 + soa - 12) : 0;
-  if (name == "write_blocks") return 2 * s.data_block_bytes;
-  if (name == "write_filters") return m * (8ull * W + 4) + m * 2;
+  const u64 tail = s.bytes_written > s.data_block_bytes ? s.bytes_written - s.data_block_bytes : 0;
+  if (name == "decode") return s.bytes_read + n * soa;               // input tables in, planes out
+  if (name == "merge") return rounds * n * (4 + 4 + 8ull * W + 12);  // per round: perm in/out + keys
   if (name == "drop") return n * (4 + 4 + 8 + 8ull * W + 4);
   if (name == "gather") return n * 8 + 2 * m * soa;
+  if (name == "entry_sizes") return m * (2 * (8ull * W + 12) + 12);  // two keys in, three sizes out
+  if (name == "bnext") return m * (4 + 4 + 4 + 4 + 4);               // sizes + forced in, successor + bytes out
+  if (name == "write_blocks") return 2 * s.data_block_bytes;          // entries in, blocks out
+  if (name == "write_filters") return m * (8ull * W + 4) + m * 2;     // keys in, ~10 bits per key out
+  if (name == "write_tail") return 2 * tail;                          // filter data in (CRC), tail blocks out
+  if (name == "index") return s.blocks_in * 40;                       // index entries in, block handles out
   return 0;
 }
 

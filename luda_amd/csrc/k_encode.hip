@@ -10,6 +10,8 @@
 // must start).  Every size the sequential writer would compute one entry at a
 // time is computed per entry, block or table in parallel; the two greedy cut
 // sequences (blocks, tables) come from the orbit kernels.
+#include <algorithm>
+
 #include "engine.h"
 
 namespace {
@@ -192,28 +194,33 @@ __global__ __launch_bounds__(256) void k_bnext(Kept k, const u32* m_dev, EncPara
     __syncthreads();
     const u64 q = q0 + threadIdx.x;
     if (q < m) {
+      // acc: entry bytes so far; ng: restart points so far; rpos: position
+      // of the next entry in its restart group (no division in the walk)
       u64 acc = 0, j = q;
+      u32 ng = 0, rpos = 0;
       bool done = false;
       // within the staged span (LDS only) ...
       while (!done && j < tend) {
-        const u64 kq = j - q;
         const u32 x = (u32)(j - q0);
-        acc += kq % R == 0 ? s_r[x] : s_nr[x];
+        const bool rs = rpos == 0;
+        acc += rs ? s_r[x] : s_nr[x];
+        ng += rs;
+        rpos = rpos + 1 == R ? 0 : rpos + 1;
         j++;
-        const u64 est = acc + 4 * (kq / R + 1) + 4;
-        done = est >= ep.block_size || j >= m || s_f[x + 1] != 0;
+        done = acc + 4ull * ng + 4 >= ep.block_size || j >= m || s_f[x + 1] != 0;
       }
       // ... then past it (global memory only): the two never share a pointer
       while (!done) {
-        const u64 kq = j - q;
-        acc += kq % R == 0 ? b.szr[j] : b.sznr[j];
+        const bool rs = rpos == 0;
+        acc += rs ? b.szr[j] : b.sznr[j];
+        ng += rs;
+        rpos = rpos + 1 == R ? 0 : rpos + 1;
         j++;
-        const u64 est = acc + 4 * (kq / R + 1) + 4;
-        done = est >= ep.block_size || j >= m || k.forced[j] != 0;
+        done = acc + 4ull * ng + 4 >= ep.block_size || j >= m || k.forced[j] != 0;
       }
       const u64 cnt = j - q;
       b.bnext[q] = (u32)j;
-      b.bbytes[q] = (u32)(acc + 4 * ((cnt - 1) / R + 1) + 4);
+      b.bbytes[q] = (u32)(acc + 4ull * ng + 4);
       smax = max(smax, (u32)cnt);
     }
     __syncthreads();
@@ -435,14 +442,12 @@ __device__ __forceinline__ void ikey_words(const Planes& K, u64 q, u32 ulen, u64
 // restart word, and the value copied with up to 16 loads in flight.
 // NWMAX = 0 reads key bytes from the planes one at a time (any key width).
 template <u32 NWMAX, typename Buf>
-__device__ __forceinline__ void assemble_block(const Kept& k, const EncParams& ep, const EncBufs& b, u64 i, Buf buf,
-                                               const u32 (*sl)[256], const u32 (*z256)[256], DevStatus* st) {
+__device__ __forceinline__ void assemble_block(const Kept& k, const EncParams& ep, const EncBufs& b, u64 q0, u32 cnt,
+                                               u64 bytes, Buf buf, const u32 (*sl)[256], const u32 (*z256)[256],
+                                               DevStatus* st) {
   const u32 lane = threadIdx.x & 63;
   const u32 R = ep.interval;
   const Planes& K = k.K;
-  const u64 q0 = b.ob_first[i];
-  const u32 cnt = b.ob_n[i];
-  const u64 bytes = b.ob_bytes[i];
   const u32 ngroups = (cnt + R - 1) / R;
   const u64 dend = bytes - 4ull * ngroups - 4;  // where the restart array starts
   u64 base = 0;
@@ -561,9 +566,29 @@ __device__ __forceinline__ void wave_copy_out(u8* dst, const uint4* stage4, u32 
   if (t0 + lane < n) dst[t0 + lane] = stage[mis + t0 + lane];
 }
 
-// One wavefront per data block.  Blocks that fit are assembled in the wave's
-// LDS stage and written out with 16-byte stores; larger ones (a single big
-// value) are assembled in place.
+// Where output block i goes and which entries it holds.
+struct BlockMeta {
+  u64 off;    // offset in its table
+  u32 file;   // its table
+  u32 first;  // first entry
+  u32 n;      // entries
+  u32 bytes;  // block bytes without the trailer
+};
+__device__ __forceinline__ BlockMeta block_meta(const EncBufs& b, u64 i) {
+  BlockMeta m;
+  m.off = b.ob_off[i];
+  m.file = b.ob_file[i];
+  m.first = b.ob_first[i];
+  m.n = b.ob_n[i];
+  m.bytes = b.ob_bytes[i];
+  return m;
+}
+
+// Each wavefront writes data blocks i, i + S, i + 2S, ... (S = all waves of
+// the grid).  Blocks that fit are assembled in the wave's LDS stage and
+// written out with 16-byte stores; larger ones (a single big value) are
+// assembled in place.  The block after next has its metadata, and the next
+// its table offset, loaded while the current one is assembled.
 template <u32 NWMAX>
 __global__ __launch_bounds__(kThreads) void k_write_blocks(Kept k, const u32* nbo_dev, EncParams ep, EncBufs b,
                                                             u8* out, u64 out_cap, DevStatus* st) {
@@ -575,17 +600,34 @@ __global__ __launch_bounds__(kThreads) void k_write_blocks(Kept k, const u32* nb
   __syncthreads();
   const u64 nbo = *nbo_dev;
   const u32 wave = threadIdx.x >> 6;
-  for (u64 i = (u64)blockIdx.x * kWaves + wave; i < nbo; i += (u64)gridDim.x * kWaves) {
-    u8* dst = out + b.f_off[b.ob_file[i]] + b.ob_off[i];
-    const u64 bytes = b.ob_bytes[i];
+  const u64 S = (u64)gridDim.x * kWaves;
+  u64 i = (u64)blockIdx.x * kWaves + wave;
+  BlockMeta m0 = {0, 0, 0, 0, 0}, m1 = {0, 0, 0, 0, 0};
+  u64 d0 = 0;
+  if (i < nbo) {
+    m0 = block_meta(b, i);
+    d0 = b.f_off[m0.file] + m0.off;
+  }
+  if (i + S < nbo) m1 = block_meta(b, i + S);
+  for (; i < nbo; i += S) {
+    u64 d1 = 0;
+    BlockMeta m2 = {0, 0, 0, 0, 0};
+    if (i + S < nbo) d1 = b.f_off[m1.file] + m1.off;
+    if (i + 2 * S < nbo) m2 = block_meta(b, i + 2 * S);
+    u8* dst = out + d0;
+    const u64 bytes = m0.bytes;
     if (bytes + 5 + 16 <= kStage) {
       const u32 mis = (u32)((u64)dst & 15);
-      assemble_block<NWMAX>(k, ep, b, i, LdsBuf{reinterpret_cast<u8*>(stage_all[wave]) + mis}, sl, z256, st);
+      assemble_block<NWMAX>(k, ep, b, m0.first, m0.n, bytes, LdsBuf{reinterpret_cast<u8*>(stage_all[wave]) + mis},
+                            sl, z256, st);
       wave_copy_out(dst, stage_all[wave], mis, bytes + 5);
     } else {
-      assemble_block<NWMAX>(k, ep, b, i, GlobalBuf{dst}, sl, z256, st);
+      assemble_block<NWMAX>(k, ep, b, m0.first, m0.n, bytes, GlobalBuf{dst}, sl, z256, st);
     }
     wave_sync();
+    m0 = m1;
+    d0 = d1;
+    m1 = m2;
   }
 }
 
@@ -825,7 +867,8 @@ void launch_file_sizes(KTimer& T, const u32* nf_dev, u32 nf_cap, EncParams ep, E
 }
 void launch_write_blocks(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u8* out,
                          u64 out_cap, DevStatus* st) {
-  const u32 g = grid_for(nbo_cap, kWaves);
+  // enough workgroups to fill every CU a few times; each wave loops
+  const u32 g = std::min<u32>(grid_for(nbo_cap, kWaves), 2048);
   if (k.K.W <= 2)
     KLAUNCH(T, "write_blocks", k_write_blocks<3>, g, kThreads, 0, k, nbo_dev, ep, b, out, out_cap, st);
   else if (k.K.W <= 4)
