@@ -332,6 +332,18 @@ __device__ __forceinline__ int ukey_cmp_fixed(const Planes& k, u64 e, const u64*
   return 0;
 }
 
+// Global-memory views of device addresses that kernels get as integers or
+// load from memory (value references, table and block addresses).  Reading
+// through an address-space-1 pointer tells the compiler these bytes are not
+// in LDS, so the loads are neither generic (flat) nor ordered behind the
+// kernel's LDS traffic.  The unaligned types allow any byte offset.
+#define GMEM __attribute__((address_space(1)))
+typedef u32 __attribute__((aligned(1))) u32_una;
+typedef u64 __attribute__((aligned(1))) u64_una;
+__device__ __forceinline__ const GMEM u8* gbytes(u64 addr) { return (const GMEM u8*)addr; }
+__device__ __forceinline__ u32 gload_u32(const GMEM u8* p) { return *(const GMEM u32_una*)p; }
+__device__ __forceinline__ u64 gload_u64(const GMEM u8* p) { return *(const GMEM u64_una*)p; }
+
 // Grid-stride helpers.
 #define GRID_STRIDE(i, n) for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < (u64)(n); i += (u64)gridDim.x * blockDim.x)
 

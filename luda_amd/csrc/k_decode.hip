@@ -335,7 +335,7 @@ struct LdsWin {
   __device__ __forceinline__ u32 at(u32 x) const { return (u32)(win8(x) >> 56); }
 };
 struct GlobalWin {
-  const u8* p;
+  const GMEM u8* p;
   u32 n;  // block bytes: nothing at or past n is read
   __device__ __forceinline__ u64 win8(u32 x) const {
     u64 v = 0;
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(64) void k_decode_entries(const BlockDesc* blks, u3
     const u64 a1 = ((u64)gp + n + 15) & ~15ull;
     const bool staged = a1 - a0 + 16 <= stage;
     const u32 n16 = staged ? (u32)((a1 - a0) / 16) : 0;
-    const u64* g8 = reinterpret_cast<const u64*>(a0);
+    const GMEM u64* g8 = (const GMEM u64*)a0;
     u64* s8 = reinterpret_cast<u64*>(smem);
     // every staging load of a batch in flight before the first LDS store
     constexpr u32 kBatch = 8;
@@ -568,7 +568,7 @@ __global__ __launch_bounds__(64) void k_decode_entries(const BlockDesc* blks, u3
       const LdsWin src{reinterpret_cast<const u64*>(smem), (u32)((u64)gp - a0)};
       decode_entries<NWMAX>(src, n, (u64)gp, ebase, gpre, max_nr, K, vptr, vlen, st, maxu);
     } else {
-      decode_entries<NWMAX>(GlobalWin{gp, n}, n, (u64)gp, ebase, gpre, max_nr, K, vptr, vlen, st, maxu);
+      decode_entries<NWMAX>(GlobalWin{gbytes((u64)gp), n}, n, (u64)gp, ebase, gpre, max_nr, K, vptr, vlen, st, maxu);
     }
     __syncthreads();
   }

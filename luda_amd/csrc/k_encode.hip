@@ -175,21 +175,34 @@ __global__ void k_entry_sizes(Kept k, const u32* m_dev, EncBufs b, DevStatus* st
 // start) in LDS; a walk that runs past the staged span reads on from global
 // memory.
 constexpr u32 kBnextSpan = 768;
+constexpr u32 kBnextTile = 256 + kBnextSpan + 4;
 __global__ __launch_bounds__(256) void k_bnext(Kept k, const u32* m_dev, EncParams ep, EncBufs b, DevStatus* st) {
-  __shared__ u32 s_r[256 + kBnextSpan], s_nr[256 + kBnextSpan];
-  __shared__ u8 s_f[256 + kBnextSpan + 1];
+  // per staged entry: szr | sznr << 31 | forced << 63 (sizes < 2^31, checked)
+  __shared__ u64 s_pk[kBnextTile];
   const u64 m = *m_dev;
   const u32 R = ep.interval;
   u32 smax = 0;
   for (u64 q0 = (u64)blockIdx.x * 256; q0 < m; q0 += (u64)gridDim.x * 256) {
-    const u64 tend = min(q0 + 256 + kBnextSpan, m);  // staged entries [q0, tend)
-    for (u32 x = threadIdx.x; x < 256 + kBnextSpan + 1; x += 256) {
-      const u64 j = q0 + x;
-      if (j < tend) {
-        s_r[x] = b.szr[j];
-        s_nr[x] = b.sznr[j];
+    const u64 tend = min(q0 + kBnextTile, m);  // staged entries [q0, tend)
+    {
+      constexpr u32 kPer = (kBnextTile + 255) / 256;
+      u32 r[kPer], nr[kPer], f[kPer];
+#pragma unroll
+      for (u32 t = 0; t < kPer; t++) {
+        const u64 j = q0 + threadIdx.x + 256 * t;
+        const u64 jc = j < tend ? j : q0;
+        r[t] = b.szr[jc];
+        nr[t] = b.sznr[jc];
+        f[t] = k.forced[jc];
       }
-      s_f[x] = (j < m && k.forced[j]) ? 1 : 0;
+#pragma unroll
+      for (u32 t = 0; t < kPer; t++) {
+        const u32 x = threadIdx.x + 256 * t;
+        if (x < kBnextTile) {
+          if (r[t] >= (1u << 31) || nr[t] >= (1u << 31)) set_err(st, ERR_UNSUPPORTED, SITE_BLOCK_ENTRIES);
+          s_pk[x] = q0 + x < tend ? ((u64)r[t] | ((u64)nr[t] << 31) | ((u64)(f[t] != 0) << 63)) : 0;
+        }
+      }
     }
     __syncthreads();
     const u64 q = q0 + threadIdx.x;
@@ -199,17 +212,25 @@ __global__ __launch_bounds__(256) void k_bnext(Kept k, const u32* m_dev, EncPara
       u64 acc = 0, j = q;
       u32 ng = 0, rpos = 0;
       bool done = false;
-      // within the staged span (LDS only) ...
-      while (!done && j < tend) {
+      // within the staged span, four entries per LDS round trip ...
+      while (!done && j + 5 <= tend) {
         const u32 x = (u32)(j - q0);
-        const bool rs = rpos == 0;
-        acc += rs ? s_r[x] : s_nr[x];
-        ng += rs;
-        rpos = rpos + 1 == R ? 0 : rpos + 1;
-        j++;
-        done = acc + 4ull * ng + 4 >= ep.block_size || j >= m || s_f[x + 1] != 0;
+        u64 pk[5];
+#pragma unroll
+        for (u32 t = 0; t < 5; t++) pk[t] = s_pk[x + t];
+#pragma unroll
+        for (u32 t = 0; t < 4; t++) {
+          if (!done) {
+            const bool rs = rpos == 0;
+            acc += rs ? (pk[t] & 0x7fffffffu) : ((pk[t] >> 31) & 0x7fffffffu);
+            ng += rs;
+            rpos = rpos + 1 == R ? 0 : rpos + 1;
+            j++;
+            done = acc + 4ull * ng + 4 >= ep.block_size || j >= m || (pk[t + 1] >> 63) != 0;
+          }
+        }
       }
-      // ... then past it (global memory only): the two never share a pointer
+      // ... then from global memory: the two never share a pointer
       while (!done) {
         const bool rs = rpos == 0;
         acc += rs ? b.szr[j] : b.sznr[j];
@@ -464,7 +485,7 @@ __device__ __forceinline__ void assemble_block(const Kept& k, const EncParams& e
       const u32 il = ulen + 8;
       const u32 sh = rs ? 0 : b.shared[q];
       const u32 vl = k.vlen[q];
-      const u8* vs = reinterpret_cast<const u8*>(k.vptr[q]);
+      const GMEM u8* vs = gbytes(k.vptr[q]);
       const u64 tag = K.tag[q];
       u64 p = off;
       p = buf.put_varint(p, sh);
@@ -489,7 +510,7 @@ __device__ __forceinline__ void assemble_block(const Kept& k, const EncParams& e
         u32 v4[16];
 #pragma unroll
         for (u32 t = 0; t < 16; t++)
-          if (x + 4 * t + 4 <= vl) __builtin_memcpy(&v4[t], vs + x + 4 * t, 4);
+          if (x + 4 * t + 4 <= vl) v4[t] = gload_u32(vs + x + 4 * t);
 #pragma unroll
         for (u32 t = 0; t < 16; t++)
           if (x + 4 * t + 4 <= vl) buf.put32(p + x + 4 * t, v4[t]);
