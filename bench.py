@@ -82,6 +82,18 @@ def build_device_inputs(eng, wl, opts):
     return dev, keep
 
 
+def copy_probe(nbytes):
+    """A plain copy of nbytes between two HBM buffers (the runtime's blit
+    kernel), three times: the reference kernel for HBM byte counters."""
+    import torch
+    x = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,7 +103,13 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (the known-size kernel that rocprofv3's "
+                         "FETCH_SIZE/WRITE_SIZE are scaled against) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: a compaction job runs whole on one device and this build has no "
               "multi-GPU path; run it without --gpus or with --gpus 1", file=sys.stderr)
