@@ -687,7 +687,7 @@ void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 
     const size_t lds = stage + 4ull * (max_nr + 1);
     if (lds > 65536) launch_refused("decode (restart arrays too long for LDS)", hipErrorInvalidValue);
     // a few blocks per wavefront, so that the next descriptor's load overlaps
-    const u32 grid = std::max<u32>(1, std::min<u32>((nb + 3) / 4, 16384));
+    const u32 grid = std::max<u32>(1, std::min<u32>((nb + 3) / 4, 4096));
     if (K.W <= 2)
       KLAUNCH(T, "decode", k_decode_entries<3>, grid, 64, lds, blks, nb, blk_ebase, K, vptr, vlen, st, stage, max_nr);
     else if (K.W <= 4)

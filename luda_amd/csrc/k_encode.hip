@@ -847,7 +847,7 @@ void launch_entry_sizes(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncBufs 
   KLAUNCH(T, "entry_sizes", k_entry_sizes, grid_for(m_cap, 256), 256, 0, k, m_dev, b, st);
 }
 void launch_bnext(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncParams ep, EncBufs b, DevStatus* st) {
-  KLAUNCH(T, "bnext", k_bnext, grid_for(m_cap, 256), 256, 0, k, m_dev, ep, b, st);
+  KLAUNCH(T, "bnext", k_bnext, std::min<u32>(grid_for(m_cap, 256), 2048), 256, 0, k, m_dev, ep, b, st);
 }
 void launch_block_info(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncBufs b, DevStatus* st) {
   KLAUNCH(T, "block_info", k_block_info, grid_for(m_cap, 256), 256, 0, k, m_dev, b, st);
