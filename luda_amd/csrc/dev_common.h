@@ -133,7 +133,7 @@ __device__ __forceinline__ void st32(u8* p, u32 v) {
 // CRC-32C (Castagnoli), util/crc32c.cc:276-377.  Tables are derived at
 // compile time from the reflected polynomial 0x82F63B78.
 
-#define CRC_ZERO_LEVELS 7  // shifts by 4, 8, 16, ..., 256 zero bytes
+#define CRC_ZERO_LEVELS 22  // shifts by 4, 8, 16, ..., 8 MiB zero bytes
 
 // CRC-32C on a wavefront.  The message is left-padded with zeros to whole
 // rows of 256 bytes; lane i folds the 4-byte words at row offsets 4i with
@@ -196,23 +196,27 @@ __device__ __forceinline__ u32 crc_shift(const u32 (*z)[256], u32 s) {
 // crc32c::Value of a message of L >= 4 bytes, byte(i) returning byte i.
 // Called by all 64 lanes of a wavefront; every lane gets the result.
 // `slice` and `z256` may point at LDS copies of g_crc.slice / g_crc.zeros[6].
+// Raw CRC-32C register (zero start, no final xor) of rows [r0, r1) of the
+// message of L >= 4 bytes left-padded to whole 256-byte rows, with the ~0
+// initial value folded into the message's first four bytes.  Called by all
+// 64 lanes of a wavefront; every lane gets the result.
 template <typename ByteAt>
-__device__ __forceinline__ u32 wave_crc32c(u64 L, const u32 (*slice)[256], const u32 (*z256)[256], ByteAt at) {
+__device__ __forceinline__ u32 wave_crc32c_rows(u64 L, u64 r0, u64 r1, const u32 (*slice)[256],
+                                                const u32 (*z256)[256], ByteAt at) {
   const u32 lane = threadIdx.x & 63;
   const u64 P = (256 - (L & 255)) & 255;
-  const u64 rows = (L + P) >> 8;
   u32 s = 0;
   // Rows go in batches whose bytes are all fetched before any is folded, so
   // a message in global memory costs one load round trip per batch.
   constexpr u32 kRows = 8;
-  for (u64 r0 = 0; r0 < rows; r0 += kRows) {
+  for (u64 rb = r0; rb < r1; rb += kRows) {
     u32 b[kRows][4];
 #pragma unroll
     for (u32 rr = 0; rr < kRows; rr++) {
 #pragma unroll
       for (int t = 0; t < 4; t++) {
-        const long long x = (long long)(256 * (r0 + rr) + 4 * lane + t) - (long long)P;
-        const bool in = r0 + rr < rows && x >= 0 && (u64)x < L;
+        const long long x = (long long)(256 * (rb + rr) + 4 * lane + t) - (long long)P;
+        const bool in = rb + rr < r1 && x >= 0 && (u64)x < L;
         // The index is clamped before it reaches an address: a padding byte
         // reads byte 0 and is then zeroed.  No address below the message is
         // ever formed, not even as the base of a folded immediate offset.
@@ -225,7 +229,7 @@ __device__ __forceinline__ u32 wave_crc32c(u64 L, const u32 (*slice)[256], const
     }
 #pragma unroll
     for (u32 rr = 0; rr < kRows; rr++) {
-      if (r0 + rr < rows) {
+      if (rb + rr < r1) {
         const u32 w = slice[3][b[rr][0]] ^ slice[2][b[rr][1]] ^ slice[1][b[rr][2]] ^ slice[0][b[rr][3]];
         s = crc_shift(z256, s) ^ w;
       }
@@ -236,7 +240,24 @@ __device__ __forceinline__ u32 wave_crc32c(u64 L, const u32 (*slice)[256], const
     u32 other = __shfl_down(s, 1u << j, 64);
     if ((lane & ((2u << j) - 1)) == 0) s = crc_shift(g_crc.zeros[j], s) ^ other;
   }
-  return __shfl(s, 0, 64) ^ 0xffffffffu;
+  return __shfl(s, 0, 64);
+}
+
+// Raw register s followed by n zero bytes (n a multiple of 4, < 16 MiB).
+__device__ __forceinline__ u32 crc_append_zeros(u32 s, u64 n) {
+  u64 w = n >> 2;
+  for (int j = 0; j < CRC_ZERO_LEVELS && w; j++, w >>= 1)
+    if (w & 1) s = crc_shift(g_crc.zeros[j], s);
+  return s;
+}
+
+// crc32c::Value of a message of L >= 4 bytes, byte(i) returning byte i.
+// Called by all 64 lanes of a wavefront; every lane gets the result.
+// `slice` and `z256` may point at LDS copies of g_crc.slice / g_crc.zeros[6].
+template <typename ByteAt>
+__device__ __forceinline__ u32 wave_crc32c(u64 L, const u32 (*slice)[256], const u32 (*z256)[256], ByteAt at) {
+  const u64 rows = (L + 255) >> 8;
+  return wave_crc32c_rows(L, 0, rows, slice, z256, at) ^ 0xffffffffu;
 }
 
 // util/crc32c.h:29-32

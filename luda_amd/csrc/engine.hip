@@ -94,11 +94,6 @@ struct PinnedBuf {
   }
 };
 
-void set_msg(char* err, size_t errlen, const char* fmt, const char* a = "", long long b = 0) {
-  if (!err || !errlen) return;
-  snprintf(err, errlen, fmt, a, b);
-}
-
 const char* site_name(u32 s) {
   switch (s) {
     case SITE_FOOTER: return "table footer";

@@ -706,6 +706,28 @@ __device__ __forceinline__ u8* put_fixed32(u8* p, u32 v) {
   return p + 4;
 }
 
+// crc32c::Value of L bytes with the rows split over the workgroup's waves:
+// each wave folds a contiguous run of rows, and the runs are joined by
+// appending to each the zero bytes of the rows after it.  Called by every
+// thread of the workgroup; every thread gets the result.
+template <typename ByteAt>
+__device__ __forceinline__ u32 block_crc_split(u64 L, const u32 (*sl)[256], const u32 (*z256)[256], u32* part,
+                                               ByteAt at) {
+  const u32 wave = threadIdx.x >> 6;
+  const u64 rows = (L + 255) >> 8;
+  const u64 per = (rows + kWaves - 1) / kWaves;
+  const u64 r0 = min(rows, per * wave), r1 = min(rows, r0 + per);
+  u32 s = wave_crc32c_rows(L, r0, r1, sl, z256, at);
+  s = crc_append_zeros(s, 256 * (rows - r1));
+  if ((threadIdx.x & 63) == 0) part[wave] = s;
+  __syncthreads();
+  u32 x = 0;
+#pragma unroll
+  for (u32 w = 0; w < kWaves; w++) x ^= part[w];
+  __syncthreads();
+  return x ^ 0xffffffffu;
+}
+
 // One workgroup per table: filter offsets, filter/metaindex/index blocks with
 // trailers, footer, and the table's metadata.
 __global__ __launch_bounds__(kThreads) void k_write_tail(Kept k, const u32* nf_dev, EncParams ep, EncBufs b, u8* out,
@@ -713,6 +735,7 @@ __global__ __launch_bounds__(kThreads) void k_write_tail(Kept k, const u32* nf_d
   __shared__ u32 sl[4][256];
   __shared__ u32 z256[4][256];
   __shared__ u64 s_moff, s_msize;
+  __shared__ u32 s_part[kWaves];
   if (st->out_bytes > out_cap || st->err) return;
   load_crc_tables(sl, z256);
   __syncthreads();
@@ -784,26 +807,29 @@ __global__ __launch_bounds__(kThreads) void k_write_tail(Kept k, const u32* nf_d
       }
       if (threadIdx.x == 0) st32(ib + isum + 4ull * (b1 - b0), b1 - b0);
     }
+    // type bytes, then the filter and index blocks' CRCs with the rows of
+    // each split over the four waves (the long ones: ~47 KB of filters per
+    // 2 MiB table), the metaindex's by wave 0
+    if (threadIdx.x == 0) {
+      if (ep.bloom_bits) base[dend + fbsize] = 0;
+      base[ioff + isize] = 0;
+      base[s_moff + s_msize] = 0;
+    }
     __syncthreads();
-    // trailers: wave 0 filter block, wave 1 metaindex, wave 2 index
-    if (wave == 0 && ep.bloom_bits) {
+    if (ep.bloom_bits) {
       u8* fs = base + dend;
-      if (lane == 0) fs[fbsize] = 0;
-      wave_sync();
-      const u32 crc = wave_crc32c(fbsize + 1, sl, z256, [&](u64 x) -> u8 { return fs[x]; });
-      if (lane == 0) st32(fs + fbsize + 1, crc_mask(crc));
-    } else if (wave == 1) {
+      const u32 crc = block_crc_split(fbsize + 1, sl, z256, s_part, [&](u64 x) -> u8 { return fs[x]; });
+      if (threadIdx.x == 0) st32(fs + fbsize + 1, crc_mask(crc));
+    }
+    {
+      u8* is = base + ioff;
+      const u32 crc = block_crc_split(isize + 1, sl, z256, s_part, [&](u64 x) -> u8 { return is[x]; });
+      if (threadIdx.x == 0) st32(is + isize + 1, crc_mask(crc));
+    }
+    if (wave == 0) {
       u8* ms = base + s_moff;
-      if (lane == 0) ms[s_msize] = 0;
-      wave_sync();
       const u32 crc = wave_crc32c(s_msize + 1, sl, z256, [&](u64 x) -> u8 { return ms[x]; });
       if (lane == 0) st32(ms + s_msize + 1, crc_mask(crc));
-    } else if (wave == 2) {
-      u8* is = base + ioff;
-      if (lane == 0) is[isize] = 0;
-      wave_sync();
-      const u32 crc = wave_crc32c(isize + 1, sl, z256, [&](u64 x) -> u8 { return is[x]; });
-      if (lane == 0) st32(is + isize + 1, crc_mask(crc));
     } else if (wave == 3 && lane == 0) {
       // footer (Footer::EncodeTo, table/format.cc:31-40)
       u8* p = base + ioff + isize + 5;
@@ -901,7 +927,8 @@ void launch_write_blocks(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, Enc
 }
 void launch_write_filters(KTimer& T, Kept k, const u32* nw_dev, u32 nw_cap, const u32* nf_dev, EncParams ep, EncBufs b,
                           u8* out, u64 out_cap, DevStatus* st) {
-  KLAUNCH(T, "write_filters", k_write_filters, grid_for(nw_cap, kWaves), kThreads, 0, k, nw_dev, nf_dev, ep, b, out,
+  KLAUNCH(T, "write_filters", k_write_filters, std::min<u32>(grid_for(nw_cap, kWaves), 2048), kThreads, 0, k, nw_dev,
+          nf_dev, ep, b, out,
           out_cap, st);
 }
 void launch_write_tail(KTimer& T, Kept k, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b, u8* out,
