@@ -216,7 +216,7 @@ void launch_ingest(KTimer& T, u64 n, const u8* keys, const u64* key_off, const u
 
 // k_merge.hip
 void launch_merge(KTimer& T, const MergePair* pairs, u32 npairs, u32 total_tiles, const u32* in, u32* out, Planes K);
-u32 merge_items_per_tile();
+u32 merge_items_per_tile(u32 W);  // outputs per merge tile for key planes of W words
 void launch_check_sorted(KTimer& T, Planes K, u64 n, const u32* child_start, u32 nchild, DevStatus* st);
 
 // k_select.hip

@@ -723,11 +723,12 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     if (e1 > e0) runs.push_back({e0, e1});
   }
   cstart.push_back(n);
-  // merge rounds
+  // merge rounds, tiled for key planes of W words (planned again when W grows)
   std::vector<std::vector<MergePair>> rounds;
-  {
+  auto plan_rounds = [&](u32 Wk) {
+    rounds.clear();
     auto cur = runs;
-    const u32 items = merge_items_per_tile();
+    const u32 items = merge_items_per_tile(Wk);
     while (cur.size() > 1) {
       std::vector<MergePair> prs;
       std::vector<std::pair<u32, u32>> nxt;
@@ -745,13 +746,14 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
       rounds.push_back(prs);
       cur = nxt;
     }
-  }
+  };
   const u32 ngp = job->num_grandparents;
   u32 ndeep = 0;
   for (int L = 0; L < 7; L++) ndeep += (u32)lvl_files[L].size();
 
   for (u32 attempt = 1;; attempt++) {
     if (W > kMaxKeyWords) throw Fail{LUDA_NOT_SUPPORTED, "user keys longer than the device key planes"};
+    plan_rounds(W);
     EncodeSetup es = plan_encode(job->options, W, n, out_cap, ngp, nf_min, nw_min);
     // carve phase B
     Planes K;
@@ -884,7 +886,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
         u64 poff = 0;
         for (auto& r : rounds) {
           const MergePair& last = r.back();
-          const u32 items = merge_items_per_tile();
+          const u32 items = merge_items_per_tile(W);
           const u32 tiles = last.tile0 + (last.b1 - last.a0 + items - 1) / items;
           // runs not touched by this round keep their positions: copy them
           launch_merge(T, d_pairs + poff, (u32)r.size(), tiles, in, outp, K);

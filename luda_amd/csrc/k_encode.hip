@@ -452,6 +452,79 @@ __device__ __forceinline__ void ikey_words(const Planes& K, u64 q, u32 ulen, u64
   }
 }
 
+// Byte j of big-endian words w.
+template <u32 N>
+__device__ __forceinline__ u32 word_byte(const u64 (&w)[N], u32 j) {
+  u64 x = 0;
+#pragma unroll
+  for (u32 t = 0; t < N; t++)
+    if (t == (j >> 3)) x = w[t];
+  return (u32)(x >> (56 - 8 * (j & 7))) & 0xff;
+}
+
+// The index key after a block whose last entry is a, in NWMAX big-endian
+// words, and its length: InternalKeyComparator::FindShortestSeparator
+// against b's key (db/dbformat.cc:69-86, util/comparator.cc:29-48) or, with
+// no b, FindShortSuccessor (db/dbformat.cc:88-101, util/comparator.cc:50-63).
+// A shortened user key gets the tag PackSequenceAndType(kMaxSequenceNumber,
+// kValueTypeForSeek); otherwise the key is a's internal key.
+template <u32 NWMAX>
+__device__ __forceinline__ u32 index_key_words(const Planes& K, u64 a, bool has_b, u64 bq, u64 (&out)[NWMAX]) {
+  const u32 la = K.klen[a];
+  u64 ua[NWMAX];
+#pragma unroll
+  for (u32 w = 0; w < NWMAX; w++) ua[w] = w < K.W ? K.P[(u64)w * K.stride + a] : 0;
+  u32 cut = 0xffffffffu;  // user-key bytes kept before the incremented byte
+  u32 inc = 0;            // the incremented byte
+  if (has_b) {
+    const u32 lb = K.klen[bq];
+    const u32 m = min(la, lb);
+    u32 d = m;
+#pragma unroll
+    for (u32 w = 0; w < NWMAX; w++) {
+      const u64 x = w < K.W ? ua[w] ^ K.P[(u64)w * K.stride + bq] : 0;
+      if (x && d == m) d = min(8 * w + ((u32)__clzll((long long)x) >> 3), m);
+    }
+    if (d < m) {
+      const u32 da = word_byte(ua, d);
+      u32 db = 0;
+#pragma unroll
+      for (u32 w = 0; w < NWMAX; w++)
+        if (w == (d >> 3) && w < K.W) db = (u32)(K.P[(u64)w * K.stride + bq] >> (56 - 8 * (d & 7))) & 0xff;
+      if (da < 0xff && da + 1 < db && d + 1 < la) {
+        cut = d;
+        inc = da + 1;
+      }
+    }
+  } else {
+    u32 i = la;
+#pragma unroll
+    for (u32 w = 0; w < NWMAX; w++) {
+      const u64 nb = ~ua[w];  // first byte that is not 0xff
+      if (nb && i == la) i = min(8 * w + ((u32)__clzll((long long)nb) >> 3), la);
+    }
+    if (i + 1 < la) {
+      cut = i;
+      inc = word_byte(ua, i) + 1;
+    }
+  }
+  const bool shortened = cut != 0xffffffffu;
+  const u32 plen = shortened ? cut : la;                 // user-key bytes copied
+  const u32 at = shortened ? cut + 1 : la;                // where the tag goes
+  const u64 tb = __builtin_bswap64(shortened ? kMaxSeqTag : K.tag[a]);
+  const u32 wi = at >> 3, sft = (at & 7) * 8;
+#pragma unroll
+  for (u32 w = 0; w < NWMAX; w++) {
+    const u32 nb = plen > 8 * w ? min(plen - 8 * w, 8u) : 0u;
+    u64 x = nb ? ua[w] & (~0ull << (64 - 8 * nb)) : 0ull;
+    if (shortened && w == (cut >> 3)) x |= (u64)inc << (56 - 8 * (cut & 7));
+    if (w == wi) x |= tb >> sft;
+    if (sft && w == wi + 1) x |= tb << (64 - sft);
+    out[w] = x;
+  }
+  return at + 8;
+}
+
 // The bytes of output block i (entries, restart array, type byte, masked CRC)
 // assembled at buf: BlockBuilder::Add/Finish (table/block_builder.cc:62-114)
 // with the fork's (offset << 8) | count restart words, and
@@ -730,6 +803,7 @@ __device__ __forceinline__ u32 block_crc_split(u64 L, const u32 (*sl)[256], cons
 
 // One workgroup per table: filter offsets, filter/metaindex/index blocks with
 // trailers, footer, and the table's metadata.
+template <u32 NWMAX>
 __global__ __launch_bounds__(kThreads) void k_write_tail(Kept k, const u32* nf_dev, EncParams ep, EncBufs b, u8* out,
                                                           u64 out_cap, DevStatus* st) {
   __shared__ u32 sl[4][256];
@@ -793,16 +867,33 @@ __global__ __launch_bounds__(kThreads) void k_write_tail(Kept k, const u32* nf_d
         const u64 eoff = b.ob_isz[i] - ibase;
         u8* p = ib + eoff;
         const u64 last = (u64)b.ob_first[i] + b.ob_n[i] - 1;
-        const u32 kl = (i + 1 < b1) ? index_sep(K, last, b.ob_first[i + 1], nullptr) : index_succ(K, last, nullptr);
-        const u32 hl = varint_len(b.ob_off[i]) + varint_len(b.ob_bytes[i]);
-        *p++ = 0;
-        p = enc_varint(p, kl);
-        p = enc_varint(p, hl);
-        if (i + 1 < b1) index_sep(K, last, b.ob_first[i + 1], p);
-        else index_succ(K, last, p);
-        p += kl;
-        p = enc_varint(p, b.ob_off[i]);
-        p = enc_varint(p, b.ob_bytes[i]);
+        const bool has_b = i + 1 < b1;
+        const u64 boff = b.ob_off[i], bsz = b.ob_bytes[i];
+        const u32 hl = varint_len(boff) + varint_len(bsz);
+        if constexpr (NWMAX > 0) {
+          // the key in registers first, then its bytes: no load waits on a store
+          u64 kw[NWMAX];
+          const u32 kl = index_key_words<NWMAX>(K, last, has_b, has_b ? b.ob_first[i + 1] : 0, kw);
+          *p++ = 0;
+          p = enc_varint(p, kl);
+          p = enc_varint(p, hl);
+#pragma unroll
+          for (u32 w = 0; w < NWMAX; w++)
+#pragma unroll
+            for (u32 t = 0; t < 8; t++)
+              if (8 * w + t < kl) p[8 * w + t] = (u8)(kw[w] >> (56 - 8 * t));
+          p += kl;
+        } else {
+          const u32 kl = has_b ? index_sep(K, last, b.ob_first[i + 1], nullptr) : index_succ(K, last, nullptr);
+          *p++ = 0;
+          p = enc_varint(p, kl);
+          p = enc_varint(p, hl);
+          if (has_b) index_sep(K, last, b.ob_first[i + 1], p);
+          else index_succ(K, last, p);
+          p += kl;
+        }
+        p = enc_varint(p, boff);
+        p = enc_varint(p, bsz);
         st32(ib + isum + 4ull * (i - b0), ((u32)eoff << 8) | 1u);
       }
       if (threadIdx.x == 0) st32(ib + isum + 4ull * (b1 - b0), b1 - b0);
@@ -933,5 +1024,13 @@ void launch_write_filters(KTimer& T, Kept k, const u32* nw_dev, u32 nw_cap, cons
 }
 void launch_write_tail(KTimer& T, Kept k, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b, u8* out,
                        u64 out_cap, DevStatus* st) {
-  KLAUNCH(T, "write_tail", k_write_tail, grid_for(nf_cap, 1), kThreads, 0, k, nf_dev, ep, b, out, out_cap, st);
+  const u32 g = grid_for(nf_cap, 1);
+  if (k.K.W <= 2)
+    KLAUNCH(T, "write_tail", k_write_tail<3>, g, kThreads, 0, k, nf_dev, ep, b, out, out_cap, st);
+  else if (k.K.W <= 4)
+    KLAUNCH(T, "write_tail", k_write_tail<5>, g, kThreads, 0, k, nf_dev, ep, b, out, out_cap, st);
+  else if (k.K.W <= 8)
+    KLAUNCH(T, "write_tail", k_write_tail<9>, g, kThreads, 0, k, nf_dev, ep, b, out, out_cap, st);
+  else
+    KLAUNCH(T, "write_tail", k_write_tail<0>, g, kThreads, 0, k, nf_dev, ep, b, out, out_cap, st);
 }
