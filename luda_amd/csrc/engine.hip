@@ -346,6 +346,7 @@ void run_encode(luda_ctx* c, const EncodeSetup& s, const EncodeArrays& A, u8* ou
   launch_bnext(T, k, m_dev, s.m_cap, s.ep, b, st);
   run_orbit(T, s.ob, b.bnext, m_dev, &st->blk_step_max, b.bs, A.ftab_b, A.ent_b, st);
   scan_u32(T, b.bs, b.bid, m_dev, 0, s.m_cap, &st->n_blocks_out, A.scan_tmp);
+  KLAUNCH(T, "guard", k_guard, 1, 1, 0, &st->n_blocks_out, s.nbo_cap, st, &st->need_blocks);
   launch_block_info(T, k, m_dev, s.m_cap, b, st);
   launch_block_tot(T, nbo_dev, s.nbo_cap, b, b.ob_tot);
   scan_u32_to_u64(T, b.ob_tot, b.O, nbo_dev, 1, s.nbo_cap + 1, &st->data_bytes, A.scan_tmp);
@@ -390,7 +391,7 @@ u32 orbit_chunk(u64 step, u32 min_chunk) {
 
 // Capacities and orbit plans for an encoder run over at most m entries.
 EncodeSetup plan_encode(const luda_options& o, u32 W, u64 m, u64 out_cap, u64 extra_files, u64 nf_cap_min,
-                        u64 nw_cap_min) {
+                        u64 nw_cap_min, u64 nbo_cap_min) {
   EncodeSetup s;
   memset(&s.ep, 0, sizeof(s.ep));
   s.ep.block_size = o.block_size;
@@ -403,11 +404,15 @@ EncodeSetup plan_encode(const luda_options& o, u32 W, u64 m, u64 out_cap, u64 ex
   s.ep.bloom_k = (u32)k;
   s.ep.kw = 8 * W + 8;
   s.m_cap = (u32)std::max<u64>(m, 1);
-  s.nbo_cap = s.m_cap;
   s.out_cap = out_cap;
   s.size_cuts = o.max_file_size < out_cap;
   u64 nf = s.size_cuts ? out_cap / std::max<u64>(o.max_file_size, 1) + extra_files + 2 : extra_files + 2;
   nf = std::max<u64>(nf, nf_cap_min);
+  // A block closed for size holds at least block_size bytes; the others end
+  // a table (<= nf), precede a ShouldStopBefore cut (<= extra_files) or end
+  // the stream.  A count over this bound is caught on the device and retried.
+  const u64 nbo = out_cap / o.block_size + nf + extra_files + 2;
+  s.nbo_cap = (u32)std::min<u64>(s.m_cap, std::max<u64>(nbo, nbo_cap_min));
   s.nf_cap = (u32)std::min<u64>(nf, s.nbo_cap);
   u64 nw = o.bloom_bits_per_key ? out_cap / 2048 + s.nf_cap + 2 : 1;
   s.nw_cap = (u32)std::max<u64>(nw, nw_cap_min);
@@ -713,7 +718,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
   // ---- phase B
   u32 W = std::max<u32>(1, (std::max<u32>(stA.max_rulen, 8 * c->kw_hint) + 7) / 8);
   u64 out_cap = bytes_in + bytes_in / 4 + (1 << 20);
-  u64 nf_min = 0, nw_min = 0;
+  u64 nf_min = 0, nw_min = 0, nbo_min = 0;
   // Runs to merge: children's entry ranges, empty ones left out.
   std::vector<u32> cstart;
   std::vector<std::pair<u32, u32>> runs;
@@ -754,7 +759,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
   for (u32 attempt = 1;; attempt++) {
     if (W > kMaxKeyWords) throw Fail{LUDA_NOT_SUPPORTED, "user keys longer than the device key planes"};
     plan_rounds(W);
-    EncodeSetup es = plan_encode(job->options, W, n, out_cap, ngp, nf_min, nw_min);
+    EncodeSetup es = plan_encode(job->options, W, n, out_cap, ngp, nf_min, nw_min, nbo_min);
     // carve phase B
     Planes K;
     u64 *vptr, *gp_words, *gp_tag, *gp_prefix, *deep_words;
@@ -935,6 +940,10 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
         nw_min = (u64)stB.need_windows + 16;
         grew = true;
       }
+      if (stB.need_blocks) {
+        nbo_min = (u64)stB.need_blocks + 16;
+        grew = true;
+      }
       if (!grew) throw Fail{LUDA_NOT_SUPPORTED, std::string("capacity: ") + site_name(stB.site)};
       continue;
     }
@@ -1009,9 +1018,9 @@ luda_result* build_tables(luda_ctx* c, const luda_options* opts, u64 n, const u8
   stats.bytes_read = kbytes + vbytes;
   HIPCHK(hipEventRecord(c->ev0, c->s));
   u64 out_cap = kbytes + vbytes + 16 * n + (1 << 20);
-  u64 nf_min = 0, nw_min = 0;
+  u64 nf_min = 0, nw_min = 0, nbo_min = 0;
   for (u32 attempt = 1;; attempt++) {
-    EncodeSetup es = plan_encode(*opts, W, n, out_cap, 0, nf_min, nw_min);
+    EncodeSetup es = plan_encode(*opts, W, n, out_cap, 0, nf_min, nw_min, nbo_min);
     EncodeArrays EA;
     {
       Arena dry;
@@ -1053,6 +1062,7 @@ luda_result* build_tables(luda_ctx* c, const luda_options* opts, u64 n, const u8
       else out_cap *= 2;
       if (stB.need_files) nf_min = (u64)stB.need_files + 16;
       if (stB.need_windows) nw_min = (u64)stB.need_windows + 16;
+      if (stB.need_blocks) nbo_min = (u64)stB.need_blocks + 16;
       continue;
     }
     stats.attempts = attempt;

@@ -252,16 +252,17 @@ __global__ __launch_bounds__(256) void k_bnext(Kept k, const u32* m_dev, EncPara
 
 __global__ void k_block_info(Kept k, const u32* m_dev, EncBufs b, DevStatus* st) {
   const u64 m = *m_dev;
+  const u32 nbo = st->n_blocks_out;  // 0 when the count overflowed its capacity (reported)
   GRID_STRIDE(q, m) {
     if (!b.bs[q]) continue;
     const u32 id = b.bid[q];
+    if (id >= nbo) continue;
     b.ob_first[id] = (u32)q;
     b.ob_n[id] = b.bnext[q] - (u32)q;
     b.ob_bytes[id] = b.bbytes[q];
     b.ob_forced[id] = (q > 0 && k.forced[q]) ? 1u : 0u;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    const u32 nbo = st->n_blocks_out;
     b.ob_forced[nbo] = 0;
     b.ob_bytes[nbo] = 0;
   }

@@ -10,11 +10,14 @@
 // LDS).  Those functions are composed in groups of G, level by level, until
 // one is left; the entry offsets are then pushed back down the levels, and
 // each chunk marks its own orbit nodes.
+#include <string.h>
+
 #include "engine.h"
 
 namespace {
 
 constexpr u16 kEnd = 0xffff;
+constexpr u32 kMaxOrbitLevels = 16;
 constexpr u32 kThreads = 256;
 
 __device__ __forceinline__ u64 level_items(u64 n, u32 C, u32 G, u32 level) {
@@ -120,6 +123,60 @@ __global__ __launch_bounds__(kThreads) void k_orbit_expand(const u16* fin, const
 
 __global__ void k_orbit_root(u32* e) { *e = 0; }
 
+// All levels above the chunks in one workgroup: compose level by level up to
+// the single top function, then push entry offsets back down, with a
+// workgroup barrier between levels (levels are small: the chunk functions
+// number n / C).  Same tables and results as k_orbit_compose / _root /
+// _expand.  eoff[l] is the offset of level l's entries in `entries`.
+struct OrbitLevels {
+  u32 levels;
+  u64 level_offset[kMaxOrbitLevels];  // of each level's functions in ftab
+  u64 entry_offset[kMaxOrbitLevels];  // of each level's entries in entries
+};
+__global__ __launch_bounds__(1024) void k_orbit_top(u16* ftab, u32* entries, const u32* n_dev, const u32* step_dev,
+                                                    u32 C, u32 G, u32 Wc, OrbitLevels L) {
+  const u64 n = *n_dev;
+  const u32 step = *step_dev;
+  if (step > Wc) return;  // k_orbit_chunk reported the overflow
+  const u32 Wd = step ? step : 1;
+  const u32 tid = threadIdx.x;
+  for (u32 l = 0; l < L.levels; l++) {
+    const u16* fin = ftab + L.level_offset[l];
+    u16* fout = ftab + L.level_offset[l + 1];
+    const u64 nin = level_items(n, C, G, l);
+    const u64 nout = (nin + G - 1) / G;
+    for (u64 i = tid; i < nout * Wd; i += blockDim.x) {
+      const u64 o = i / Wd;
+      const u32 x = (u32)(i % Wd);
+      const u64 t0 = o * G;
+      const u32 tc = (u32)min((u64)G, nin - t0);
+      u32 y = x;
+      for (u32 t = 0; t < tc && y != kEnd; t++) y = y < Wd ? fin[(t0 + t) * Wc + y] : kEnd;
+      fout[o * Wc + x] = (u16)y;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) entries[L.entry_offset[L.levels]] = 0;
+  __syncthreads();
+  for (u32 l = L.levels; l-- > 0;) {
+    const u16* fin = ftab + L.level_offset[l];
+    const u32* eout = entries + L.entry_offset[l + 1];
+    u32* ein = entries + L.entry_offset[l];
+    const u64 nin = level_items(n, C, G, l);
+    const u64 nout = (nin + G - 1) / G;
+    for (u64 o = tid; o < nout; o += blockDim.x) {
+      const u64 t0 = o * G;
+      const u32 tc = (u32)min((u64)G, nin - t0);
+      u32 y = eout[o];
+      for (u32 t = 0; t < tc; t++) {
+        ein[t0 + t] = y;
+        if (y != kEnd) y = y < Wd ? fin[(t0 + t) * Wc + y] : kEnd;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 __global__ __launch_bounds__(kThreads) void k_orbit_mark(const u32* nxt, const u32* n_dev, const u32* step_dev,
                                                           u32 C, u32 Wc, const u32* e0, u32* flag) {
   extern __shared__ u32 loc[];
@@ -196,6 +253,20 @@ void run_orbit(KTimer& T, const OrbitPlan& p, const u32* nxt, const u32* n_dev, 
   }
   KLAUNCH(T, "orbit_chunk", k_orbit_chunk, grid_for(p.level_items[0]), kThreads, p.C * sizeof(u32), nxt, n_dev,
           step_dev, p.C, p.Wc, ftab, st);
+  if (L + 1 <= kMaxOrbitLevels && p.level_items[0] <= 4096) {
+    // few chunks: every level above them in one launch
+    OrbitLevels lv;
+    memset(&lv, 0, sizeof(lv));
+    lv.levels = L;
+    for (u32 l = 0; l <= L; l++) {
+      lv.level_offset[l] = p.level_offset[l];
+      lv.entry_offset[l] = eoff[l];
+    }
+    KLAUNCH(T, "orbit_top", k_orbit_top, 1, 1024, 0, ftab, entries, n_dev, step_dev, p.C, p.G, p.Wc, lv);
+    KLAUNCH(T, "orbit_mark", k_orbit_mark, grid_for(p.level_items[0]), kThreads, p.C * sizeof(u32) + p.C, nxt, n_dev,
+            step_dev, p.C, p.Wc, entries, flag);
+    return;
+  }
   for (u32 l = 0; l < L; l++)
     KLAUNCH(T, "orbit_compose", k_orbit_compose, grid_for(p.level_items[l + 1]), kThreads,
             (size_t)p.G * p.Wc * sizeof(u16), ftab + p.level_offset[l], ftab + p.level_offset[l + 1], n_dev, step_dev,
