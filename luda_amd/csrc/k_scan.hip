@@ -111,7 +111,7 @@ __global__ __launch_bounds__(kThreads) void k_scan_down(const Tin* in, Tout* out
 // running carry: for short arrays one launch beats three.  In place is
 // allowed (a chunk is read into registers before any of it is written).
 constexpr u32 kSingleItems = 8;
-constexpr u64 kSingleMax = 131072;
+constexpr u64 kSingleMax = 16384;
 template <typename Tin, typename Tout>
 __global__ __launch_bounds__(kTopThreads) void k_scan_single(const Tin* in, Tout* out, const u32* n_dev, u32 n_add,
                                                              Tout* total) {
