@@ -245,6 +245,57 @@ __device__ __forceinline__ u32 wave_crc32c_rows(u64 L, u64 r0, u64 r1, const u32
   return __shfl(s, 0, 64);
 }
 
+// wave_crc32c_rows for a message held in LDS at byte `mis` of the 4-byte
+// words s32 (readable for 4 bytes past the message): every row but the ones
+// touching the padding or the first four bytes reads its four bytes per
+// lane with two aligned LDS words and a byte-align.
+__device__ __forceinline__ u32 wave_crc32c_rows_lds(u64 L, u64 r0, u64 r1, const u32* s32, u32 mis,
+                                                    const u32 (*slice)[256], const u32 (*z256)[256]) {
+  const u32 lane = threadIdx.x & 63;
+  const u64 P = (256 - (L & 255)) & 255;
+  const u8* s8 = reinterpret_cast<const u8*>(s32);
+  u32 s = 0;
+  constexpr u32 kRows = 8;
+  for (u64 rb = r0; rb < r1; rb += kRows) {
+    u32 wv[kRows];
+#pragma unroll
+    for (u32 rr = 0; rr < kRows; rr++) {
+      const long long x = (long long)(256 * (rb + rr) + 4 * lane) - (long long)P;
+      u32 v = 0;
+      if (rb + rr < r1) {
+        if (x >= 4 && (u64)x + 4 <= L) {
+          const u32 o = mis + (u32)x;
+          v = __builtin_amdgcn_alignbyte(s32[(o >> 2) + 1], s32[o >> 2], o & 3);
+        } else {
+#pragma unroll
+          for (int t = 0; t < 4; t++) {
+            const long long xt = x + t;
+            const bool in = xt >= 0 && (u64)xt < L;
+            u32 bt = in ? (u32)s8[mis + (u32)(in ? xt : 0)] : 0u;
+            if (xt >= 0 && xt < 4) bt ^= 0xffu;
+            v |= bt << (8 * t);
+          }
+        }
+      }
+      wv[rr] = v;
+    }
+#pragma unroll
+    for (u32 rr = 0; rr < kRows; rr++) {
+      if (rb + rr < r1) {
+        const u32 v = wv[rr];
+        const u32 w = slice[3][v & 0xff] ^ slice[2][(v >> 8) & 0xff] ^ slice[1][(v >> 16) & 0xff] ^ slice[0][v >> 24];
+        s = crc_shift(z256, s) ^ w;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    u32 other = __shfl_down(s, 1u << j, 64);
+    if ((lane & ((2u << j) - 1)) == 0) s = crc_shift(g_crc.zeros[j], s) ^ other;
+  }
+  return __shfl(s, 0, 64);
+}
+
 // Raw register s followed by n zero bytes (n a multiple of 4, < 16 MiB).
 __device__ __forceinline__ u32 crc_append_zeros(u32 s, u64 n) {
   u64 w = n >> 2;

@@ -554,6 +554,8 @@ __device__ __forceinline__ void assemble_block(const Kept& k, const EncParams& e
     const u32 sz = act ? (rs ? b.szr[q] : b.sznr[q]) : 0;
     const u32 incl = wave_incl_scan(sz);
     const u64 off = base + incl - sz;
+    u64 my_voff = 0, my_vsrc = 0;
+    u32 my_vl = 0;
     if (act) {
       const u32 ulen = K.klen[q];
       const u32 il = ulen + 8;
@@ -580,17 +582,23 @@ __device__ __forceinline__ void assemble_block(const Kept& k, const EncParams& e
       }
       p += il - sh;
       if (rs) buf.put32(dend + 4ull * (j / R), ((u32)off << 8) | min(R, cnt - j));
-      for (u32 x = 0; x < vl; x += 64) {
-        u32 v4[16];
+      if constexpr (!Buf::kLds) {
+        for (u32 x = 0; x < vl; x += 64) {
+          u32 v4[16];
 #pragma unroll
-        for (u32 t = 0; t < 16; t++)
-          if (x + 4 * t + 4 <= vl) v4[t] = gload_u32(vs + x + 4 * t);
+          for (u32 t = 0; t < 16; t++)
+            if (x + 4 * t + 4 <= vl) v4[t] = gload_u32(vs + x + 4 * t);
 #pragma unroll
-        for (u32 t = 0; t < 16; t++)
-          if (x + 4 * t + 4 <= vl) buf.put32(p + x + 4 * t, v4[t]);
+          for (u32 t = 0; t < 16; t++)
+            if (x + 4 * t + 4 <= vl) buf.put32(p + x + 4 * t, v4[t]);
+        }
+        for (u32 x = vl & ~3u; x < vl; x++) buf.put(p + x, vs[x]);
       }
-      for (u32 x = vl & ~3u; x < vl; x++) buf.put(p + x, vs[x]);
+      my_voff = p;
+      my_vl = vl;
+      my_vsrc = k.vptr[q];
     }
+    if constexpr (Buf::kLds) copy_values_chunked(buf, act, my_voff, my_vsrc, my_vl);
     base += __shfl(incl, 63, 64);
   }
   if (lane == 0) {
@@ -599,33 +607,54 @@ __device__ __forceinline__ void assemble_block(const Kept& k, const EncParams& e
     buf.put(bytes, 0);  // kNoCompression
   }
   wave_sync();
-  const u32 crc = wave_crc32c(bytes + 1, sl, z256, [&](u64 x) -> u8 { return buf.get(x); });
+  const u32 crc = buf.crc(bytes + 1, sl, z256);
   if (lane == 0) buf.put32(bytes + 1, crc_mask(crc));
   wave_sync();
 }
 
-// Byte sinks for assemble_block.  LdsBuf holds a pointer into a __shared__
-// array and nothing else, so the compiler addresses it with ds_* instructions.
+// Where the values of one group of up to 64 entries go: per entry lane, its
+// first 16-byte chunk in the group's chunk sequence, its offset in the
+// block, its length and its source address.
+struct ValueTab {
+  u32 cp[64];
+  u32 off[64];
+  u32 vl[64];
+  u64 src[64];
+};
+
+// Byte sinks for assemble_block.  LdsBuf points into the wave's __shared__
+// stage (the block starts `mis` bytes in) and nothing else, so the compiler
+// addresses it with ds_* instructions.
 struct LdsBuf {
-  u8* p;
-  __device__ __forceinline__ void put(u64 x, u8 v) const { p[x] = v; }
-  __device__ __forceinline__ u8 get(u64 x) const { return p[x]; }
+  static constexpr bool kLds = true;
+  u8* base;  // the stage, 16-byte aligned
+  u32 mis;
+  ValueTab* vt;
+  __device__ __forceinline__ void put(u64 x, u8 v) const { base[mis + x] = v; }
+  __device__ __forceinline__ u8 get(u64 x) const { return base[mis + x]; }
   __device__ __forceinline__ void put32(u64 x, u32 v) const {
-    p[x] = (u8)v;
-    p[x + 1] = (u8)(v >> 8);
-    p[x + 2] = (u8)(v >> 16);
-    p[x + 3] = (u8)(v >> 24);
+    u8* p = base + mis + x;
+    p[0] = (u8)v;
+    p[1] = (u8)(v >> 8);
+    p[2] = (u8)(v >> 16);
+    p[3] = (u8)(v >> 24);
   }
   __device__ __forceinline__ u64 put_varint(u64 x, u64 v) const {
+    u8* p = base + mis;
     while (v >= 128) {
       p[x++] = (u8)(v | 128);
       v >>= 7;
     }
     p[x++] = (u8)v;
     return x;
+  }
+  __device__ __forceinline__ u32 crc(u64 L, const u32 (*sl)[256], const u32 (*z256)[256]) const {
+    const u64 rows = (L + 255) >> 8;
+    return wave_crc32c_rows_lds(L, 0, rows, reinterpret_cast<const u32*>(base), mis, sl, z256) ^ 0xffffffffu;
   }
 };
 struct GlobalBuf {
+  static constexpr bool kLds = false;
   u8* p;
   __device__ __forceinline__ void put(u64 x, u8 v) const { p[x] = v; }
   __device__ __forceinline__ u8 get(u64 x) const { return p[x]; }
@@ -643,7 +672,68 @@ struct GlobalBuf {
     p[x++] = (u8)v;
     return x;
   }
+  __device__ __forceinline__ u32 crc(u64 L, const u32 (*sl)[256], const u32 (*z256)[256]) const {
+    return wave_crc32c(L, sl, z256, [&](u64 x) -> u8 { return p[x]; });
+  }
 };
+
+// The values of a group of entries (lanes with act) copied into an LDS
+// block as 16-byte chunks spread over the whole wave: lane c of a round
+// takes chunk c of the group, found by a binary search of the chunk prefix.
+// Sources are read 16 bytes at a time; they are always followed by at least
+// 15 more readable bytes (a table's restart array, trailer and footer, or
+// the padded value buffer of luda_build_tables).
+__device__ __forceinline__ void copy_values_chunked(const LdsBuf& buf, bool act, u64 voff, u64 vsrc, u32 vl) {
+  const u32 lane = threadIdx.x & 63;
+  ValueTab& vt = *buf.vt;
+  const u32 nch = act ? (vl + 15) >> 4 : 0;
+  const u32 cincl = wave_incl_scan(nch);
+  const u32 C = __shfl(cincl, 63, 64);
+  vt.cp[lane] = cincl - nch;
+  vt.off[lane] = (u32)voff;
+  vt.vl[lane] = vl;
+  vt.src[lane] = vsrc;
+  wave_sync();
+  constexpr u32 kRounds = 4;
+  for (u32 c0 = 0; c0 < C; c0 += 64 * kRounds) {
+    u32 d[kRounds][4];
+    u32 dst[kRounds], n[kRounds];
+#pragma unroll
+    for (u32 r = 0; r < kRounds; r++) {
+      const u32 c = c0 + 64 * r + lane;
+      n[r] = 0;
+      dst[r] = 0;
+      if (c < C) {
+        u32 lo = 0, hi = 64;  // last entry with cp <= c
+        while (hi - lo > 1) {
+          const u32 mid = (lo + hi) >> 1;
+          if (vt.cp[mid] <= c) lo = mid;
+          else hi = mid;
+        }
+        const u32 k = c - vt.cp[lo];
+        const GMEM u8* src = gbytes(vt.src[lo] + 16ull * k);
+        n[r] = min(16u, vt.vl[lo] - 16 * k);
+        dst[r] = vt.off[lo] + 16 * k;
+#pragma unroll
+        for (u32 t = 0; t < 4; t++) d[r][t] = gload_u32(src + 4 * t);
+      }
+    }
+#pragma unroll
+    for (u32 r = 0; r < kRounds; r++) {
+#pragma unroll
+      for (u32 t = 0; t < 4; t++) {
+        if (4 * t + 4 <= n[r]) {
+          buf.put32(dst[r] + 4 * t, d[r][t]);
+        } else {
+#pragma unroll
+          for (u32 b = 0; b < 4; b++)
+            if (4 * t + b < n[r]) buf.put(dst[r] + 4 * t + b, (u8)(d[r][t] >> (8 * b)));
+        }
+      }
+    }
+  }
+  wave_sync();
+}
 
 // Copy n bytes from the LDS stage (src, which agrees with dst mod 16) to dst.
 __device__ __forceinline__ void wave_copy_out(u8* dst, const uint4* stage4, u32 mis, u64 n) {
@@ -690,6 +780,7 @@ __global__ __launch_bounds__(kThreads) void k_write_blocks(Kept k, const u32* nb
   __shared__ u32 sl[4][256];
   __shared__ u32 z256[4][256];
   __shared__ uint4 stage_all[kWaves][(kStage + 32) / 16];
+  __shared__ ValueTab vtab[kWaves];
   if (st->out_bytes > out_cap || st->err) return;
   load_crc_tables(sl, z256);
   __syncthreads();
@@ -713,8 +804,8 @@ __global__ __launch_bounds__(kThreads) void k_write_blocks(Kept k, const u32* nb
     const u64 bytes = m0.bytes;
     if (bytes + 5 + 16 <= kStage) {
       const u32 mis = (u32)((u64)dst & 15);
-      assemble_block<NWMAX>(k, ep, b, m0.first, m0.n, bytes, LdsBuf{reinterpret_cast<u8*>(stage_all[wave]) + mis},
-                            sl, z256, st);
+      assemble_block<NWMAX>(k, ep, b, m0.first, m0.n, bytes,
+                            LdsBuf{reinterpret_cast<u8*>(stage_all[wave]), mis, &vtab[wave]}, sl, z256, st);
       wave_copy_out(dst, stage_all[wave], mis, bytes + 5);
     } else {
       assemble_block<NWMAX>(k, ep, b, m0.first, m0.n, bytes, GlobalBuf{dst}, sl, z256, st);
