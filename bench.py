@@ -34,6 +34,19 @@ from luda_amd import ffi, jobs  # noqa: E402
 
 METRIC = "compaction input GB/s (byte-identical SSTables) vs CPU; p50/p99 job latency"
 HBM_PEAK_GBPS = 8000.0
+# HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled
+# by a plain copy of known size) for the default workload; see the .md beside it.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_l1l2_256MiB.json")
+
+
+def measured_traffic(kernel, config, scale):
+    """HBM bytes per launch of `kernel` from the committed counter run, for
+    the workload that run measured (config 1 at full size), else None."""
+    if config != 1 or scale != 1.0 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        k = json.load(f).get("kernels", {}).get(kernel)
+    return k["bytes"] if k else None
 
 WORKLOADS = {
     0: "L0->L1 compaction: 4 overlapping L0 tables + 6 L1 tables, ~20 MiB, 16-byte keys mod 1e6, "
@@ -232,7 +245,7 @@ def main():
                      "achieved": round(achieved, 2) if achieved else None, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 5) if achieved else None,
                      "bytes_per_launch": per_launch_bytes, "ms_per_launch": round(per_launch_ms, 4),
-                     "traffic": None},
+                     "traffic": measured_traffic(top, args.config, args.scale)},
         "cpu_baseline": cpu,
         "parity": parity,
     }

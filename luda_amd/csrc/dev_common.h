@@ -418,6 +418,60 @@ __device__ __forceinline__ const GMEM u8* gbytes(u64 addr) { return (const GMEM 
 __device__ __forceinline__ u32 gload_u32(const GMEM u8* p) { return *(const GMEM u32_una*)p; }
 __device__ __forceinline__ u64 gload_u64(const GMEM u8* p) { return *(const GMEM u64_una*)p; }
 
+// ---------------------------------------------------------------------------
+// Wavefront scans on the VALU.  DPP moves a value between lanes without the
+// LDS crossbar that __shfl (ds_bpermute) uses: row_shr:1,2,4,8 scan within
+// rows of 16 lanes, row_bcast:15 and row_bcast:31 carry the row totals.  A
+// lane with no source (or in a row the step leaves alone) reads 0, which is
+// the identity of every operator scanned here.
+
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ u32 dpp_u32(u32 v) {
+  return (u32)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, false);
+}
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ u64 dpp_u64(u64 v) {
+  return (u64)dpp_u32<CTRL, ROW_MASK>((u32)v) | ((u64)dpp_u32<CTRL, ROW_MASK>((u32)(v >> 32)) << 32);
+}
+
+// Inclusive scan over the 64 lanes with an associative operator
+// op(earlier, later); `Mov` moves one lane's state (DPP on each word).
+template <typename S, typename Op>
+__device__ __forceinline__ S wave_incl_scan_op(S x, Op op) {
+  x = op(x.template dpp<0x111, 0xf>(), x);  // row_shr:1
+  x = op(x.template dpp<0x112, 0xf>(), x);  // row_shr:2
+  x = op(x.template dpp<0x114, 0xf>(), x);  // row_shr:4
+  x = op(x.template dpp<0x118, 0xf>(), x);  // row_shr:8
+  x = op(x.template dpp<0x142, 0xa>(), x);  // row_bcast:15 into rows 1 and 3
+  x = op(x.template dpp<0x143, 0xc>(), x);  // row_bcast:31 into rows 2 and 3
+  return x;
+}
+
+__device__ __forceinline__ u32 wave_incl_sum(u32 v) {
+  v += dpp_u32<0x111, 0xf>(v);
+  v += dpp_u32<0x112, 0xf>(v);
+  v += dpp_u32<0x114, 0xf>(v);
+  v += dpp_u32<0x118, 0xf>(v);
+  v += dpp_u32<0x142, 0xa>(v);
+  v += dpp_u32<0x143, 0xc>(v);
+  return v;
+}
+__device__ __forceinline__ u64 wave_incl_sum(u64 v) {
+  v += dpp_u64<0x111, 0xf>(v);
+  v += dpp_u64<0x112, 0xf>(v);
+  v += dpp_u64<0x114, 0xf>(v);
+  v += dpp_u64<0x118, 0xf>(v);
+  v += dpp_u64<0x142, 0xa>(v);
+  v += dpp_u64<0x143, 0xc>(v);
+  return v;
+}
+// Lane 63's value in every lane (a scalar read, no LDS).
+__device__ __forceinline__ u32 wave_last(u32 v) { return (u32)__builtin_amdgcn_readlane((int)v, 63); }
+__device__ __forceinline__ u64 wave_last(u64 v) {
+  return (u64)(u32)__builtin_amdgcn_readlane((int)(u32)v, 63) |
+         ((u64)(u32)__builtin_amdgcn_readlane((int)(u32)(v >> 32), 63) << 32);
+}
+
 // Grid-stride helpers.
 #define GRID_STRIDE(i, n) for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < (u64)(n); i += (u64)gridDim.x * blockDim.x)
 

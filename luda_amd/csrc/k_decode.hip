@@ -17,15 +17,7 @@ namespace {
 
 constexpr u32 kStage = 8192;  // bytes of a data block staged in LDS
 
-__device__ __forceinline__ u32 wave_incl_scan_u32(u32 v) {
-  const u32 lane = threadIdx.x & 63;
-#pragma unroll
-  for (u32 d = 1; d < 64; d <<= 1) {
-    u32 o = __shfl_up(v, d, 64);
-    if (lane >= d) v += o;
-  }
-  return v;
-}
+__device__ __forceinline__ u32 wave_incl_scan_u32(u32 v) { return wave_incl_sum(v); }
 
 // One thread per input table: footer -> index block handle.
 __global__ void k_files(const FileDesc* files, u32 nfiles, FileInfo* info, u32* nblk, DevStatus* st) {
@@ -236,7 +228,7 @@ __device__ __forceinline__ void decode_block(const Src& src, u32 n, u64 gp, u64 
     const u32 start = w >> 8;
     const u32 end = g + 1 < nr ? (src_ld32(src, roff + 4 * (g + 1)) >> 8) : roff;
     const u32 incl = wave_incl_scan_u32(cnt);
-    const u32 chunk_total = __shfl(incl, 63, 64);
+    const u32 chunk_total = wave_last(incl);
     u64 e = ebase + incl - cnt;
     if (cnt > 0) {
       u32 q = start;
@@ -366,6 +358,23 @@ __device__ __forceinline__ bool win_entry_header(const Src& s, u32& off, u32 lim
   return (u64)(lim - off) >= (u64)ns + vl;
 }
 
+// Scan state of decode_entries: the internal key's words and which of their
+// bytes are known.  (0, 0) is the identity.
+template <u32 NWMAX>
+struct KeyBytes {
+  u64 m[NWMAX], v[NWMAX];
+  template <int CTRL, int ROW_MASK>
+  __device__ __forceinline__ KeyBytes dpp() const {
+    KeyBytes r;
+#pragma unroll
+    for (u32 w = 0; w < NWMAX; w++) {
+      r.m[w] = dpp_u64<CTRL, ROW_MASK>(m[w]);
+      r.v[w] = dpp_u64<CTRL, ROW_MASK>(v[w]);
+    }
+    return r;
+  }
+};
+
 template <u32 NWMAX, typename Src>
 __device__ __forceinline__ void decode_entries(const Src& src, u32 n, u64 gp, u64 ebase, u32* gpre, u32 max_nr,
                                                const Planes& K, u64* vptr, u32* vlen, DevStatus* st, u32& maxu) {
@@ -385,7 +394,7 @@ __device__ __forceinline__ void decode_entries(const Src& src, u32 n, u64 gp, u6
     const u32 cnt = (g < nr && roff > 0) ? (win_ld32(src, roff + 4 * g) & 0xff) : 0;
     const u32 incl = wave_incl_scan_u32(cnt);
     if (g < nr) gpre[g] = total + incl - cnt;
-    total += __shfl(incl, 63, 64);
+    total += wave_last(incl);
   }
   if (lane == 0) gpre[nr] = total;
   __syncthreads();
@@ -435,15 +444,26 @@ __device__ __forceinline__ void decode_entries(const Src& src, u32 n, u64 gp, u6
       }
     }
     // every byte from the latest entry that stored it
-#pragma unroll
-    for (u32 d = 1; d < 64; d <<= 1) {
+    {
+      KeyBytes<NWMAX> x;
 #pragma unroll
       for (u32 w = 0; w < NWMAX; w++) {
-        const u64 om = __shfl_up(m[w], d, 64), ov = __shfl_up(v[w], d, 64);
-        if (lane >= d) {
-          v[w] = (ov & ~m[w]) | v[w];
-          m[w] |= om;
+        x.m[w] = m[w];
+        x.v[w] = v[w];
+      }
+      x = wave_incl_scan_op(x, [](const KeyBytes<NWMAX>& a, const KeyBytes<NWMAX>& bb) {
+        KeyBytes<NWMAX> r;
+#pragma unroll
+        for (u32 w = 0; w < NWMAX; w++) {
+          r.v[w] = (a.v[w] & ~bb.m[w]) | bb.v[w];
+          r.m[w] = a.m[w] | bb.m[w];
         }
+        return r;
+      });
+#pragma unroll
+      for (u32 w = 0; w < NWMAX; w++) {
+        m[w] = x.m[w];
+        v[w] = x.v[w];
       }
     }
 #pragma unroll
@@ -452,7 +472,7 @@ __device__ __forceinline__ void decode_entries(const Src& src, u32 n, u64 gp, u6
       m[w] |= cm[w];
     }
     // shared may not exceed the previous key (DecodeEntry's key_.size() >= shared)
-    const u32 up = __shfl_up(ilen, 1, 64);
+    const u32 up = dpp_u32<0x138, 0xf>(ilen);  // wave_shr:1: the previous lane's
     const u32 prev_ilen = lane == 0 ? cilen : up;
     if (ok && r > 0 && sh > prev_ilen) ok = false;
     if (ok && ilen < 8) {
@@ -497,10 +517,10 @@ __device__ __forceinline__ void decode_entries(const Src& src, u32 n, u64 gp, u6
     // carry the chunk's last entry
 #pragma unroll
     for (u32 w = 0; w < NWMAX; w++) {
-      cm[w] = __shfl(m[w], 63, 64);
-      cv[w] = __shfl(v[w], 63, 64);
+      cm[w] = wave_last(m[w]);
+      cv[w] = wave_last(v[w]);
     }
-    cilen = __shfl(ilen, 63, 64);
+    cilen = wave_last(ilen);
   }
   if (bad) set_err(st, ERR_CORRUPT, SITE_GROUP_COUNT);
   __syncthreads();

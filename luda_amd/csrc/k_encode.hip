@@ -27,15 +27,7 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-__device__ __forceinline__ u32 wave_incl_scan(u32 v) {
-  const u32 lane = threadIdx.x & 63;
-#pragma unroll
-  for (u32 d = 1; d < 64; d <<= 1) {
-    u32 o = __shfl_up(v, d, 64);
-    if (lane >= d) v += o;
-  }
-  return v;
-}
+__device__ __forceinline__ u32 wave_incl_scan(u32 v) { return wave_incl_sum(v); }
 
 __device__ __forceinline__ u32 wave_max(u32 v) {
 #pragma unroll
@@ -599,7 +591,7 @@ __device__ __forceinline__ void assemble_block(const Kept& k, const EncParams& e
       my_vsrc = k.vptr[q];
     }
     if constexpr (Buf::kLds) copy_values_chunked(buf, act, my_voff, my_vsrc, my_vl);
-    base += __shfl(incl, 63, 64);
+    base += wave_last(incl);
   }
   if (lane == 0) {
     buf.put32(dend + 4ull * ngroups, ngroups);
@@ -688,7 +680,7 @@ __device__ __forceinline__ void copy_values_chunked(const LdsBuf& buf, bool act,
   ValueTab& vt = *buf.vt;
   const u32 nch = act ? (vl + 15) >> 4 : 0;
   const u32 cincl = wave_incl_scan(nch);
-  const u32 C = __shfl(cincl, 63, 64);
+  const u32 C = wave_last(cincl);
   vt.cp[lane] = cincl - nch;
   vt.off[lane] = (u32)voff;
   vt.vl[lane] = vl;

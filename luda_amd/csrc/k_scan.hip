@@ -16,13 +16,7 @@ constexpr u32 kTopThreads = 1024;
 
 template <typename T>
 __device__ __forceinline__ T wave_incl_scan(T v) {
-  const u32 lane = threadIdx.x & 63;
-#pragma unroll
-  for (u32 d = 1; d < 64; d <<= 1) {
-    T o = __shfl_up(v, d, 64);
-    if (lane >= d) v += o;
-  }
-  return v;
+  return wave_incl_sum(v);
 }
 
 // Exclusive scan across the workgroup; returns this thread's exclusive prefix
