@@ -296,6 +296,15 @@ __device__ __forceinline__ u32 wave_crc32c_rows_lds(u64 L, u64 r0, u64 r1, const
   return __shfl(s, 0, 64);
 }
 
+// crc_append_zeros with the first nz levels' maps read from zt (copies of
+// g_crc.zeros[0 .. nz)), the longer ones from g_crc.
+__device__ __forceinline__ u32 crc_append_zeros_tab(u32 s, u64 n, const u32 (*zt)[4][256], u32 nz) {
+  u64 w = n >> 2;
+  for (u32 j = 0; j < CRC_ZERO_LEVELS && w; j++, w >>= 1)
+    if (w & 1) s = j < nz ? crc_shift(zt[j], s) : crc_shift(g_crc.zeros[j], s);
+  return s;
+}
+
 // Raw register s followed by n zero bytes (n a multiple of 4, < 16 MiB).
 __device__ __forceinline__ u32 crc_append_zeros(u32 s, u64 n) {
   u64 w = n >> 2;
@@ -418,6 +427,56 @@ __device__ __forceinline__ const GMEM u8* gbytes(u64 addr) { return (const GMEM 
 __device__ __forceinline__ u32 gload_u32(const GMEM u8* p) { return *(const GMEM u32_una*)p; }
 __device__ __forceinline__ u64 gload_u64(const GMEM u8* p) { return *(const GMEM u64_una*)p; }
 
+// wave_crc32c_rows for a message at p in global memory, readable for its L
+// bytes only: rows away from the ends read their four bytes per lane with
+// one (unaligned) dword load, the others byte by byte at clamped indices.
+// zt[j] (j < 6) are copies of g_crc.zeros[j] for the join of the lanes.
+__device__ __forceinline__ u32 wave_crc32c_rows_global(u64 L, u64 r0, u64 r1, const GMEM u8* p,
+                                                       const u32 (*slice)[256], const u32 (*z256)[256],
+                                                       const u32 (*zt)[4][256]) {
+  const u32 lane = threadIdx.x & 63;
+  const u64 P = (256 - (L & 255)) & 255;
+  u32 s = 0;
+  constexpr u32 kRows = 8;
+  for (u64 rb = r0; rb < r1; rb += kRows) {
+    u32 wv[kRows];
+#pragma unroll
+    for (u32 rr = 0; rr < kRows; rr++) {
+      const long long x = (long long)(256 * (rb + rr) + 4 * lane) - (long long)P;
+      u32 v = 0;
+      if (rb + rr < r1) {
+        if (x >= 4 && (u64)x + 4 <= L) {
+          v = gload_u32(p + x);
+        } else {
+#pragma unroll
+          for (int t = 0; t < 4; t++) {
+            const long long xt = x + t;
+            const bool in = xt >= 0 && (u64)xt < L;
+            u32 bt = in ? (u32)p[in ? xt : 0] : 0u;
+            if (xt >= 0 && xt < 4) bt ^= 0xffu;
+            v |= bt << (8 * t);
+          }
+        }
+      }
+      wv[rr] = v;
+    }
+#pragma unroll
+    for (u32 rr = 0; rr < kRows; rr++) {
+      if (rb + rr < r1) {
+        const u32 v = wv[rr];
+        const u32 w = slice[3][v & 0xff] ^ slice[2][(v >> 8) & 0xff] ^ slice[1][(v >> 16) & 0xff] ^ slice[0][v >> 24];
+        s = crc_shift(z256, s) ^ w;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    u32 other = __shfl_down(s, 1u << j, 64);
+    if ((lane & ((2u << j) - 1)) == 0) s = crc_shift(zt[j], s) ^ other;
+  }
+  return __shfl(s, 0, 64);
+}
+
 // ---------------------------------------------------------------------------
 // Wavefront scans on the VALU.  DPP moves a value between lanes without the
 // LDS crossbar that __shfl (ds_bpermute) uses: row_shr:1,2,4,8 scan within
@@ -454,6 +513,15 @@ __device__ __forceinline__ u32 wave_incl_sum(u32 v) {
   v += dpp_u32<0x118, 0xf>(v);
   v += dpp_u32<0x142, 0xa>(v);
   v += dpp_u32<0x143, 0xc>(v);
+  return v;
+}
+__device__ __forceinline__ u32 wave_incl_max(u32 v) {
+  v = max(v, dpp_u32<0x111, 0xf>(v));
+  v = max(v, dpp_u32<0x112, 0xf>(v));
+  v = max(v, dpp_u32<0x114, 0xf>(v));
+  v = max(v, dpp_u32<0x118, 0xf>(v));
+  v = max(v, dpp_u32<0x142, 0xa>(v));
+  v = max(v, dpp_u32<0x143, 0xc>(v));
   return v;
 }
 __device__ __forceinline__ u64 wave_incl_sum(u64 v) {

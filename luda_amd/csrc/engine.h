@@ -175,12 +175,14 @@ struct EncBufs {
   u64* f_dend;     // data end offset
   u64* f_fbytes;   // filter data bytes
   u64* f_ibytes;   // index block size
+  u64* f_fbase;    // f_dend - (filter data offset of the table's first window)
   u64* f_off;      // table size, then (scanned in place) table offset in the output
   FileMeta* meta;
   u8* keys;        // 2 x kw bytes per table: smallest, largest
   // per filter window (nw + 1 entries)
   u32* w_keys;
   u32* w_first;
+  u32* w_file;     // its table (windows with keys only)
   u64* w_fsz;      // filter bytes, then (scanned in place) offset in the filter data
 };
 
@@ -204,9 +206,7 @@ void iota_u32(KTimer& T, u32* p, u64 n);
 // k_decode.hip
 void launch_files(KTimer& T, const FileDesc* files, u32 nfiles, FileInfo* info, u32* nblk, DevStatus* st);
 void launch_index(KTimer& T, const FileDesc* files, const FileInfo* info, const u32* blk_base, u32 nfiles,
-                  BlockDesc* blks, DevStatus* st);
-void launch_blkscan(KTimer& T, const FileDesc* files, const BlockDesc* blks, const u32* nb_dev, u32 nb_cap,
-                    u32* blk_nent, DevStatus* st);
+                  u32 nb_cap, BlockDesc* blks, u32* blk_nent, DevStatus* st);
 void launch_file_ebase(KTimer& T, const u32* blk_base, const u32* blk_ebase, u32 nfiles, const u32* n_entries,
                        u32* file_ebase);
 void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 nb, const u32* blk_ebase, Planes K,

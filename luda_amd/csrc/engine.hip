@@ -296,11 +296,13 @@ EncodeArrays carve_encode(Arena& a, const EncodeSetup& s, u32 W, bool with_kept)
   b.f_dend = a.take<u64>(nf + 1);
   b.f_fbytes = a.take<u64>(nf + 1);
   b.f_ibytes = a.take<u64>(nf + 1);
+  b.f_fbase = a.take<u64>(nf + 1);
   b.f_off = a.take<u64>(nf + 1);
   b.meta = a.take<FileMeta>(nf + 1);
   b.keys = a.take<u8>((nf + 1) * 2ull * s.ep.kw);
   b.w_keys = a.take<u32>(nw + 1);
   b.w_first = a.take<u32>(nw + 1);
+  b.w_file = a.take<u32>(nw + 1);
   b.w_fsz = a.take<u64>(nw + 1);
   r.ftab_b = a.take<u16>(s.ob.table_entries);
   r.ent_b = a.take<u32>(s.ob.entry_entries);
@@ -484,7 +486,7 @@ u64 kernel_bytes(const std::string& name, const luda_stats& s, u32 W, u64 rounds
   if (name == "write_blocks") return 2 * s.data_block_bytes;          // entries in, blocks out
   if (name == "write_filters") return m * (8ull * W + 4) + m * 2;     // keys in, ~10 bits per key out
   if (name == "write_tail") return 2 * tail;                          // filter data in (CRC), tail blocks out
-  if (name == "index") return s.blocks_in * 40;                       // index entries in, block handles out
+  if (name == "index") return s.blocks_in * 56;  // index entries and restart arrays in, handles and counts out
   return 0;
 }
 
@@ -691,12 +693,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
   fill_u32(T, d_nblk, 0, nfiles + 1);
   launch_files(T, d_files, nfiles, d_info, d_nblk, c->st);
   scan_u32(T, d_nblk, d_bbase, nullptr, nfiles + 1, nfiles + 1, nullptr, tmpA);
-  {
-    // n_blocks_in = blk_base[nfiles]
-    HIPCHK(hipMemcpyAsync(&c->st->n_blocks_in, d_bbase + nfiles, 4, hipMemcpyDeviceToDevice, c->s));
-  }
-  launch_index(T, d_files, d_info, d_bbase, nfiles, d_blks, c->st);
-  launch_blkscan(T, d_files, d_blks, &c->st->n_blocks_in, (u32)nb_cap, d_nent, c->st);
+  launch_index(T, d_files, d_info, d_bbase, nfiles, (u32)nb_cap, d_blks, d_nent, c->st);  // sets n_blocks_in
   scan_u32(T, d_nent, d_ebase, &c->st->n_blocks_in, 0, (u32)nb_cap, &c->st->n_entries, tmpA);
   launch_file_ebase(T, d_bbase, d_ebase, nfiles, &c->st->n_entries, d_febase);
   DevStatus stA;

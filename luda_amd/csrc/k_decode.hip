@@ -68,102 +68,204 @@ __global__ void k_files(const FileDesc* files, u32 nfiles, FileInfo* info, u32* 
   }
 }
 
-// One workgroup per table: each index entry (a restart of its own, the index
-// block's restart interval being 1, table/table_builder.cc:31) -> block handle.
-__global__ void k_index(const FileDesc* files, const FileInfo* info, const u32* blk_base, u32 nfiles,
-                        BlockDesc* blks, DevStatus* st) {
-  for (u32 f = blockIdx.x; f < nfiles; f += gridDim.x) {
-    const u32 nb = blk_base[f + 1] - blk_base[f];
-    if (nb == 0) continue;
+// 24 bytes of global memory as three little-endian words, and their bytes:
+// the varints of an index entry or handle are decoded from registers after
+// one round of loads instead of one load per byte.
+struct Win24 {
+  u64 w0, w1, w2;
+};
+__device__ __forceinline__ Win24 load_win24(const GMEM u8* p) { return {gload_u64(p), gload_u64(p + 8), gload_u64(p + 16)}; }
+__device__ __forceinline__ u32 win_byte(const Win24& w, u32 i) {
+  const u64 x = i < 8 ? w.w0 : (i < 16 ? w.w1 : w.w2);
+  return (u32)(x >> (8 * (i & 7))) & 0xff;
+}
+// GetVarint32Ptr / GetVarint64Ptr (util/coding.cc:121-165) on bytes [i, avail)
+// of the window (avail <= 24); returns the position after it, or ~0u.
+__device__ __forceinline__ u32 win_varint32(const Win24& w, u32 i, u32 avail, u32* v) {
+  u32 r = 0;
+  for (u32 shift = 0; shift <= 28 && i < avail; shift += 7) {
+    const u32 b = win_byte(w, i++);
+    r |= (b & 127) << shift;
+    if (!(b & 128)) {
+      *v = r;
+      return i;
+    }
+  }
+  return ~0u;
+}
+__device__ __forceinline__ u32 win_varint64(const Win24& w, u32 i, u32 avail, u64* v) {
+  u64 r = 0;
+  for (u32 shift = 0; shift <= 63 && i < avail; shift += 7) {
+    const u64 b = win_byte(w, i++);
+    r |= (b & 127) << shift;
+    if (!(b & 128)) {
+      *v = r;
+      return i;
+    }
+  }
+  return ~0u;
+}
+
+// The last f < n with a[f] <= j (a[0] = 0), and a[f] in *base, from LDS or
+// from global memory: two functions whose loads all sit inside their own
+// loops, so that no load can be merged into one through a generic pointer.
+__device__ __forceinline__ u32 last_le_lds(const u32* a, u32 n, u32 j, u32* base) {
+  u32 lo = 0, hi = n, v = 0;
+  while (hi - lo > 1) {
+    const u32 mid = (lo + hi) >> 1;
+    const u32 x = a[mid];
+    if (x <= j) {
+      lo = mid;
+      v = x;
+    } else {
+      hi = mid;
+    }
+  }
+  *base = v;
+  return lo;
+}
+__device__ __forceinline__ u32 last_le_global(const u32* a, u32 n, u32 j, u32* base) {
+  const GMEM u32* g = (const GMEM u32*)a;
+  u32 lo = 0, hi = n, v = 0;
+  while (hi - lo > 1) {
+    const u32 mid = (lo + hi) >> 1;
+    const u32 x = g[mid];
+    if (x <= j) {
+      lo = mid;
+      v = x;
+    } else {
+      hi = mid;
+    }
+  }
+  *base = v;
+  return lo;
+}
+
+// One thread per data block of any table: its index entry (a restart of its
+// own, the index block's restart interval being 1, table/table_builder.cc:31)
+// -> block handle, then the block's restart array -> entry count (the sum of
+// the count bytes), with the structural checks the decoder relies on.  Also
+// records the longest user key found at a restart point and the largest
+// block and restart array, which size the decoder's LDS.  Thread j finds its
+// table by a binary search of blk_base (staged in LDS when it is short).
+// Reads that run past an entry stay inside its table: an index entry is
+// followed by at least the restart array, the trailer and the footer (24
+// bytes are read), and a data block by the index block and footer (16 bytes
+// are read at a restart point).
+constexpr u32 kIndexStage = 2048;
+
+__global__ __launch_bounds__(256) void k_index(const FileDesc* files, const FileInfo* info, const u32* blk_base,
+                                               u32 nfiles, u32 nb_cap, BlockDesc* blks, u32* blk_nent,
+                                               DevStatus* st) {
+  __shared__ u32 s_bb[kIndexStage];
+  const u32 total = blk_base[nfiles];
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->n_blocks_in = total > nb_cap ? 0 : total;
+  if (total > nb_cap) {  // more handles than the tables' sizes allow
+    if (blockIdx.x == 0 && threadIdx.x == 0) set_err(st, ERR_CORRUPT, SITE_INDEX);
+    return;
+  }
+  const bool staged = nfiles + 1 <= kIndexStage;
+  if (staged) {
+    for (u32 i = threadIdx.x; i <= nfiles; i += blockDim.x) s_bb[i] = blk_base[i];
+    __syncthreads();
+  }
+  u32 maxr = 0, maxn = 0, maxg = 0;
+  GRID_STRIDE(jj, total) {
+    const u32 j = (u32)jj;
+    blk_nent[j] = 0;
+    u32 f, fb;
+    if (staged) f = last_le_lds(s_bb, nfiles, j, &fb);
+    else f = last_le_global(blk_base, nfiles, j, &fb);
+    const u32 jf = j - fb;
+    BlockDesc bd = {0, 0, f};
+    blks[j] = bd;
     const u8* base = files[f].base;
     const u64 size = files[f].size;
     const FileInfo fi = info[f];
-    const u8* ib = base + fi.ioff;
-    const u64 roff = fi.isize - 4 - 4ull * fi.nr;
-    for (u32 j = threadIdx.x; j < nb; j += blockDim.x) {
-      BlockDesc bd = {0, 0, f};
-      blks[blk_base[f] + j] = bd;
-      const u32 w = ld32(ib + roff + 4ull * j);
-      const u32 off = w >> 8;
-      if ((w & 0xff) != 1 || off >= roff) {
-        set_err(st, ERR_CORRUPT, SITE_INDEX);
-        continue;
-      }
-      const u8* p = ib + off;
-      const u8* lim = ib + roff;
-      u32 sh, ns, vl;
-      if (!(p = dec_varint32(p, lim, &sh)) || !(p = dec_varint32(p, lim, &ns)) || !(p = dec_varint32(p, lim, &vl)) ||
-          sh != 0 || (u64)(lim - p) < (u64)ns + vl) {
-        set_err(st, ERR_CORRUPT, SITE_INDEX);
-        continue;
-      }
-      const u8* h = p + ns;
-      const u8* hl = h + vl;
-      u64 boff, bsz;
-      if (!(h = dec_varint64(h, hl, &boff)) || !(h = dec_varint64(h, hl, &bsz)) || boff > size || bsz > size ||
-          boff + bsz + 5 > size || bsz > 0xffffffffull) {
-        set_err(st, ERR_CORRUPT, SITE_INDEX);
-        continue;
-      }
-      bd.addr = (u64)(base + boff);
-      bd.size = (u32)bsz;
-      blks[blk_base[f] + j] = bd;
+    const GMEM u8* ib = gbytes((u64)base + fi.ioff);
+    const u64 iroff = fi.isize - 4 - 4ull * fi.nr;
+    const u32 iw = gload_u32(ib + iroff + 4ull * jf);
+    const u32 ioff = iw >> 8;
+    if ((iw & 0xff) != 1 || ioff >= iroff) {
+      set_err(st, ERR_CORRUPT, SITE_INDEX);
+      continue;
     }
-  }
-}
+    const u64 iavail = iroff - ioff;  // bytes up to the restart array
+    const Win24 e = load_win24(ib + ioff);
+    const u32 ea = (u32)min<u64>(iavail, 24);
+    u32 ish = 0, ins = 0, ivl = 0, x = 0;
+    if ((x = win_varint32(e, 0, ea, &ish)) == ~0u || (x = win_varint32(e, x, ea, &ins)) == ~0u ||
+        (x = win_varint32(e, x, ea, &ivl)) == ~0u || ish != 0 || iavail - x < (u64)ins + ivl) {
+      set_err(st, ERR_CORRUPT, SITE_INDEX);
+      continue;
+    }
+    const Win24 hw = load_win24(ib + ioff + x + ins);
+    const u32 ha = min(ivl, 24u);
+    u64 boff = 0, bsz = 0;
+    u32 y = 0;
+    if ((y = win_varint64(hw, 0, ha, &boff)) == ~0u || (y = win_varint64(hw, y, ha, &bsz)) == ~0u ||
+        boff > size || bsz > size || boff + bsz + 5 > size || bsz > 0xffffffffull) {
+      set_err(st, ERR_CORRUPT, SITE_INDEX);
+      continue;
+    }
+    bd.addr = (u64)(base + boff);
+    bd.size = (u32)bsz;
+    blks[j] = bd;
 
-// One thread per data block: restart array -> entry count (sum of the count
-// bytes), with the structural checks the decoder relies on.  Also records the
-// longest user key found at a restart point.
-__global__ void k_blkscan(const FileDesc* files, const BlockDesc* blks, const u32* nb_dev, u32* blk_nent,
-                          DevStatus* st) {
-  const u32 nb = *nb_dev;
-  u32 maxr = 0, maxn = 0, maxg = 0;
-  GRID_STRIDE(b, nb) {
-    blk_nent[b] = 0;
-    const BlockDesc bd = blks[b];
-    const u8* p = reinterpret_cast<const u8*>(bd.addr);
-    const u32 n = bd.size;
+    const GMEM u8* p = gbytes(bd.addr);
+    const u32 n = (u32)bsz;
+    if (n < 4) {
+      set_err(st, p[n] == 1 ? ERR_UNSUPPORTED : ERR_CORRUPT, p[n] == 1 ? SITE_COMPRESSED : SITE_BLOCK_TRAILER);
+      continue;
+    }
+    const u32 nr = gload_u32(p + n - 4);
     if (p[n] == 1) {
       set_err(st, ERR_UNSUPPORTED, SITE_COMPRESSED);
       continue;
     }
-    if (n < 4) {
-      set_err(st, ERR_CORRUPT, SITE_BLOCK_TRAILER);
-      continue;
-    }
-    const u32 nr = ld32(p + n - 4);
     if (nr > (n - 4) / 4) {
       set_err(st, ERR_CORRUPT, SITE_BLOCK_TRAILER);
       continue;
     }
-    // sizes of the decoder's LDS: every block that it will look at
     maxn = max(maxn, n);
     maxg = max(maxg, nr);
     if (nr == 0) continue;  // Block::NewIterator -> NewEmptyIterator
     const u32 roff = n - 4 - 4 * nr;
-    u32 total = 0, prev = 0;
+    u32 cnt_total = 0, prev = 0;
     bool bad = false;
-    for (u32 k = 0; k < nr && !bad; k++) {
-      const u32 w = ld32(p + roff + 4 * k);
+    // restart words and the first bytes of their entries, two at a time
+    auto one = [&](u32 kk, u32 w, u64 h0, u64 h1) {
       const u32 off = w >> 8, cnt = w & 0xff;
       if (roff == 0) {  // no entries at all
         if (cnt) bad = true;
-        continue;
+        return;
       }
-      if ((k == 0 && off != 0) || (k > 0 && off <= prev) || off >= roff || cnt == 0) bad = true;
+      if ((kk == 0 && off != 0) || (kk > 0 && off <= prev) || off >= roff || cnt == 0) bad = true;
       prev = off;
-      total += cnt;
-      const u8* q = p + off;
-      u32 sh, ns;
-      const u8* r = dec_varint32(q, p + roff, &sh);
-      if (r && (r = dec_varint32(r, p + roff, &ns)) && sh == 0 && ns >= 8) maxr = max(maxr, ns - 8);
+      cnt_total += cnt;
+      // shared (0 at a restart) and key length: two varint32s, at most 10 bytes
+      const Win24 hwin = {h0, h1, 0};
+      const u32 avail = (u32)min<u64>(roff - off, 16);
+      u32 sh = 0, ns = 0, z = 0;
+      if (off < roff && (z = win_varint32(hwin, 0, avail, &sh)) != ~0u && win_varint32(hwin, z, avail, &ns) != ~0u &&
+          sh == 0 && ns >= 8)
+        maxr = max(maxr, ns - 8);
+    };
+    for (u32 k0 = 0; k0 < nr && !bad; k0 += 2) {
+      const bool two = k0 + 1 < nr;
+      const u32 wa = gload_u32(p + roff + 4 * k0);
+      const u32 wb = two ? gload_u32(p + roff + 4 * (k0 + 1)) : 0;
+      const bool ina = (wa >> 8) < roff, inb = two && (wb >> 8) < roff;
+      const u64 a0 = ina ? gload_u64(p + (wa >> 8)) : 0, a1 = ina ? gload_u64(p + (wa >> 8) + 8) : 0;
+      const u64 b0 = inb ? gload_u64(p + (wb >> 8)) : 0, b1 = inb ? gload_u64(p + (wb >> 8) + 8) : 0;
+      one(k0, wa, a0, a1);
+      if (two && !bad) one(k0 + 1, wb, b0, b1);
     }
     if (bad) {
       set_err(st, ERR_CORRUPT, SITE_BLOCK_TRAILER);
       continue;
     }
-    blk_nent[b] = total;
+    blk_nent[j] = cnt_total;
   }
   if (maxr) atomicMax(&st->max_rulen, maxr);
   if (maxn) atomicMax(&st->max_blk, maxn);
@@ -381,8 +483,8 @@ __device__ __forceinline__ void decode_entries(const Src& src, u32 n, u64 gp, u6
   const u32 lane = threadIdx.x & 63;
   const u32 NW = K.W + 1;  // words of the internal key: user key words + the tag
   const u32 nr = n >= 4 ? win_ld32(src, n - 4) : 0;
-  if (nr == 0 || nr > (n - 4) / 4) return;  // k_blkscan reported it
-  if (nr > max_nr) {  // not what k_blkscan saw: gpre holds max_nr + 1 entries
+  if (nr == 0 || nr > (n - 4) / 4) return;  // k_index reported it
+  if (nr > max_nr) {  // not what k_index saw: gpre holds max_nr + 1 entries
     if (lane == 0) set_err(st, ERR_CORRUPT, SITE_BLOCK_TRAILER);
     return;
   }
@@ -685,12 +787,9 @@ void launch_files(KTimer& T, const FileDesc* files, u32 nfiles, FileInfo* info, 
   KLAUNCH(T, "files", k_files, grid_for(nfiles, 64), 64, 0, files, nfiles, info, nblk, st);
 }
 void launch_index(KTimer& T, const FileDesc* files, const FileInfo* info, const u32* blk_base, u32 nfiles,
-                  BlockDesc* blks, DevStatus* st) {
-  KLAUNCH(T, "index", k_index, nfiles < 4096 ? nfiles : 4096, 256, 0, files, info, blk_base, nfiles, blks, st);
-}
-void launch_blkscan(KTimer& T, const FileDesc* files, const BlockDesc* blks, const u32* nb_dev, u32 nb_cap,
-                    u32* blk_nent, DevStatus* st) {
-  KLAUNCH(T, "blkscan", k_blkscan, grid_for(nb_cap, 256), 256, 0, files, blks, nb_dev, blk_nent, st);
+                  u32 nb_cap, BlockDesc* blks, u32* blk_nent, DevStatus* st) {
+  KLAUNCH(T, "index", k_index, grid_for(nb_cap, 256), 256, 0, files, info, blk_base, nfiles, nb_cap, blks, blk_nent,
+          st);
 }
 void launch_file_ebase(KTimer& T, const u32* blk_base, const u32* blk_ebase, u32 nfiles, const u32* n_entries,
                        u32* file_ebase) {

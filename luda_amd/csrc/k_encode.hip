@@ -162,79 +162,157 @@ __global__ void k_entry_sizes(Kept k, const u32* m_dev, EncBufs b, DevStatus* st
 
 // The block that TableBuilder would build starting at entry q: entries are
 // added until CurrentSizeEstimate() >= block_size (table_builder.cc:119-123),
-// a table cut is due, or the stream ends.  A workgroup takes 256 starts and
-// stages the sizes of the entries they can reach (kBnextSpan past the last
-// start) in LDS; a walk that runs past the staged span reads on from global
-// memory.
-constexpr u32 kBnextSpan = 768;
-constexpr u32 kBnextTile = 256 + kBnextSpan + 4;
+// a table cut is due, or the stream ends.  With entries [q, y) in a block
+// starting at q, the estimate is
+//   S(q, y) = sum of sznr over [q, y) + sum of (szr - sznr) over its restart
+//             points q, q + R, ... + 4 * (restart points) + 4,
+// which only grows with y.  A workgroup takes 1024 starts (4 consecutive ones
+// per thread) and stages kBnSpan entries past them in LDS as a prefix sum of
+// sznr (with the count of forced entries packed above it) and the restart
+// surcharges.  Per start, the prefix alone bounds the cut from above
+// (S >= prefix difference + 8); a backward walk then finds the exact first y
+// with S >= block_size.  The bound only moves forward from one start to the
+// next.  A block that runs past the staged span is finished by walking the
+// entries in global memory.
+constexpr u32 kBnStarts = 1024;
+constexpr u32 kBnSpan = 768;
+constexpr u32 kBnTile = kBnStarts + kBnSpan;     // staged entries
+constexpr u32 kBnPer = (kBnTile + 255) / 256;    // staged entries per thread
+constexpr u64 kCfOne = 1ull << 48;               // one forced entry in the packed prefix
+constexpr u64 kByteMask = kCfOne - 1;
 __global__ __launch_bounds__(256) void k_bnext(Kept k, const u32* m_dev, EncParams ep, EncBufs b, DevStatus* st) {
-  // per staged entry: szr | sznr << 31 | forced << 63 (sizes < 2^31, checked)
-  __shared__ u64 s_pk[kBnextTile];
+  __shared__ u64 sP[kBnTile + 1];  // sP[x]: sznr of staged entries [0, x), plus kCfOne per forced one
+  __shared__ u32 sD[kBnTile];      // szr - sznr: what a restart point adds (>= 0)
+  __shared__ u64 s_wsum[4];
   const u64 m = *m_dev;
   const u32 R = ep.interval;
+  const u64 bs = ep.block_size;
+  const u32 tid = threadIdx.x, wave = tid >> 6;
   u32 smax = 0;
-  for (u64 q0 = (u64)blockIdx.x * 256; q0 < m; q0 += (u64)gridDim.x * 256) {
-    const u64 tend = min(q0 + kBnextTile, m);  // staged entries [q0, tend)
-    {
-      constexpr u32 kPer = (kBnextTile + 255) / 256;
-      u32 r[kPer], nr[kPer], f[kPer];
+  for (u64 q0 = (u64)blockIdx.x * kBnStarts; q0 < m; q0 += (u64)gridDim.x * kBnStarts) {
+    const u32 lim = (u32)min((u64)kBnTile, m - q0);  // staged entries [q0, q0 + lim)
+    // stage: thread t owns staged entries [kBnPer t, kBnPer t + kBnPer)
+    // all loads first: a possible error report between them would order them
+    u32 r[kBnPer], nr[kBnPer], fo[kBnPer];
 #pragma unroll
-      for (u32 t = 0; t < kPer; t++) {
-        const u64 j = q0 + threadIdx.x + 256 * t;
-        const u64 jc = j < tend ? j : q0;
-        r[t] = b.szr[jc];
-        nr[t] = b.sznr[jc];
-        f[t] = k.forced[jc];
-      }
+    for (u32 i = 0; i < kBnPer; i++) {
+      const u32 x = kBnPer * tid + i;
+      const u64 qx = q0 + (x < lim ? x : 0);
+      r[i] = b.szr[qx];
+      nr[i] = b.sznr[qx];
+      fo[i] = k.forced[qx];
+    }
+    u64 v[kBnPer];
+    u64 tot = 0;
+    bool bad = false;
 #pragma unroll
-      for (u32 t = 0; t < kPer; t++) {
-        const u32 x = threadIdx.x + 256 * t;
-        if (x < kBnextTile) {
-          if (r[t] >= (1u << 31) || nr[t] >= (1u << 31)) set_err(st, ERR_UNSUPPORTED, SITE_BLOCK_ENTRIES);
-          s_pk[x] = q0 + x < tend ? ((u64)r[t] | ((u64)nr[t] << 31) | ((u64)(f[t] != 0) << 63)) : 0;
-        }
+    for (u32 i = 0; i < kBnPer; i++) {
+      const u32 x = kBnPer * tid + i;
+      u64 e = 0;
+      if (x < lim) {
+        bad |= r[i] >= (1u << 31) || nr[i] >= (1u << 31) || r[i] < nr[i];
+        e = (u64)nr[i] + (fo[i] ? kCfOne : 0);
+        sD[x] = r[i] - nr[i];
       }
+      v[i] = e;
+      tot += e;
+    }
+    if (bad) set_err(st, ERR_UNSUPPORTED, SITE_BLOCK_ENTRIES);
+    const u64 incl = wave_incl_sum(tot);
+    if ((tid & 63) == 63) s_wsum[wave] = incl;
+    __syncthreads();
+    u64 run = incl - tot;
+    for (u32 w = 0; w < wave; w++) run += s_wsum[w];
+    if (tid == 0) sP[0] = 0;
+#pragma unroll
+    for (u32 i = 0; i < kBnPer; i++) {
+      const u32 x = kBnPer * tid + i;
+      run += v[i];
+      if (x < lim) sP[x + 1] = run;
     }
     __syncthreads();
-    const u64 q = q0 + threadIdx.x;
-    if (q < m) {
-      // acc: entry bytes so far; ng: restart points so far; rpos: position
-      // of the next entry in its restart group (no division in the walk)
-      u64 acc = 0, j = q;
-      u32 ng = 0, rpos = 0;
-      bool done = false;
-      // within the staged span, four entries per LDS round trip ...
-      while (!done && j + 5 <= tend) {
-        const u32 x = (u32)(j - q0);
-        u64 pk[5];
-#pragma unroll
-        for (u32 t = 0; t < 5; t++) pk[t] = s_pk[x + t];
-#pragma unroll
-        for (u32 t = 0; t < 4; t++) {
-          if (!done) {
-            const bool rs = rpos == 0;
-            acc += rs ? (pk[t] & 0x7fffffffu) : ((pk[t] >> 31) & 0x7fffffffu);
-            ng += rs;
-            rpos = rpos + 1 == R ? 0 : rpos + 1;
-            j++;
-            done = acc + 4ull * ng + 4 >= ep.block_size || j >= m || (pk[t + 1] >> 63) != 0;
-          }
+    u32 ub = 0;  // the upper bound of the previous start
+    for (u32 s = 0; s < 4; s++) {
+      const u32 x = 4 * tid + s;
+      const u64 q = q0 + x;
+      if (x >= kBnStarts || q >= m) break;
+      const u64 px = sP[x];
+      const u64 bx = px & kByteMask;
+      // smallest y in (x, lim] with bytes(x, y) + 8 >= bs, else lim + 1
+      u32 y;
+      if (s == 0) {
+        u32 lo = x + 1, hi = lim + 1;
+        while (lo < hi) {
+          const u32 mid = (lo + hi) >> 1;
+          if ((sP[mid] & kByteMask) - bx + 8 >= bs) hi = mid;
+          else lo = mid + 1;
         }
+        y = lo;
+      } else {
+        y = max(ub, x + 1);
+        while (y <= lim && (sP[y] & kByteMask) - bx + 8 < bs) y++;
       }
-      // ... then from global memory: the two never share a pointer
-      while (!done) {
-        const bool rs = rpos == 0;
-        acc += rs ? b.szr[j] : b.sznr[j];
-        ng += rs;
-        rpos = rpos + 1 == R ? 0 : rpos + 1;
-        j++;
-        done = acc + 4ull * ng + 4 >= ep.block_size || j >= m || k.forced[j] != 0;
+      ub = y;
+      // exact S(x, y0) for y0 = min(y, lim); g restart points
+      const u32 y0 = min(y, lim);
+      const u32 g = (y0 - x + R - 1) / R;
+      u64 sur = 0;
+      for (u32 t = 0; t < g; t++) sur += sD[x + t * R];
+      u64 S = (sP[y0] & kByteMask) - bx + sur + 4ull * g + 4;
+      u32 end;       // the cut, when it lies within the staged entries
+      bool found = S >= bs;
+      if (found) {
+        // walk back while the block one entry shorter still reaches bs
+        end = y0;
+        u32 gg = g;
+        while (end > x + 1) {
+          const u32 e = end - 1;  // the entry that would be dropped
+          const bool rs = (e - x) % R == 0;
+          const u64 sz = (sP[e + 1] & kByteMask) - (sP[e] & kByteMask) + (rs ? (u64)sD[e] + 4 : 0);
+          if (S - sz < bs) break;
+          S -= sz;
+          gg -= rs;
+          end = e;
+        }
+      } else {
+        end = lim;  // not within the staged entries
       }
-      const u64 cnt = j - q;
+      // a forced entry in (x, end) cuts first
+      const u64 cf0 = sP[x + 1] >> 48;
+      if (end > x + 1 && (sP[end] >> 48) > cf0) {
+        u32 lo = x + 1, hi = end - 1;  // smallest e in [x + 1, end) with forced[e]: sP[e + 1] counts it
+        while (lo < hi) {
+          const u32 mid = (lo + hi) >> 1;
+          if ((sP[mid + 1] >> 48) > cf0) hi = mid;
+          else lo = mid + 1;
+        }
+        end = lo;
+        const u32 ge = (end - x + R - 1) / R;
+        u64 se = 0;
+        for (u32 t = 0; t < ge; t++) se += sD[x + t * R];
+        S = (sP[end] & kByteMask) - bx + se + 4ull * ge + 4;
+        found = true;
+      }
+      u64 j = q0 + end;
+      if (!found && j < m) {
+        // past the staged entries: carry on from global memory
+        const u32 cnt = end - x;
+        u64 acc = S - 4ull * g - 4;
+        u32 ng = g, rpos = cnt % R;
+        bool done = k.forced[j] != 0;
+        while (!done) {
+          const bool rs = rpos == 0;
+          acc += rs ? b.szr[j] : b.sznr[j];
+          ng += rs;
+          rpos = rpos + 1 == R ? 0 : rpos + 1;
+          j++;
+          done = acc + 4ull * ng + 4 >= bs || j >= m || k.forced[j] != 0;
+        }
+        S = acc + 4ull * ng + 4;
+      }
       b.bnext[q] = (u32)j;
-      b.bbytes[q] = (u32)(acc + 4ull * ng + 4);
-      smax = max(smax, (u32)cnt);
+      b.bbytes[q] = (u32)S;
+      smax = max(smax, (u32)(j - q));
     }
     __syncthreads();
   }
@@ -371,6 +449,7 @@ __global__ void k_window_counts(const u32* nbo_dev, const u32* nf_dev, const u32
     if (w >= nw) continue;
     atomicAdd(&b.w_keys[w], b.ob_n[i]);
     atomicMin(&b.w_first[w], b.ob_first[i]);
+    b.w_file[w] = b.ob_file[i];  // every block of the window writes the same table
   }
 }
 
@@ -409,6 +488,7 @@ __global__ void k_file_sizes(const u32* nf_dev, EncParams ep, EncBufs b) {
       const u32 w0 = b.f_win[f], w1 = b.f_win[f + 1];
       const u64 fbytes = b.w_fsz[w1] - b.w_fsz[w0];
       b.f_fbytes[f] = fbytes;
+      b.f_fbase[f] = dend - b.w_fsz[w0];
       fb = fbytes + 4ull * (w1 - w0) + 5;
       size += fb + 5;
     }
@@ -433,16 +513,23 @@ __device__ __forceinline__ void load_crc_tables(u32 (*sl)[256], u32 (*z)[256]) {
 // The internal key user_key || fixed64(tag) of entry q as NWMAX big-endian
 // words (bytes past the key are zero); the planes hold the user key's words.
 template <u32 NWMAX>
-__device__ __forceinline__ void ikey_words(const Planes& K, u64 q, u32 ulen, u64 tag, u64 (&ik)[NWMAX]) {
+__device__ __forceinline__ void ikey_words_of(const u64 (&uk)[NWMAX], u32 ulen, u64 tag, u64 (&ik)[NWMAX]) {
   const u64 tb = __builtin_bswap64(tag);
   const u32 wi = ulen >> 3, sft = (ulen & 7) * 8;
 #pragma unroll
   for (u32 w = 0; w < NWMAX; w++) {
-    u64 x = w < K.W ? K.P[(u64)w * K.stride + q] : 0;
+    u64 x = uk[w];
     if (w == wi) x |= tb >> sft;
     if (sft && w == wi + 1) x |= tb << (64 - sft);
     ik[w] = x;
   }
+}
+template <u32 NWMAX>
+__device__ __forceinline__ void ikey_words(const Planes& K, u64 q, u32 ulen, u64 tag, u64 (&ik)[NWMAX]) {
+  u64 uk[NWMAX];
+#pragma unroll
+  for (u32 w = 0; w < NWMAX; w++) uk[w] = w < K.W ? K.P[(u64)w * K.stride + q] : 0;
+  ikey_words_of<NWMAX>(uk, ulen, tag, ik);
 }
 
 // Byte j of big-endian words w.
@@ -727,6 +814,389 @@ __device__ __forceinline__ void copy_values_chunked(const LdsBuf& buf, bool act,
   wave_sync();
 }
 
+// ---------------------------------------------------------------------------
+// The data block writer's LDS path.  gfx950 stores any width to any LDS byte
+// address, so an entry's key delta, restart word and value go into the stage
+// a word or 16 bytes at a time, always exactly (a store never touches a byte
+// that belongs to another entry).
+
+typedef u16 __attribute__((aligned(1))) u16_una;
+struct __attribute__((aligned(1))) U4una {
+  u32 x, y, z, w;
+};
+__device__ __forceinline__ void lds_st16(u8* p, u16 v) { *reinterpret_cast<u16_una*>(p) = v; }
+__device__ __forceinline__ void lds_st32(u8* p, u32 v) { *reinterpret_cast<u32_una*>(p) = v; }
+__device__ __forceinline__ void lds_st64(u8* p, u64 v) { *reinterpret_cast<u64_una*>(p) = v; }
+__device__ __forceinline__ void lds_st128(u8* p, const uint4& v) {
+  *reinterpret_cast<U4una*>(p) = U4una{v.x, v.y, v.z, v.w};
+}
+
+template <u32 N>
+__device__ __forceinline__ u64 sel_word(const u64 (&w)[N], u32 i) {
+  u64 x = 0;
+#pragma unroll
+  for (u32 t = 0; t < N; t++)
+    if (t == i) x = w[t];
+  return x;
+}
+
+// The n <= 8N bytes held by little-endian words d, stored at p.
+template <u32 N>
+__device__ __forceinline__ void lds_put_words(u8* p, const u64 (&d)[N], u32 n) {
+#pragma unroll
+  for (u32 w = 0; w < N; w++)
+    if (8 * w + 8 <= n) lds_st64(p + 8 * w, d[w]);
+  const u32 t0 = n & ~7u;
+  u64 last = sel_word(d, t0 >> 3);
+  u8* q = p + t0;
+  if (n & 4) {
+    lds_st32(q, (u32)last);
+    last >>= 32;
+    q += 4;
+  }
+  if (n & 2) {
+    lds_st16(q, (u16)last);
+    last >>= 16;
+    q += 2;
+  }
+  if (n & 1) *q = (u8)last;
+}
+
+// Bytes [sh, sh + 8N) of the big-endian key words ik as little-endian words.
+template <u32 N>
+__device__ __forceinline__ void key_delta_words(const u64 (&ik)[N], u32 sh, u64 (&d)[N]) {
+  const u32 s = 8 * (sh & 7);
+#pragma unroll
+  for (u32 j = 0; j < N; j++) {
+    const u32 q = (sh >> 3) + j;
+    const u64 hi = sel_word(ik, q), lo = sel_word(ik, q + 1);
+    d[j] = __builtin_bswap64((hi << s) | ((lo >> 1) >> (63 - s)));
+  }
+}
+
+// 16 bytes of a value source (always followed by at least 15 more readable
+// bytes: a table's restart array, trailer and footer, or the padded value
+// buffer of luda_build_tables).
+__device__ __forceinline__ uint4 gload_16(u64 addr) {
+  const GMEM u8* p = gbytes(addr);
+  return uint4{gload_u32(p), gload_u32(p + 4), gload_u32(p + 8), gload_u32(p + 12)};
+}
+
+// The vl < 16 bytes of v stored at p.
+__device__ __forceinline__ void lds_put_short(u8* p, const uint4& v, u32 vl) {
+  u64 lo = (u64)v.x | ((u64)v.y << 32), hi = (u64)v.z | ((u64)v.w << 32);
+  if (vl & 8) {
+    lds_st64(p, lo);
+    lo = hi;
+    p += 8;
+  }
+  if (vl & 4) {
+    lds_st32(p, (u32)lo);
+    lo >>= 32;
+    p += 4;
+  }
+  if (vl & 2) {
+    lds_st16(p, (u16)lo);
+    lo >>= 16;
+    p += 2;
+  }
+  if (vl & 1) *p = (u8)lo;
+}
+
+// The value of one entry lane copied by that lane, 16 bytes at a time (up to
+// 8 chunks, all loaded before any is stored); the last chunk of a value of
+// 16 bytes or more is the one ending at its end, overlapping the one before
+// with the same bytes.
+__device__ __forceinline__ void copy_value_lane(u8* dst, u64 src, u32 vl) {
+  constexpr u32 kMax = 8;
+  const u32 nch = (vl + 15) >> 4;
+  uint4 v[kMax];
+#pragma unroll
+  for (u32 t = 0; t < kMax; t++)
+    if (t < nch) v[t] = gload_16(vl >= 16 && t + 1 == nch ? src + vl - 16 : src + 16 * t);
+  if (vl < 16) {
+    if (vl) lds_put_short(dst, v[0], vl);
+    return;
+  }
+#pragma unroll
+  for (u32 t = 0; t < kMax; t++)
+    if (t < nch) lds_st128(dst + (t + 1 == nch ? vl - 16 : 16 * t), v[t]);
+}
+
+// Which value chunk each lane of a round takes: entry lanes mark where their
+// chunks start, and a max-scan over the lanes carries each mark forward.
+struct ChunkTab {
+  u32 mark[64];
+  u32 cst[64];  // first chunk of the entry lane
+  u32 off[64];  // its value's offset in the block
+  u32 vl[64];
+  u64 src[64];
+};
+
+// The values of a group of up to 64 entries spread over the wave in 16-byte
+// chunks (for values longer than 8 chunks somewhere in the group).
+__device__ __forceinline__ void copy_values_spread(u8* stage, ChunkTab& ct, bool act, u32 voff, u64 vsrc, u32 vl) {
+  const u32 lane = threadIdx.x & 63;
+  const u32 nch = act ? (vl + 15) >> 4 : 0;
+  const u32 cincl = wave_incl_sum(nch);
+  const u32 C = wave_last(cincl);
+  const u32 cst = cincl - nch;
+  ct.cst[lane] = cst;
+  ct.off[lane] = voff;
+  ct.vl[lane] = vl;
+  ct.src[lane] = vsrc;
+  u32 carry = 0;
+  for (u32 c0 = 0; c0 < C; c0 += 64) {
+    ct.mark[lane] = 0;
+    wave_sync();
+    if (nch && cst >= c0 && cst < c0 + 64) ct.mark[cst - c0] = lane + 1;
+    wave_sync();
+    u32 m = ct.mark[lane];
+    m = wave_incl_max(m);
+    const u32 e = m ? m - 1 : carry;
+    carry = wave_last(e);
+    const u32 c = c0 + lane;
+    if (c < C) {
+      const u32 k = c - ct.cst[e], evl = ct.vl[e];
+      const u32 at = evl >= 16 ? min(16 * k, evl - 16) : 0;
+      const uint4 v = gload_16(ct.src[e] + at);
+      u8* d = stage + ct.off[e] + at;
+      if (evl >= 16) lds_st128(d, v);
+      else lds_put_short(d, v, evl);
+    }
+    wave_sync();
+  }
+}
+
+// The bytes of output block i (entries, restart array, type byte) assembled
+// in the wave's LDS stage at byte mis: BlockBuilder::Add/Finish
+// (table/block_builder.cc:62-114) with the fork's (offset << 8) | count
+// restart words.  One lane per entry: header, key delta from the key words
+// in registers, restart word, and (values of up to 128 bytes) the value.
+template <u32 NWMAX>
+__device__ __forceinline__ void assemble_block_lds(const Kept& k, const EncParams& ep, const EncBufs& b, u64 q0,
+                                                   u32 cnt, u64 bytes, u8* stage, ChunkTab& ct, DevStatus* st) {
+  const u32 lane = threadIdx.x & 63;
+  const u32 R = ep.interval;
+  const Planes& K = k.K;
+  const u32 ngroups = (cnt + R - 1) / R;
+  const u64 dend = bytes - 4ull * ngroups - 4;  // where the restart array starts
+  u64 base = 0;
+  for (u32 j0 = 0; j0 < cnt; j0 += 64) {
+    const u32 j = j0 + lane;
+    const bool act = j < cnt;
+    const u64 q = q0 + j;
+    const bool rs = (j % R) == 0;
+    const u32 sz = act ? (rs ? b.szr[q] : b.sznr[q]) : 0;
+    const u32 incl = wave_incl_scan(sz);
+    const u64 off = base + incl - sz;
+    u32 vl = 0, voff = 0;
+    u64 vsrc = 0;
+    if (act) {
+      const u32 ulen = K.klen[q];
+      const u32 il = ulen + 8;
+      const u32 sh = rs ? 0 : b.shared[q];
+      vl = k.vlen[q];
+      vsrc = k.vptr[q];
+      u64 ik[NWMAX];
+      ikey_words<NWMAX>(K, q, ulen, K.tag[q], ik);
+      u8* p = stage + off;
+      p = enc_varint(p, sh);
+      p = enc_varint(p, il - sh);
+      p = enc_varint(p, vl);
+      u64 d[NWMAX];
+      key_delta_words<NWMAX>(ik, sh, d);
+      lds_put_words<NWMAX>(p, d, il - sh);
+      p += il - sh;
+      voff = (u32)(p - stage);
+      if (rs) lds_st32(stage + dend + 4ull * (j / R), ((u32)off << 8) | min(R, cnt - j));
+    }
+    const u32 maxch = wave_max(act ? (vl + 15) >> 4 : 0);
+    if (maxch <= 8) {
+      if (act) copy_value_lane(stage + voff, vsrc, vl);
+    } else {
+      copy_values_spread(stage, ct, act, voff, vsrc, vl);
+    }
+    base += wave_last(incl);
+  }
+  if (lane == 0) {
+    lds_st32(stage + dend + 4ull * ngroups, ngroups);
+    if (base != dend) set_err(st, ERR_CORRUPT, SITE_CAPACITY);
+    stage[bytes] = 0;  // kNoCompression
+  }
+  wave_sync();
+}
+
+// One lane's entry of a block (lane j < cnt holds entry q0 + j), loaded a
+// block ahead of its assembly so that the loads overlap the block before.
+template <u32 NWMAX>
+struct EntryRegs {
+  u32 sz, sh, ulen, vl;
+  u64 vptr, tag;
+  u64 uk[NWMAX];  // user-key words (zero past the planes)
+};
+template <u32 NWMAX>
+__device__ __forceinline__ EntryRegs<NWMAX> load_entry(const Kept& k, const EncBufs& b, u32 R, u64 q0, u32 cnt) {
+  const u32 j = threadIdx.x & 63;
+  const Planes& K = k.K;
+  const bool act = j < cnt;
+  const u64 q = act ? q0 + j : q0;  // idle lanes read entry q0 and drop it
+  const bool rs = (j % R) == 0;
+  EntryRegs<NWMAX> e;
+  const u32 szr = b.szr[q], sznr = b.sznr[q], sh = b.shared[q];
+  e.ulen = K.klen[q];
+  e.vl = k.vlen[q];
+  e.vptr = k.vptr[q];
+  e.tag = K.tag[q];
+#pragma unroll
+  for (u32 w = 0; w < NWMAX; w++) e.uk[w] = w < K.W ? K.P[(u64)w * K.stride + q] : 0;
+  e.sz = act ? (rs ? szr : sznr) : 0;
+  e.sh = rs ? 0 : sh;
+  if (!act) e.vl = 0;
+  return e;
+}
+
+__device__ __forceinline__ u32 wave_max_dpp(u32 v) { return (u32)__builtin_amdgcn_readlane((int)wave_incl_max(v), 63); }
+
+// assemble_block_lds for a block of at most 64 entries whose entries were
+// loaded ahead.
+template <u32 NWMAX>
+__device__ __forceinline__ void assemble_block_pre(const EncParams& ep, const EntryRegs<NWMAX>& e, u32 cnt, u64 bytes,
+                                                   u8* stage, ChunkTab& ct, DevStatus* st) {
+  const u32 j = threadIdx.x & 63;
+  const u32 R = ep.interval;
+  const u32 ngroups = (cnt + R - 1) / R;
+  const u64 dend = bytes - 4ull * ngroups - 4;  // where the restart array starts
+  const bool act = j < cnt;
+  const bool rs = (j % R) == 0;
+  const u32 incl = wave_incl_sum(e.sz);
+  const u32 off = incl - e.sz;
+  u32 voff = 0;
+  if (act) {
+    const u32 il = e.ulen + 8;
+    u64 ik[NWMAX];
+    ikey_words_of<NWMAX>(e.uk, e.ulen, e.tag, ik);
+    u8* p = stage + off;
+    p = enc_varint(p, e.sh);
+    p = enc_varint(p, il - e.sh);
+    p = enc_varint(p, e.vl);
+    u64 d[NWMAX];
+    key_delta_words<NWMAX>(ik, e.sh, d);
+    lds_put_words<NWMAX>(p, d, il - e.sh);
+    voff = (u32)(p - stage) + il - e.sh;
+    if (rs) lds_st32(stage + dend + 4ull * (j / R), (off << 8) | min(R, cnt - j));
+  }
+  if (wave_max_dpp(act ? (e.vl + 15) >> 4 : 0) <= 8) {
+    if (act) copy_value_lane(stage + voff, e.vptr, e.vl);
+  } else {
+    copy_values_spread(stage, ct, act, voff, e.vptr, e.vl);
+  }
+  const u32 total = wave_last(incl);
+  if (j == 0) {
+    lds_st32(stage + dend + 4ull * ngroups, ngroups);
+    if (total != dend) set_err(st, ERR_CORRUPT, SITE_CAPACITY);
+    stage[bytes] = 0;  // kNoCompression
+  }
+  wave_sync();
+}
+
+typedef u32 u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ u32 sel4(const u32 (&w)[4], u32 i) {
+  return i == 0 ? w[0] : (i == 1 ? w[1] : (i == 2 ? w[2] : w[3]));
+}
+
+// Bytes [lo, hi) of a 4-byte word (lo, hi may fall outside 0..4) as a mask.
+__device__ __forceinline__ u32 byte_span_mask(int lo, int hi) {
+  lo = lo < 0 ? 0 : (lo > 4 ? 4 : lo);
+  hi = hi < 0 ? 0 : (hi > 4 ? 4 : hi);
+  if (hi <= lo) return 0;
+  const u32 a = lo == 4 ? 0u : (0xffffffffu << (8 * lo));
+  const u32 z = hi == 4 ? 0xffffffffu : ~(0xffffffffu << (8 * hi));
+  return a & z;
+}
+
+// One CRC-32C slicing-by-4 step: the raw register after the four bytes of w
+// (little-endian) with register x before them folded in by the caller.
+__device__ __forceinline__ u32 slice4(const u32 (*sl)[256], u32 x) {
+  return sl[3][x & 0xff] ^ sl[2][(x >> 8) & 0xff] ^ sl[1][(x >> 16) & 0xff] ^ sl[0][x >> 24];
+}
+
+// The message of L bytes at stage byte mis (mis < 16; the stage is 16-byte
+// aligned and dst is congruent to it mod 16) copied to dst, followed by its
+// masked crc32c::Value.  The message is cut into the stage's 16-byte pieces:
+// the F whole ones below the message end are right-aligned into rows of 64
+// (1024 bytes), each lane folding one piece per row into its register and
+// writing it to dst with one 16-byte store; the lanes are then joined, and
+// the last e < 16 bytes appended.  The ~0 initial value is folded into the
+// message's first four bytes and the stage bytes before mis are zeroed.
+// zt[j] are copies of g_crc.zeros[j + 2] (the joins of 2^j pieces).  Needs
+// F >= 2.
+__device__ __forceinline__ void crc_copy_out(const uint4* stage4, u32 mis, u32 L, u8* dst, const u32 (*sl)[256],
+                                             const u32 (*z1024)[256], const u32 (*zt)[4][256]) {
+  const u32 lane = threadIdx.x & 63;
+  const u8* stage = reinterpret_cast<const u8*>(stage4);
+  const u32 E = mis + L, F = E >> 4, e = E & 15;
+  GMEM u8* gd = (GMEM u8*)dst;
+  GMEM u32x4* d4 = (GMEM u32x4*)(dst - mis);
+  const u32 rows = (F + 63) >> 6, slot0 = 64 * rows - F;
+  u32 s = 0;
+  for (u32 r = 0; r < rows; r++) {
+    const int f = (int)(64 * r + lane) - (int)slot0;
+    uint4 v = {0, 0, 0, 0};
+    if (f >= 0) {
+      v = stage4[f];
+      if (f > 0 || mis == 0) d4[f] = u32x4{v.x, v.y, v.z, v.w};
+    }
+    u32 w[4] = {v.x, v.y, v.z, v.w};
+    if (r < 2) {  // the first two pieces are in rows 0 and 1
+#pragma unroll
+      for (u32 t = 0; t < 4; t++) {
+        const int lo = 4 * (int)t + (f == 1 ? 16 : 0);
+        if (f == 0) w[t] &= byte_span_mask((int)mis - lo, 4);
+        if (f == 0 || f == 1) w[t] ^= byte_span_mask((int)mis - lo, (int)mis + 4 - lo);
+      }
+    }
+    u32 c = 0;
+#pragma unroll
+    for (u32 t = 0; t < 4; t++) c = slice4(sl, c ^ w[t]);
+    s = crc_shift(z1024, s) ^ c;
+  }
+  // lane l's pieces end 16 (63 - l) bytes before the last whole piece does
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    const u32 other = __shfl_down(s, 1u << j, 64);
+    if ((lane & ((2u << j) - 1)) == 0) s = crc_shift(zt[j], s) ^ other;
+  }
+  s = (u32)__builtin_amdgcn_readlane((int)s, 0);
+  if (e) {
+    // the last e bytes, right-aligned in a zero-filled piece, then folded
+    const uint4 t = stage4[F];
+    u32 w[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+    for (u32 i = 0; i < 4; i++) w[i] &= byte_span_mask(0, (int)e - 4 * (int)i);
+    const u32 z = 16 - e, a = z >> 2, bsh = z & 3;
+    u32 o[4];
+#pragma unroll
+    for (u32 i = 0; i < 4; i++) {
+      const int u = (int)i - (int)a;
+      const u32 hi = u >= 0 ? sel4(w, (u32)u) : 0u;
+      const u32 lo = u >= 1 ? sel4(w, (u32)(u - 1)) : 0u;
+      o[i] = bsh ? __builtin_amdgcn_alignbyte(hi, lo, 4 - bsh) : hi;
+    }
+    u32 c = 0;
+#pragma unroll
+    for (u32 i = 0; i < 4; i++) c = slice4(sl, c ^ o[i]);
+    for (u32 i = 0; i < (e >> 2); i++) s = slice4(sl, s);
+    for (u32 i = 0; i < (e & 3); i++) s = (s >> 8) ^ sl[0][s & 0xff];
+    s ^= c;
+  }
+  const u32 crc = crc_mask(s ^ 0xffffffffu);
+  if (mis && lane < 16 - mis) gd[lane] = stage[mis + lane];
+  if (lane < e) gd[16 * F - mis + lane] = stage[16 * F + lane];
+  if (lane < 4) gd[L + lane] = (u8)(crc >> (8 * lane));
+}
+
 // Copy n bytes from the LDS stage (src, which agrees with dst mod 16) to dst.
 __device__ __forceinline__ void wave_copy_out(u8* dst, const uint4* stage4, u32 mis, u64 n) {
   const u32 lane = threadIdx.x & 63;
@@ -766,25 +1236,56 @@ __device__ __forceinline__ BlockMeta block_meta(const EncBufs& b, u64 i) {
 // written out with 16-byte stores; larger ones (a single big value) are
 // assembled in place.  The block after next has its metadata, and the next
 // its table offset, loaded while the current one is assembled.
+// Each wavefront writes data blocks i, i + S, i + 2S, ... (S = all waves of
+// the grid); one workgroup of 16 waves per CU holds every CRC table in LDS.
+// Blocks that fit are assembled in the wave's LDS stage and written out with
+// 16-byte stores while their CRC is folded; larger ones (a single big value)
+// are assembled in place.  The next block's metadata and entries (blocks of
+// at most 64 entries) are loaded while the current one is assembled and
+// checksummed, the metadata of the one after it too.
+constexpr u32 kWbThreads = 1024;
+constexpr u32 kWbWaves = kWbThreads / 64;
 template <u32 NWMAX>
-__global__ __launch_bounds__(kThreads) void k_write_blocks(Kept k, const u32* nbo_dev, EncParams ep, EncBufs b,
-                                                            u8* out, u64 out_cap, DevStatus* st) {
+__global__ __launch_bounds__(kWbThreads) void k_write_blocks(Kept k, const u32* nbo_dev, EncParams ep, EncBufs b,
+                                                              u8* out, u64 out_cap, DevStatus* st) {
+  constexpr bool kFast = NWMAX > 0;
+  constexpr bool kPre = kFast && NWMAX <= 5;  // entries loaded a block ahead (registers allow keys <= 32 bytes)
+  constexpr u32 NWP = kFast ? NWMAX : 1;
   __shared__ u32 sl[4][256];
-  __shared__ u32 z256[4][256];
-  __shared__ uint4 stage_all[kWaves][(kStage + 32) / 16];
-  __shared__ ValueTab vtab[kWaves];
+  __shared__ u32 zr[4][256];         // 1024 zero bytes (the fast path) or 256 (the others)
+  __shared__ u32 zt[6][4][256];      // 16 << j zero bytes: the joins of crc_copy_out
+  __shared__ uint4 stage_all[kWbWaves][(kStage + 32) / 16];
+  union WaveTab {
+    ChunkTab ct;  // the LDS path
+    ValueTab vt;  // the byte-at-a-time path (NWMAX = 0)
+  };
+  __shared__ WaveTab wtab[kWbWaves];
   if (st->out_bytes > out_cap || st->err) return;
-  load_crc_tables(sl, z256);
+  for (u32 i = threadIdx.x; i < 4 * 256; i += blockDim.x) {
+    sl[i >> 8][i & 255] = g_crc.slice[i >> 8][i & 255];
+    zr[i >> 8][i & 255] = g_crc.zeros[kFast ? 8 : 6][i >> 8][i & 255];
+  }
+  if (kFast)
+    for (u32 i = threadIdx.x; i < 6 * 4 * 256; i += blockDim.x)
+      zt[i >> 10][(i >> 8) & 3][i & 255] = g_crc.zeros[2 + (i >> 10)][(i >> 8) & 3][i & 255];
   __syncthreads();
   const u64 nbo = *nbo_dev;
   const u32 wave = threadIdx.x >> 6;
-  const u64 S = (u64)gridDim.x * kWaves;
-  u64 i = (u64)blockIdx.x * kWaves + wave;
+  const u32 R = ep.interval;
+  const u64 S = (u64)gridDim.x * kWbWaves;
+  u64 i = (u64)blockIdx.x * kWbWaves + wave;
+  auto fits = [&](const BlockMeta& m, u64 d) {
+    return kFast && m.bytes + 5 + 16 <= kStage && (d & 15) + m.bytes + 1 >= 32;
+  };
   BlockMeta m0 = {0, 0, 0, 0, 0}, m1 = {0, 0, 0, 0, 0};
   u64 d0 = 0;
+  EntryRegs<NWP> e0, e1;
+  bool p0 = false;
   if (i < nbo) {
     m0 = block_meta(b, i);
     d0 = b.f_off[m0.file] + m0.off;
+    p0 = kPre && fits(m0, (u64)out + d0) && m0.n <= 64;
+    if (p0) e0 = load_entry<NWP>(k, b, R, m0.first, m0.n);
   }
   if (i + S < nbo) m1 = block_meta(b, i + S);
   for (; i < nbo; i += S) {
@@ -792,69 +1293,162 @@ __global__ __launch_bounds__(kThreads) void k_write_blocks(Kept k, const u32* nb
     BlockMeta m2 = {0, 0, 0, 0, 0};
     if (i + S < nbo) d1 = b.f_off[m1.file] + m1.off;
     if (i + 2 * S < nbo) m2 = block_meta(b, i + 2 * S);
+    const bool p1 = kPre && i + S < nbo && fits(m1, (u64)out + d1) && m1.n <= 64;
+    if (p1) e1 = load_entry<NWP>(k, b, R, m1.first, m1.n);
     u8* dst = out + d0;
     const u64 bytes = m0.bytes;
-    if (bytes + 5 + 16 <= kStage) {
-      const u32 mis = (u32)((u64)dst & 15);
-      assemble_block<NWMAX>(k, ep, b, m0.first, m0.n, bytes,
-                            LdsBuf{reinterpret_cast<u8*>(stage_all[wave]), mis, &vtab[wave]}, sl, z256, st);
-      wave_copy_out(dst, stage_all[wave], mis, bytes + 5);
+    const u32 mis = (u32)((u64)dst & 15);
+    if (fits(m0, (u64)dst)) {
+      u8* stage = reinterpret_cast<u8*>(stage_all[wave]);
+      if constexpr (kFast) {
+        if (p0) assemble_block_pre<NWMAX>(ep, e0, m0.n, bytes, stage + mis, wtab[wave].ct, st);
+        else assemble_block_lds<NWMAX>(k, ep, b, m0.first, m0.n, bytes, stage + mis, wtab[wave].ct, st);
+      }
+      crc_copy_out(stage_all[wave], mis, (u32)bytes + 1, dst, sl, zr, zt);
     } else {
-      assemble_block<NWMAX>(k, ep, b, m0.first, m0.n, bytes, GlobalBuf{dst}, sl, z256, st);
+      if (!kFast && bytes + 5 + 16 <= kStage) {
+        assemble_block<NWMAX>(k, ep, b, m0.first, m0.n, bytes,
+                              LdsBuf{reinterpret_cast<u8*>(stage_all[wave]), mis, &wtab[wave].vt}, sl, zr, st);
+        wave_copy_out(dst, stage_all[wave], mis, bytes + 5);
+      } else {
+        // the row shift of this path is 256 bytes: zr holds it only when !kFast
+        assemble_block<NWMAX>(k, ep, b, m0.first, m0.n, bytes, GlobalBuf{dst}, sl, kFast ? g_crc.zeros[6] : zr,
+                              st);
+      }
     }
     wave_sync();
     m0 = m1;
     d0 = d1;
     m1 = m2;
+    e0 = e1;
+    p0 = p1;
   }
 }
 
-// One wavefront per filter window with keys: bloom bits with LDS atomics.
-__global__ __launch_bounds__(kThreads) void k_write_filters(Kept k, const u32* nw_dev, const u32* nf_dev,
-                                                             EncParams ep, EncBufs b, u8* out, u64 out_cap,
-                                                             DevStatus* st) {
-  __shared__ u32 bits_all[kWaves][kFBits / 4];
-  if (st->out_bytes > out_cap || st->err) return;
-  const u64 nw = *nw_dev;
-  const u32 nf = *nf_dev;
-  const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  u32* lb = bits_all[wave];
-  for (u64 w = (u64)blockIdx.x * kWaves + wave; w < nw; w += (u64)gridDim.x * kWaves) {
-    const u32 n = b.w_keys[w];
-    if (!n) continue;
-    // table of this window
-    u32 lo = 0, hi = nf;
-    while (hi - lo > 1) {
-      const u32 mid = (lo + hi) >> 1;
-      if (b.f_win[mid] <= w) lo = mid;
-      else hi = mid;
-    }
-    const u32 f = lo;
-    u8* dst = out + b.f_off[f] + b.f_dend[f] + (b.w_fsz[w] - b.w_fsz[b.f_win[f]]);
-    u64 nbits = (u64)n * ep.bloom_bits;
-    if (nbits < 64) nbits = 64;
-    const u64 nbytes = (nbits + 7) / 8;
-    nbits = nbytes * 8;
-    if (nbytes > kFBits) {
-      if (lane == 0) set_err(st, ERR_UNSUPPORTED, SITE_CAPACITY);
-      continue;
-    }
-    for (u32 i = lane; i < (nbytes + 3) / 4; i += 64) lb[i] = 0;
-    wave_sync();
-    const u64 q0 = b.w_first[w];
-    for (u32 t = lane; t < n; t += 64) {
-      u32 h = hash_ukey(k.K, q0 + t, 0xbc9f1d34u);
-      const u32 delta = (h >> 17) | (h << 15);
-      for (u32 j = 0; j < ep.bloom_k; j++) {
-        const u32 bp = (u32)(h % nbits);
-        atomicOr(&lb[bp >> 5], 1u << (bp & 31));
-        h += delta;
+// util/hash.cc over a user key held as big-endian words uk (klen <= 8 KW).
+template <u32 KW>
+__device__ __forceinline__ u32 hash_words(const u64 (&uk)[KW], u32 klen, u32 seed) {
+  const u32 m = 0xc6a4a793u;
+  u32 h = seed ^ (klen * m);
+  u32 i = 0;
+#pragma unroll
+  for (u32 w = 0; w < KW; w++) {
+#pragma unroll
+    for (u32 half = 0; half < 2; half++) {
+      if (i + 4 <= klen) {
+        h += __builtin_bswap32(half ? (u32)uk[w] : (u32)(uk[w] >> 32));
+        h *= m;
+        h ^= (h >> 16);
+        i += 4;
       }
     }
-    wave_sync();
-    for (u32 i = lane; i < nbytes; i += 64) dst[i] = (u8)(lb[i >> 2] >> (8 * (i & 3)));
-    if (lane == 0) dst[nbytes] = (u8)ep.bloom_k;
-    wave_sync();
+  }
+  const u32 rest = klen - i;
+  if (rest) {
+    const u64 word = sel_word(uk, i >> 3);
+    const u32 half = (i & 4) ? (u32)word : (u32)(word >> 32);  // bytes i.. in its high bytes
+    const u32 b0 = half >> 24, b1 = (half >> 16) & 0xff, b2 = (half >> 8) & 0xff;
+    if (rest == 3) h += b2 << 16;
+    if (rest >= 2) h += b1 << 8;
+    h += b0;
+    h *= m;
+    h ^= (h >> 24);
+  }
+  return h;
+}
+
+// What a wave needs of filter window w before it can set the window's bits:
+// its key count, first entry, and where its filter goes in the output.
+struct WinHead {
+  u32 n, first;
+  u64 dst;
+};
+
+// One wavefront per filter window with keys: bloom bits with LDS atomics
+// (BloomFilterPolicy::CreateFilter, util/bloom.cc:40-63).  A wave takes
+// windows w, w + S, ...; the next window's head and user keys (its first 64)
+// are loaded while the current window is built.  KW = 0 hashes the keys
+// from the planes (any key width).
+template <u32 KW>
+__global__ __launch_bounds__(kThreads) void k_write_filters(Kept k, const u32* nw_dev, EncParams ep, EncBufs b,
+                                                             u8* out, u64 out_cap, DevStatus* st) {
+  __shared__ u32 bits_all[kWaves][kFBits / 4];
+  if (st->out_bytes > out_cap || st->err) return;
+  constexpr u32 KWP = KW ? KW : 1;
+  const u64 nw = *nw_dev;
+  const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const Planes& K = k.K;
+  u32* lb = bits_all[wave];
+  const u64 S = (u64)gridDim.x * kWaves;
+  auto head = [&](u64 w) {
+    WinHead h = {0, 0, 0};
+    const u32 n = b.w_keys[w], first = b.w_first[w], f = b.w_file[w];
+    const u64 fsz = b.w_fsz[w];
+    if (n) {
+      h.n = n;
+      h.first = first;
+      h.dst = b.f_off[f] + b.f_fbase[f] + fsz;
+    }
+    return h;
+  };
+  auto keys = [&](const WinHead& h, u64 (&uk)[KWP], u32& kl) {
+    const u64 q = h.first + (lane < h.n ? lane : 0);
+    kl = h.n ? K.klen[q] : 0;
+#pragma unroll
+    for (u32 w = 0; w < KWP; w++) uk[w] = (KW && h.n && w < K.W) ? K.P[(u64)w * K.stride + q] : 0;
+  };
+  u64 w = (u64)blockIdx.x * kWaves + wave;
+  WinHead h0 = {0, 0, 0};
+  u64 uk0[KWP];
+  u32 kl0 = 0;
+  if (w < nw) {
+    h0 = head(w);
+    keys(h0, uk0, kl0);
+  }
+  for (; w < nw; w += S) {
+    WinHead h1 = {0, 0, 0};
+    u64 uk1[KWP];
+    u32 kl1 = 0;
+    if (w + S < nw) {
+      h1 = head(w + S);
+      keys(h1, uk1, kl1);
+    }
+    const u32 n = h0.n;
+    if (n) {
+      u64 nbits = (u64)n * ep.bloom_bits;
+      if (nbits < 64) nbits = 64;
+      const u64 nbytes = (nbits + 7) / 8;
+      nbits = nbytes * 8;
+      if (nbytes > kFBits) {
+        if (lane == 0) set_err(st, ERR_UNSUPPORTED, SITE_CAPACITY);
+      } else {
+        for (u32 i = lane; i < (nbytes + 3) / 4; i += 64) lb[i] = 0;
+        wave_sync();
+        for (u32 t = lane; t < n; t += 64) {
+          u32 h;
+          if (KW && t < 64) {
+            h = hash_words<KWP>(uk0, kl0, 0xbc9f1d34u);
+          } else {
+            h = hash_ukey(K, h0.first + t, 0xbc9f1d34u);
+          }
+          const u32 delta = (h >> 17) | (h << 15);
+          for (u32 j = 0; j < ep.bloom_k; j++) {
+            const u32 bp = (u32)(h % nbits);
+            atomicOr(&lb[bp >> 5], 1u << (bp & 31));
+            h += delta;
+          }
+        }
+        wave_sync();
+        GMEM u8* dst = (GMEM u8*)(out + h0.dst);
+        for (u32 i = lane; i < nbytes; i += 64) dst[i] = (u8)(lb[i >> 2] >> (8 * (i & 3)));
+        if (lane == 0) dst[nbytes] = (u8)ep.bloom_k;
+        wave_sync();
+      }
+    }
+    h0 = h1;
+#pragma unroll
+    for (u32 x = 0; x < KWP; x++) uk0[x] = uk1[x];
+    kl0 = kl1;
   }
 }
 
@@ -863,39 +1457,46 @@ __device__ __forceinline__ u8* put_fixed32(u8* p, u32 v) {
   return p + 4;
 }
 
-// crc32c::Value of L bytes with the rows split over the workgroup's waves:
-// each wave folds a contiguous run of rows, and the runs are joined by
-// appending to each the zero bytes of the rows after it.  Called by every
-// thread of the workgroup; every thread gets the result.
-template <typename ByteAt>
-__device__ __forceinline__ u32 block_crc_split(u64 L, const u32 (*sl)[256], const u32 (*z256)[256], u32* part,
-                                               ByteAt at) {
+// crc32c::Value of the L bytes at p with the rows split over the workgroup's
+// NW waves: each wave folds a contiguous run of rows, and the runs are joined
+// by appending to each the zero bytes of the rows after it.  Called by every
+// thread of the workgroup; every thread gets the result.  zt holds the first
+// nz zero-append maps.
+template <u32 NW>
+__device__ __forceinline__ u32 block_crc_split(u64 L, const GMEM u8* p, const u32 (*sl)[256],
+                                               const u32 (*zt)[4][256], u32 nz, u32* part) {
   const u32 wave = threadIdx.x >> 6;
   const u64 rows = (L + 255) >> 8;
-  const u64 per = (rows + kWaves - 1) / kWaves;
+  const u64 per = (rows + NW - 1) / NW;
   const u64 r0 = min(rows, per * wave), r1 = min(rows, r0 + per);
-  u32 s = wave_crc32c_rows(L, r0, r1, sl, z256, at);
-  s = crc_append_zeros(s, 256 * (rows - r1));
+  u32 s = wave_crc32c_rows_global(L, r0, r1, p, sl, zt[6], zt);
+  s = crc_append_zeros_tab(s, 256 * (rows - r1), zt, nz);
   if ((threadIdx.x & 63) == 0) part[wave] = s;
   __syncthreads();
   u32 x = 0;
 #pragma unroll
-  for (u32 w = 0; w < kWaves; w++) x ^= part[w];
+  for (u32 w = 0; w < NW; w++) x ^= part[w];
   __syncthreads();
   return x ^ 0xffffffffu;
 }
 
+constexpr u32 kTailThreads = 1024;
+constexpr u32 kTailWaves = kTailThreads / 64;
+constexpr u32 kTailZeros = 14;  // zero-append maps kept in LDS: up to 32 KiB of zeros
+
 // One workgroup per table: filter offsets, filter/metaindex/index blocks with
 // trailers, footer, and the table's metadata.
 template <u32 NWMAX>
-__global__ __launch_bounds__(kThreads) void k_write_tail(Kept k, const u32* nf_dev, EncParams ep, EncBufs b, u8* out,
+__global__ __launch_bounds__(kTailThreads) void k_write_tail(Kept k, const u32* nf_dev, EncParams ep, EncBufs b, u8* out,
                                                           u64 out_cap, DevStatus* st) {
   __shared__ u32 sl[4][256];
-  __shared__ u32 z256[4][256];
+  __shared__ u32 zl[kTailZeros][4][256];
   __shared__ u64 s_moff, s_msize;
-  __shared__ u32 s_part[kWaves];
+  __shared__ u32 s_part[kTailWaves];
   if (st->out_bytes > out_cap || st->err) return;
-  load_crc_tables(sl, z256);
+  for (u32 i = threadIdx.x; i < 4 * 256; i += blockDim.x) sl[i >> 8][i & 255] = g_crc.slice[i >> 8][i & 255];
+  for (u32 i = threadIdx.x; i < kTailZeros * 4 * 256; i += blockDim.x)
+    zl[i >> 10][(i >> 8) & 3][i & 255] = g_crc.zeros[i >> 10][(i >> 8) & 3][i & 255];
   __syncthreads();
   const u64 nf = *nf_dev;
   const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -983,7 +1584,7 @@ __global__ __launch_bounds__(kThreads) void k_write_tail(Kept k, const u32* nf_d
       if (threadIdx.x == 0) st32(ib + isum + 4ull * (b1 - b0), b1 - b0);
     }
     // type bytes, then the filter and index blocks' CRCs with the rows of
-    // each split over the four waves (the long ones: ~47 KB of filters per
+    // each split over the waves (the long ones: ~47 KB of filters per
     // 2 MiB table), the metaindex's by wave 0
     if (threadIdx.x == 0) {
       if (ep.bloom_bits) base[dend + fbsize] = 0;
@@ -993,19 +1594,20 @@ __global__ __launch_bounds__(kThreads) void k_write_tail(Kept k, const u32* nf_d
     __syncthreads();
     if (ep.bloom_bits) {
       u8* fs = base + dend;
-      const u32 crc = block_crc_split(fbsize + 1, sl, z256, s_part, [&](u64 x) -> u8 { return fs[x]; });
+      const u32 crc = block_crc_split<kTailWaves>(fbsize + 1, (const GMEM u8*)fs, sl, zl, kTailZeros, s_part);
       if (threadIdx.x == 0) st32(fs + fbsize + 1, crc_mask(crc));
     }
     {
       u8* is = base + ioff;
-      const u32 crc = block_crc_split(isize + 1, sl, z256, s_part, [&](u64 x) -> u8 { return is[x]; });
+      const u32 crc = block_crc_split<kTailWaves>(isize + 1, (const GMEM u8*)is, sl, zl, kTailZeros, s_part);
       if (threadIdx.x == 0) st32(is + isize + 1, crc_mask(crc));
     }
     if (wave == 0) {
       u8* ms = base + s_moff;
-      const u32 crc = wave_crc32c(s_msize + 1, sl, z256, [&](u64 x) -> u8 { return ms[x]; });
+      const u64 mrows = (s_msize + 1 + 255) >> 8;
+      const u32 crc = wave_crc32c_rows_global(s_msize + 1, 0, mrows, (const GMEM u8*)ms, sl, zl[6], zl) ^ 0xffffffffu;
       if (lane == 0) st32(ms + s_msize + 1, crc_mask(crc));
-    } else if (wave == 3 && lane == 0) {
+    } else if (wave == 1 && lane == 0) {
       // footer (Footer::EncodeTo, table/format.cc:31-40)
       u8* p = base + ioff + isize + 5;
       u8* s = p;
@@ -1048,7 +1650,7 @@ void launch_entry_sizes(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncBufs 
   KLAUNCH(T, "entry_sizes", k_entry_sizes, grid_for(m_cap, 256), 256, 0, k, m_dev, b, st);
 }
 void launch_bnext(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncParams ep, EncBufs b, DevStatus* st) {
-  KLAUNCH(T, "bnext", k_bnext, std::min<u32>(grid_for(m_cap, 256), 2048), 256, 0, k, m_dev, ep, b, st);
+  KLAUNCH(T, "bnext", k_bnext, std::min<u32>(grid_for(m_cap, kBnStarts), 2048), 256, 0, k, m_dev, ep, b, st);
 }
 void launch_block_info(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncBufs b, DevStatus* st) {
   KLAUNCH(T, "block_info", k_block_info, grid_for(m_cap, 256), 256, 0, k, m_dev, b, st);
@@ -1089,32 +1691,39 @@ void launch_file_sizes(KTimer& T, const u32* nf_dev, u32 nf_cap, EncParams ep, E
 }
 void launch_write_blocks(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u8* out,
                          u64 out_cap, DevStatus* st) {
-  // enough workgroups to fill every CU a few times; each wave loops
-  const u32 g = std::min<u32>(grid_for(nbo_cap, kWaves), 2048);
+  // one workgroup of 16 waves per CU (its LDS holds one); each wave loops
+  const u32 g = std::min<u32>(grid_for(nbo_cap, kWbWaves), 256);
   if (k.K.W <= 2)
-    KLAUNCH(T, "write_blocks", k_write_blocks<3>, g, kThreads, 0, k, nbo_dev, ep, b, out, out_cap, st);
+    KLAUNCH(T, "write_blocks", k_write_blocks<3>, g, kWbThreads, 0, k, nbo_dev, ep, b, out, out_cap, st);
   else if (k.K.W <= 4)
-    KLAUNCH(T, "write_blocks", k_write_blocks<5>, g, kThreads, 0, k, nbo_dev, ep, b, out, out_cap, st);
+    KLAUNCH(T, "write_blocks", k_write_blocks<5>, g, kWbThreads, 0, k, nbo_dev, ep, b, out, out_cap, st);
   else if (k.K.W <= 8)
-    KLAUNCH(T, "write_blocks", k_write_blocks<9>, g, kThreads, 0, k, nbo_dev, ep, b, out, out_cap, st);
+    KLAUNCH(T, "write_blocks", k_write_blocks<9>, g, kWbThreads, 0, k, nbo_dev, ep, b, out, out_cap, st);
   else
-    KLAUNCH(T, "write_blocks", k_write_blocks<0>, g, kThreads, 0, k, nbo_dev, ep, b, out, out_cap, st);
+    KLAUNCH(T, "write_blocks", k_write_blocks<0>, g, kWbThreads, 0, k, nbo_dev, ep, b, out, out_cap, st);
 }
 void launch_write_filters(KTimer& T, Kept k, const u32* nw_dev, u32 nw_cap, const u32* nf_dev, EncParams ep, EncBufs b,
                           u8* out, u64 out_cap, DevStatus* st) {
-  KLAUNCH(T, "write_filters", k_write_filters, std::min<u32>(grid_for(nw_cap, kWaves), 2048), kThreads, 0, k, nw_dev,
-          nf_dev, ep, b, out,
-          out_cap, st);
+  (void)nf_dev;
+  const u32 g = std::min<u32>(grid_for(nw_cap, kWaves), 2048);
+  if (k.K.W <= 2)
+    KLAUNCH(T, "write_filters", k_write_filters<2>, g, kThreads, 0, k, nw_dev, ep, b, out, out_cap, st);
+  else if (k.K.W <= 4)
+    KLAUNCH(T, "write_filters", k_write_filters<4>, g, kThreads, 0, k, nw_dev, ep, b, out, out_cap, st);
+  else if (k.K.W <= 8)
+    KLAUNCH(T, "write_filters", k_write_filters<8>, g, kThreads, 0, k, nw_dev, ep, b, out, out_cap, st);
+  else
+    KLAUNCH(T, "write_filters", k_write_filters<0>, g, kThreads, 0, k, nw_dev, ep, b, out, out_cap, st);
 }
 void launch_write_tail(KTimer& T, Kept k, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b, u8* out,
                        u64 out_cap, DevStatus* st) {
   const u32 g = grid_for(nf_cap, 1);
   if (k.K.W <= 2)
-    KLAUNCH(T, "write_tail", k_write_tail<3>, g, kThreads, 0, k, nf_dev, ep, b, out, out_cap, st);
+    KLAUNCH(T, "write_tail", k_write_tail<3>, g, kTailThreads, 0, k, nf_dev, ep, b, out, out_cap, st);
   else if (k.K.W <= 4)
-    KLAUNCH(T, "write_tail", k_write_tail<5>, g, kThreads, 0, k, nf_dev, ep, b, out, out_cap, st);
+    KLAUNCH(T, "write_tail", k_write_tail<5>, g, kTailThreads, 0, k, nf_dev, ep, b, out, out_cap, st);
   else if (k.K.W <= 8)
-    KLAUNCH(T, "write_tail", k_write_tail<9>, g, kThreads, 0, k, nf_dev, ep, b, out, out_cap, st);
+    KLAUNCH(T, "write_tail", k_write_tail<9>, g, kTailThreads, 0, k, nf_dev, ep, b, out, out_cap, st);
   else
-    KLAUNCH(T, "write_tail", k_write_tail<0>, g, kThreads, 0, k, nf_dev, ep, b, out, out_cap, st);
+    KLAUNCH(T, "write_tail", k_write_tail<0>, g, kTailThreads, 0, k, nf_dev, ep, b, out, out_cap, st);
 }

@@ -42,12 +42,19 @@ __global__ __launch_bounds__(kThreads) void k_orbit_chunk(const u32* nxt, const 
     const u64 cend = c0 + C;
     for (u32 i = threadIdx.x; i < C; i += blockDim.x) ex[i] = c0 + i < n ? nxt[c0 + i] : (u32)n;
     __syncthreads();
+    // pointer jumping until every pointer has left the chunk (at most
+    // log2(C) + 1 rounds; fewer when the paths through the chunk are short)
     for (u32 r = 1; r <= C; r <<= 1) {
+      int inside = 0;
       for (u32 i = threadIdx.x; i < C; i += blockDim.x) {
         const u32 v = ex[i];
-        if (v < cend && v < n) ex[i] = ex[v - c0];
+        if (v < cend && v < n) {
+          const u32 w = ex[v - c0];
+          ex[i] = w;
+          inside |= w < cend && w < n;
+        }
       }
-      __syncthreads();
+      if (!__syncthreads_or(inside)) break;
     }
     for (u32 x = threadIdx.x; x < Wd; x += blockDim.x) {
       u16 r = kEnd;
@@ -127,32 +134,59 @@ __global__ void k_orbit_root(u32* e) { *e = 0; }
 // the single top function, then push entry offsets back down, with a
 // workgroup barrier between levels (levels are small: the chunk functions
 // number n / C).  Same tables and results as k_orbit_compose / _root /
-// _expand.  eoff[l] is the offset of level l's entries in `entries`.
+// _expand.  With `staged`, every level's functions are kept in LDS (Wd
+// apart) and the walks read them there; only the level-0 functions are read
+// from global memory, once.
 struct OrbitLevels {
   u32 levels;
   u64 level_offset[kMaxOrbitLevels];  // of each level's functions in ftab
   u64 entry_offset[kMaxOrbitLevels];  // of each level's entries in entries
 };
 __global__ __launch_bounds__(1024) void k_orbit_top(u16* ftab, u32* entries, const u32* n_dev, const u32* step_dev,
-                                                    u32 C, u32 G, u32 Wc, OrbitLevels L) {
-  const u64 n = *n_dev;
+                                                    u32 C, u32 G, u32 Wc, OrbitLevels L, u32 lds_cap) {
+  extern __shared__ u16 lt[];
+  // u32 arithmetic throughout: every level's function count times Wd is
+  // below n (C >= 2 Wd), and n < 2^32
+  const u32 n = *n_dev;
   const u32 step = *step_dev;
   if (step > Wc) return;  // k_orbit_chunk reported the overflow
   const u32 Wd = step ? step : 1;
   const u32 tid = threadIdx.x;
+  u32 items[kMaxOrbitLevels + 1], loff[kMaxOrbitLevels + 2];
+  items[0] = n / C + (n % C != 0);
+  loff[0] = 0;
+#pragma unroll
+  for (u32 l = 0; l <= kMaxOrbitLevels; l++) {
+    if (l + 1 <= kMaxOrbitLevels) items[l + 1] = items[l] / G + (items[l] % G != 0);
+    loff[l + 1] = loff[l] + (l <= L.levels ? items[l] * Wd : 0);
+  }
+  const bool staged = loff[L.levels + 1] <= lds_cap;
+  if (staged) {
+    const u32 t0 = items[0] * Wd;
+    for (u32 i = tid; i < t0; i += blockDim.x) {
+      const u32 o = i / Wd;
+      lt[i] = ftab[(u64)o * Wc + (i - o * Wd)];
+    }
+    __syncthreads();
+  }
   for (u32 l = 0; l < L.levels; l++) {
     const u16* fin = ftab + L.level_offset[l];
     u16* fout = ftab + L.level_offset[l + 1];
-    const u64 nin = level_items(n, C, G, l);
-    const u64 nout = (nin + G - 1) / G;
-    for (u64 i = tid; i < nout * Wd; i += blockDim.x) {
-      const u64 o = i / Wd;
-      const u32 x = (u32)(i % Wd);
-      const u64 t0 = o * G;
-      const u32 tc = (u32)min((u64)G, nin - t0);
+    const u32 nin = items[l], nout = items[l + 1];
+    const u32 lin = loff[l], lout = loff[l + 1];
+    for (u32 i = tid; i < nout * Wd; i += blockDim.x) {
+      const u32 o = i / Wd;
+      const u32 x = i - o * Wd;
+      const u32 t0 = o * G;
+      const u32 tc = min(G, nin - t0);
       u32 y = x;
-      for (u32 t = 0; t < tc && y != kEnd; t++) y = y < Wd ? fin[(t0 + t) * Wc + y] : kEnd;
-      fout[o * Wc + x] = (u16)y;
+      if (staged) {
+        for (u32 t = 0; t < tc && y != kEnd; t++) y = y < Wd ? lt[lin + (t0 + t) * Wd + y] : kEnd;
+        lt[lout + i] = (u16)y;
+      } else {
+        for (u32 t = 0; t < tc && y != kEnd; t++) y = y < Wd ? fin[(u64)(t0 + t) * Wc + y] : kEnd;
+      }
+      fout[(u64)o * Wc + x] = (u16)y;
     }
     __syncthreads();
   }
@@ -162,15 +196,18 @@ __global__ __launch_bounds__(1024) void k_orbit_top(u16* ftab, u32* entries, con
     const u16* fin = ftab + L.level_offset[l];
     const u32* eout = entries + L.entry_offset[l + 1];
     u32* ein = entries + L.entry_offset[l];
-    const u64 nin = level_items(n, C, G, l);
-    const u64 nout = (nin + G - 1) / G;
-    for (u64 o = tid; o < nout; o += blockDim.x) {
-      const u64 t0 = o * G;
-      const u32 tc = (u32)min((u64)G, nin - t0);
+    const u32 nin = items[l], nout = items[l + 1];
+    const u32 lin = loff[l];
+    for (u32 o = tid; o < nout; o += blockDim.x) {
+      const u32 t0 = o * G;
+      const u32 tc = min(G, nin - t0);
       u32 y = eout[o];
       for (u32 t = 0; t < tc; t++) {
         ein[t0 + t] = y;
-        if (y != kEnd) y = y < Wd ? fin[(t0 + t) * Wc + y] : kEnd;
+        if (y != kEnd) {
+          if (staged) y = y < Wd ? lt[lin + (t0 + t) * Wd + y] : kEnd;
+          else y = y < Wd ? fin[(u64)(t0 + t) * Wc + y] : kEnd;
+        }
       }
     }
     __syncthreads();
@@ -262,7 +299,10 @@ void run_orbit(KTimer& T, const OrbitPlan& p, const u32* nxt, const u32* n_dev, 
       lv.level_offset[l] = p.level_offset[l];
       lv.entry_offset[l] = eoff[l];
     }
-    KLAUNCH(T, "orbit_top", k_orbit_top, 1, 1024, 0, ftab, entries, n_dev, step_dev, p.C, p.G, p.Wc, lv);
+    // every level's functions in LDS when the step found on the device allows
+    const u32 lds_cap = 30 * 1024;  // u16 entries: 60 KiB
+    KLAUNCH(T, "orbit_top", k_orbit_top, 1, 1024, lds_cap * sizeof(u16), ftab, entries, n_dev, step_dev, p.C, p.G,
+            p.Wc, lv, lds_cap);
     KLAUNCH(T, "orbit_mark", k_orbit_mark, grid_for(p.level_items[0]), kThreads, p.C * sizeof(u32) + p.C, nxt, n_dev,
             step_dev, p.C, p.Wc, entries, flag);
     return;

@@ -101,6 +101,22 @@ def test_build_tables_match_oracle(eng, case):
     assert summary(out[0]) == summary(want)
 
 
+@pytest.mark.parametrize("block_size,interval,nkeys,vmax", [(8192, 16, 40, 0), (8192, 1000, 60, 0), (7000, 5, 30, 3),
+                                                            (8192, 1, 30, 1), (1, 16, 3, 30)])
+def test_block_cuts_many_small_entries(eng, block_size, interval, nkeys, vmax):
+    """Blocks of hundreds of 10-byte entries (versions of a few user keys):
+    the block cut's staged span is exceeded and finished from global
+    memory; restart intervals from 1 to 1000."""
+    rng = random.Random(block_size * 7 + interval)
+    ents = [(ikey(b"k%03d" % u, 100000 - v, 1), bytes(rng.randrange(256) for _ in range(rng.randint(0, vmax))))
+            for u in range(nkeys) for v in range(100)]
+    opts = ffi.default_options(block_size=block_size, block_restart_interval=interval, max_file_size=20000)
+    print("cuts", block_size, interval, len(ents), flush=True)
+    rc, out, msg = eng.build_entries(ents, opts)
+    assert rc == 0, msg
+    assert summary(out[0]) == summary(oracle.build_tables(ents, opts, "oracle"))
+
+
 def engine_tables(eng, ents, opts):
     rc, out, msg = eng.build_entries(ents, opts)
     assert rc == 0, msg
