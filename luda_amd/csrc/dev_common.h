@@ -64,7 +64,7 @@ struct DevStatus {
   u32 need_windows;   // filter windows, when over the host's capacity
   u32 max_nr;         // most restart points in one input data block
   u32 need_blocks;    // output blocks, when over the host's capacity
-  u32 pad0;
+  u32 max_ent;        // most entries in one input data block
   u64 out_bytes;      // sum of output table sizes
   u64 data_bytes;     // output data blocks with trailers
   u64 keys_needed;    // bytes of smallest/largest keys

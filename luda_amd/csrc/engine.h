@@ -210,7 +210,7 @@ void launch_index(KTimer& T, const FileDesc* files, const FileInfo* info, const 
 void launch_file_ebase(KTimer& T, const u32* blk_base, const u32* blk_ebase, u32 nfiles, const u32* n_entries,
                        u32* file_ebase);
 void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 nb, const u32* blk_ebase, Planes K,
-                   u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr);
+                   u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr, u32 max_ent);
 void launch_ingest(KTimer& T, u64 n, const u8* keys, const u64* key_off, const u8* vals, const u64* val_off, Kept out,
                    DevStatus* st);
 

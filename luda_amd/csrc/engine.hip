@@ -514,19 +514,17 @@ void fill_kernel_stats(luda_ctx* c, luda_stats* s, u32 W, u64 rounds) {
 }
 
 // Reads the outputs back and builds the result.
+// hmeta / hkeys: host copies of the table metadata and boundary keys, read
+// back with the job's status (nf <= the capacity they were read at).
 luda_result* finish_result(luda_ctx* c, const EncodeSetup& s, const EncodeArrays& A, void* out, const DevStatus& st,
-                           u32 flags, luda_stats stats) {
+                           u32 flags, luda_stats stats, const FileMeta* hmeta, const u8* hkeys) {
+  (void)A;
   const u32 nf = st.n_files_out;
   luda_result* r = (luda_result*)calloc(1, sizeof(luda_result));
   r->num_outputs = nf;
   r->outputs = (luda_output*)calloc(nf ? nf : 1, sizeof(luda_output));
-  std::vector<FileMeta> meta(nf);
-  std::vector<u8> keys((u64)nf * 2 * s.ep.kw);
-  if (nf) {
-    HIPCHK(hipMemcpyAsync(meta.data(), A.eb.meta, nf * sizeof(FileMeta), hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipMemcpyAsync(keys.data(), A.eb.keys, keys.size(), hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
-  }
+  std::vector<FileMeta> meta(hmeta, hmeta + nf);
+  std::vector<u8> keys(hkeys, hkeys + (u64)nf * 2 * s.ep.kw);
   u64 kbytes = 0;
   for (u32 i = 0; i < nf; i++) kbytes += meta[i].slen + meta[i].llen;
   r->keys = (uint8_t*)malloc(kbytes ? kbytes : 1);
@@ -757,14 +755,58 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     if (W > kMaxKeyWords) throw Fail{LUDA_NOT_SUPPORTED, "user keys longer than the device key planes"};
     plan_rounds(W);
     EncodeSetup es = plan_encode(job->options, W, n, out_cap, ngp, nf_min, nw_min, nbo_min);
+    // host-side tables (grandparents, deeper levels, children, merge pairs)
+    // staged in one blob that a single copy uploads
+    std::vector<u64> gw((u64)std::max<u32>(ngp, 1) * W, 0), gt(std::max<u32>(ngp, 1), 0), gpre(ngp + 1, 0);
+    std::vector<u32> gl(std::max<u32>(ngp, 1), 0);
+    for (u32 i = 0; i < ngp; i++) {
+      const luda_slice& k = job->grandparents[i].largest;
+      to_words(k.data, k.size - 8, W, &gw[(u64)i * W]);
+      gl[i] = (u32)(k.size - 8);
+      memcpy(&gt[i], k.data + k.size - 8, 8);
+      gpre[i + 1] = gpre[i] + job->grandparents[i].file_size;
+    }
+    std::vector<u64> dw((u64)std::max<u32>(ndeep, 1) * 2 * W, 0);
+    std::vector<u32> dl(std::max<u32>(ndeep, 1) * 2, 0);
+    DropParams dp;
+    memset(&dp, 0, sizeof(dp));
+    {
+      u32 di = 0;
+      for (int L = 0; L < 7; L++) {
+        dp.lvl_begin[L] = di;
+        for (u32 idx : lvl_files[L]) {
+          const luda_level_file& f = job->deeper[idx];
+          to_words(f.smallest_user_key.data, f.smallest_user_key.size, W, &dw[(2ull * di) * W]);
+          to_words(f.largest_user_key.data, f.largest_user_key.size, W, &dw[(2ull * di + 1) * W]);
+          dl[2 * di] = (u32)f.smallest_user_key.size;
+          dl[2 * di + 1] = (u32)f.largest_user_key.size;
+          di++;
+        }
+        dp.lvl_end[L] = di;
+      }
+    }
+    std::vector<MergePair> allp;
+    for (auto& r : rounds) allp.insert(allp.end(), r.begin(), r.end());
+    if (allp.empty()) allp.push_back(MergePair{});  // no rounds: a placeholder keeps the region non-empty
+    std::vector<u8> blob;
+    auto put = [&](const void* p, u64 bytes) {
+      const u64 off = align_up(blob.size(), 16);
+      blob.resize(off + bytes);
+      if (bytes) memcpy(blob.data() + off, p, bytes);
+      return off;
+    };
+    const u64 o_gw = put(gw.data(), gw.size() * 8), o_gl = put(gl.data(), gl.size() * 4),
+              o_gt = put(gt.data(), gt.size() * 8), o_gp = put(gpre.data(), gpre.size() * 8),
+              o_dw = put(dw.data(), dw.size() * 8), o_dl = put(dl.data(), dl.size() * 4),
+              o_cs = put(cstart.data(), cstart.size() * 4),
+              o_mp = put(allp.data(), allp.size() * sizeof(MergePair));
     // carve phase B
     Planes K;
     u64 *vptr, *gp_words, *gp_tag, *gp_prefix, *deep_words;
     u32 *vlen, *permA, *permB, *keep, *kpos, *gcnt, *events, *gp_len, *deep_len, *d_cstart;
     MergePair* d_pairs;
+    u8* d_tab;
     EncodeArrays EA;
-    u64 npairs_total = 0;
-    for (auto& r : rounds) npairs_total += r.size();
     auto carve_b = [&](Arena& a) {
       K.P = a.take<u64>((u64)W * n);
       K.klen = a.take<u32>(n);
@@ -779,14 +821,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
       kpos = a.take<u32>(n);
       gcnt = a.take<u32>(ngp ? n : 1);
       events = a.take<u32>(ngp + 2);
-      gp_words = a.take<u64>((u64)std::max<u32>(ngp, 1) * W);
-      gp_len = a.take<u32>(std::max<u32>(ngp, 1));
-      gp_tag = a.take<u64>(std::max<u32>(ngp, 1));
-      gp_prefix = a.take<u64>(ngp + 1);
-      deep_words = a.take<u64>((u64)std::max<u32>(ndeep, 1) * 2 * W);
-      deep_len = a.take<u32>(std::max<u32>(ndeep, 1) * 2);
-      d_cstart = a.take<u32>(cstart.size());
-      d_pairs = a.take<MergePair>(std::max<u64>(npairs_total, 1));
+      d_tab = a.take<u8>(blob.size());
       EA = carve_encode(a, es, W, true);
     };
     {
@@ -800,65 +835,22 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
       a.cap = c->b_buf.cap;
       carve_b(a);
     }
+    gp_words = reinterpret_cast<u64*>(d_tab + o_gw);
+    gp_len = reinterpret_cast<u32*>(d_tab + o_gl);
+    gp_tag = reinterpret_cast<u64*>(d_tab + o_gt);
+    gp_prefix = reinterpret_cast<u64*>(d_tab + o_gp);
+    deep_words = reinterpret_cast<u64*>(d_tab + o_dw);
+    deep_len = reinterpret_cast<u32*>(d_tab + o_dl);
+    d_cstart = reinterpret_cast<u32*>(d_tab + o_cs);
+    d_pairs = reinterpret_cast<MergePair*>(d_tab + o_mp);
     void* out = c->take_out(std::max<u64>(out_cap, 1));
     if (!out) throw Fail{LUDA_IO_ERROR, "output allocation failed"};
-    // host-side tables: grandparents, deeper levels, children, merge pairs
     {
-      std::vector<u64> gw((u64)std::max<u32>(ngp, 1) * W, 0), gt(std::max<u32>(ngp, 1), 0), gpre(ngp + 1, 0);
-      std::vector<u32> gl(std::max<u32>(ngp, 1), 0);
-      for (u32 i = 0; i < ngp; i++) {
-        const luda_slice& k = job->grandparents[i].largest;
-        to_words(k.data, k.size - 8, W, &gw[(u64)i * W]);
-        gl[i] = (u32)(k.size - 8);
-        memcpy(&gt[i], k.data + k.size - 8, 8);
-        gpre[i + 1] = gpre[i] + job->grandparents[i].file_size;
-      }
-      std::vector<u64> dw((u64)std::max<u32>(ndeep, 1) * 2 * W, 0);
-      std::vector<u32> dl(std::max<u32>(ndeep, 1) * 2, 0);
-      DropParams dp;
-      memset(&dp, 0, sizeof(dp));
-      u32 di = 0;
-      for (int L = 0; L < 7; L++) {
-        dp.lvl_begin[L] = di;
-        for (u32 idx : lvl_files[L]) {
-          const luda_level_file& f = job->deeper[idx];
-          to_words(f.smallest_user_key.data, f.smallest_user_key.size, W, &dw[(2ull * di) * W]);
-          to_words(f.largest_user_key.data, f.largest_user_key.size, W, &dw[(2ull * di + 1) * W]);
-          dl[2 * di] = (u32)f.smallest_user_key.size;
-          dl[2 * di + 1] = (u32)f.largest_user_key.size;
-          di++;
-        }
-        dp.lvl_end[L] = di;
-      }
-      std::vector<MergePair> allp;
-      for (auto& r : rounds) allp.insert(allp.end(), r.begin(), r.end());
-      // one staging copy
-      std::vector<u8> blob;
-      auto put = [&](const void* p, u64 bytes) {
-        const u64 off = align_up(blob.size(), 16);
-        blob.resize(off + bytes);
-        memcpy(blob.data() + off, p, bytes);
-        return off;
-      };
-      const u64 o_gw = put(gw.data(), gw.size() * 8), o_gl = put(gl.data(), gl.size() * 4),
-                o_gt = put(gt.data(), gt.size() * 8), o_gp = put(gpre.data(), gpre.size() * 8),
-                o_dw = put(dw.data(), dw.size() * 8), o_dl = put(dl.data(), dl.size() * 4),
-                o_cs = put(cstart.data(), cstart.size() * 4),
-                o_mp = put(allp.data(), std::max<u64>(allp.size(), 0) * sizeof(MergePair));
       HIPCHK(c->pin2.reserve(blob.size() + 16));
-      HIPCHK(hipStreamSynchronize(c->s));
+      HIPCHK(hipStreamSynchronize(c->s));  // the staging buffer may still feed an earlier copy
       memcpy(c->pin2.p, blob.data(), blob.size());
-      auto cp = [&](void* dst, u64 off, u64 bytes) {
-        if (bytes) HIPCHK(hipMemcpyAsync(dst, c->pin2.p + off, bytes, hipMemcpyHostToDevice, c->s));
-      };
-      cp(gp_words, o_gw, gw.size() * 8);
-      cp(gp_len, o_gl, gl.size() * 4);
-      cp(gp_tag, o_gt, gt.size() * 8);
-      cp(gp_prefix, o_gp, gpre.size() * 8);
-      cp(deep_words, o_dw, dw.size() * 8);
-      cp(deep_len, o_dl, dl.size() * 4);
-      cp(d_cstart, o_cs, cstart.size() * 4);
-      cp(d_pairs, o_mp, allp.size() * sizeof(MergePair));
+      HIPCHK(hipMemcpyAsync(d_tab, c->pin2.p, blob.size(), hipMemcpyHostToDevice, c->s));
+
       dp.S = job->smallest_snapshot;
       dp.limit = 10 * job->options.max_file_size;  // MaxGrandParentOverlapBytes, db/version_set.cc:31-33
       dp.ngp = ngp;
@@ -878,7 +870,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
       sb.max_ulen = 0;
       fill_status(c, sb);
       // decode, check, merge, drop, gather, stops
-      launch_decode(T, d_files, d_blks, nb, d_ebase, K, vptr, vlen, c->st, stA.max_blk, stA.max_nr);
+      launch_decode(T, d_files, d_blks, nb, d_ebase, K, vptr, vlen, c->st, stA.max_blk, stA.max_nr, stA.max_ent);
       launch_check_sorted(T, K, n, d_cstart, (u32)cstart.size() - 1, c->st);
       const u32* perm = permA;
       iota_u32(T, permA, n);
@@ -905,7 +897,15 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev1, c->s));
+    // status, table metadata and boundary keys (at capacity) in one round trip
+    const u64 o_meta = align_up(sizeof(DevStatus), 64);
+    const u64 meta_bytes = (u64)es.nf_cap * sizeof(FileMeta);
+    const u64 o_keys = align_up(o_meta + meta_bytes, 64);
+    const u64 key_bytes = (u64)es.nf_cap * 2 * es.ep.kw;
+    HIPCHK(c->pin.reserve(o_keys + key_bytes + 64));
     HIPCHK(hipMemcpyAsync(c->pin.p, c->st, sizeof(DevStatus), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(c->pin.p + o_meta, EA.eb.meta, meta_bytes, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(c->pin.p + o_keys, EA.eb.keys, key_bytes, hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     DevStatus stB;
     memcpy(&stB, c->pin.p, sizeof(DevStatus));
@@ -950,7 +950,8 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     stats.entries_kept = stB.n_kept;
     stats.data_block_bytes = stB.data_bytes;
     fill_kernel_stats(c, &stats, W, rounds.size());
-    luda_result* r = finish_result(c, es, EA, out, stB, job->flags, stats);
+    luda_result* r = finish_result(c, es, EA, out, stB, job->flags, stats,
+                                   reinterpret_cast<const FileMeta*>(c->pin.p + o_meta), c->pin.p + o_keys);
     r->stats.total_ms = now_ms() - t_call;
     return r;
   }
@@ -1041,7 +1042,15 @@ luda_result* build_tables(luda_ctx* c, const luda_options* opts, u64 n, const u8
     run_encode(c, es, EA, (u8*)out);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev1, c->s));
+    // status, table metadata and boundary keys (at capacity) in one round trip
+    const u64 o_meta = align_up(sizeof(DevStatus), 64);
+    const u64 meta_bytes = (u64)es.nf_cap * sizeof(FileMeta);
+    const u64 o_keys = align_up(o_meta + meta_bytes, 64);
+    const u64 key_bytes = (u64)es.nf_cap * 2 * es.ep.kw;
+    HIPCHK(c->pin.reserve(o_keys + key_bytes + 64));
     HIPCHK(hipMemcpyAsync(c->pin.p, c->st, sizeof(DevStatus), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(c->pin.p + o_meta, EA.eb.meta, meta_bytes, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(c->pin.p + o_keys, EA.eb.keys, key_bytes, hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     DevStatus stB;
     memcpy(&stB, c->pin.p, sizeof(DevStatus));
@@ -1067,7 +1076,8 @@ luda_result* build_tables(luda_ctx* c, const luda_options* opts, u64 n, const u8
     stats.entries_kept = stB.n_kept;
     stats.data_block_bytes = stB.data_bytes;
     fill_kernel_stats(c, &stats, W, 0);
-    luda_result* r = finish_result(c, es, EA, out, stB, flags, stats);
+    luda_result* r = finish_result(c, es, EA, out, stB, flags, stats,
+                                   reinterpret_cast<const FileMeta*>(c->pin.p + o_meta), c->pin.p + o_keys);
     r->stats.total_ms = now_ms() - t_call;
     return r;
   }

@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) void k_index(const FileDesc* files, const File
     for (u32 i = threadIdx.x; i <= nfiles; i += blockDim.x) s_bb[i] = blk_base[i];
     __syncthreads();
   }
-  u32 maxr = 0, maxn = 0, maxg = 0;
+  u32 maxr = 0, maxn = 0, maxg = 0, maxe = 0;
   GRID_STRIDE(jj, total) {
     const u32 j = (u32)jj;
     blk_nent[j] = 0;
@@ -266,10 +266,12 @@ __global__ __launch_bounds__(256) void k_index(const FileDesc* files, const File
       continue;
     }
     blk_nent[j] = cnt_total;
+    maxe = max(maxe, cnt_total);
   }
   if (maxr) atomicMax(&st->max_rulen, maxr);
   if (maxn) atomicMax(&st->max_blk, maxn);
   if (maxg) atomicMax(&st->max_nr, maxg);
+  if (maxe) atomicMax(&st->max_ent, maxe);
 }
 
 __global__ void k_file_ebase(const u32* blk_base, const u32* blk_ebase, u32 nfiles, const u32* n_entries,
@@ -418,17 +420,17 @@ __device__ __forceinline__ void decode_block(const Src& src, u32 n, u64 gp, u64 
 // Byte windows over one data block: `win8(x)` is bytes [x, x+8) as a
 // big-endian word (bytes past the block are don't-cares).
 struct LdsWin {
-  const u64* w64;  // the LDS stage as 8-byte words
-  u32 mis;         // offset of the block's first byte in the stage
+  static constexpr bool kLds = true;
+  const u8* stage;  // the LDS stage
+  u32 mis;          // offset of the block's first byte in the stage
+  // gfx950 reads 8 bytes at any LDS byte address in one access
   __device__ __forceinline__ u64 win8(u32 x) const {
-    const u32 o = x + mis, a = o >> 3, sft = (o & 7) * 8;
-    const u64 lo = __builtin_bswap64(w64[a]);
-    const u64 hi = __builtin_bswap64(w64[a + 1]);
-    return sft ? (lo << sft) | (hi >> (64 - sft)) : lo;
+    return __builtin_bswap64(*reinterpret_cast<const u64_una*>(stage + mis + x));
   }
-  __device__ __forceinline__ u32 at(u32 x) const { return (u32)(win8(x) >> 56); }
+  __device__ __forceinline__ u32 at(u32 x) const { return stage[mis + x]; }
 };
 struct GlobalWin {
+  static constexpr bool kLds = false;
   const GMEM u8* p;
   u32 n;  // block bytes: nothing at or past n is read
   __device__ __forceinline__ u64 win8(u32 x) const {
@@ -477,9 +479,12 @@ struct KeyBytes {
   }
 };
 
+constexpr u32 kBadOff = 0xffff;  // eoff of an entry whose group failed its checks
+
 template <u32 NWMAX, typename Src>
 __device__ __forceinline__ void decode_entries(const Src& src, u32 n, u64 gp, u64 ebase, u32* gpre, u32 max_nr,
-                                               const Planes& K, u64* vptr, u32* vlen, DevStatus* st, u32& maxu) {
+                                               u16* eoff, u32 ecap, const Planes& K, u64* vptr, u32* vlen,
+                                               DevStatus* st, u32& maxu) {
   const u32 lane = threadIdx.x & 63;
   const u32 NW = K.W + 1;  // words of the internal key: user key words + the tag
   const u32 nr = n >= 4 ? win_ld32(src, n - 4) : 0;
@@ -500,6 +505,36 @@ __device__ __forceinline__ void decode_entries(const Src& src, u32 n, u64 gp, u6
   }
   if (lane == 0) gpre[nr] = total;
   __syncthreads();
+  // A staged block whose entries fit eoff: one lane per restart group walks
+  // the group once (DecodeEntry's checks, the first entry unshared, the last
+  // one ending at the next restart point) and records every entry's offset;
+  // the entry lanes then parse one header each instead of walking from their
+  // group's restart point.
+  const bool walked = Src::kLds && total <= ecap;
+  if (walked) {
+    for (u32 g0 = 0; g0 < nr; g0 += 64) {
+      const u32 g = g0 + lane;
+      if (g < nr) {
+        const u32 start = win_ld32(src, roff + 4 * g) >> 8;
+        const u32 end = g + 1 < nr ? (win_ld32(src, roff + 4 * (g + 1)) >> 8) : roff;
+        const u32 lim = end <= roff ? end : roff;
+        const u32 e0 = gpre[g], cnt = gpre[g + 1] - e0;
+        bool ok = start < end && end <= roff;
+        u32 off = start, i = 0;
+        for (; ok && i < cnt; i++) {
+          eoff[e0 + i] = (u16)off;
+          u32 sh = 0, ns = 0, vl = 0;
+          ok = win_entry_header(src, off, lim, sh, ns, vl);
+          if (ok && i == 0 && sh != 0) ok = false;
+          if (ok) off += ns + vl;
+        }
+        if (ok && off != lim) ok = false;
+        if (!ok)
+          for (u32 t = i ? i - 1 : 0; t < cnt; t++) eoff[e0 + t] = (u16)kBadOff;
+      }
+    }
+    __syncthreads();
+  }
   u64 cm[NWMAX], cv[NWMAX];  // words of the entry before the chunk
 #pragma unroll
   for (u32 w = 0; w < NWMAX; w++) cm[w] = cv[w] = 0;
@@ -508,28 +543,38 @@ __device__ __forceinline__ void decode_entries(const Src& src, u32 n, u64 gp, u6
   for (u32 e0 = 0; e0 < total; e0 += 64) {
     const u32 e = e0 + lane;
     const bool act = e < total;
-    u32 lo = 0, hi = nr;
-    while (hi - lo > 1) {
-      const u32 mid = (lo + hi) >> 1;
-      if (gpre[mid] <= e) lo = mid;
-      else hi = mid;
+    bool ok;
+    u32 off, sh = 0, ns = 0, vl = 0;
+    bool first = false;  // the first entry of its restart group
+    if (walked) {
+      off = act ? eoff[e] : kBadOff;
+      ok = act && off != kBadOff;
+      if (ok) ok = win_entry_header(src, off, n, sh, ns, vl);  // checked by the group's walk
+    } else {
+      u32 lo = 0, hi = nr;
+      while (hi - lo > 1) {
+        const u32 mid = (lo + hi) >> 1;
+        if (gpre[mid] <= e) lo = mid;
+        else hi = mid;
+      }
+      const u32 g = lo;
+      const u32 r = e - gpre[g], cnt = gpre[g + 1] - gpre[g];
+      first = r == 0;
+      const u32 start = win_ld32(src, roff + 4 * g) >> 8;
+      const u32 end = g + 1 < nr ? (win_ld32(src, roff + 4 * (g + 1)) >> 8) : roff;
+      const u32 lim = end <= roff ? end : roff;
+      ok = act && start < end && end <= roff;
+      off = start;
+      for (u32 i = 0; ok && i < r; i++) {
+        ok = win_entry_header(src, off, lim, sh, ns, vl);
+        off += ns + vl;
+      }
+      if (ok) ok = win_entry_header(src, off, lim, sh, ns, vl);
+      if (ok && r == 0 && sh != 0) ok = false;
+      if (ok && r + 1 == cnt && off + ns + vl != lim) ok = false;
     }
-    const u32 g = lo;
-    const u32 r = e - gpre[g], cnt = gpre[g + 1] - gpre[g];
-    const u32 start = win_ld32(src, roff + 4 * g) >> 8;
-    const u32 end = g + 1 < nr ? (win_ld32(src, roff + 4 * (g + 1)) >> 8) : roff;
-    const u32 lim = end <= roff ? end : roff;
-    bool ok = act && start < end && end <= roff;
-    u32 off = start, sh = 0, ns = 0, vl = 0;
-    for (u32 i = 0; ok && i < r; i++) {
-      ok = win_entry_header(src, off, lim, sh, ns, vl);
-      off += ns + vl;
-    }
-    if (ok) ok = win_entry_header(src, off, lim, sh, ns, vl);
     const u32 koff = off, voff = off + ns;
     const u32 ilen = sh + ns;
-    if (ok && r == 0 && sh != 0) ok = false;
-    if (ok && r + 1 == cnt && voff + vl != lim) ok = false;
     // this entry's bytes, positioned in the words of the internal key
     u64 m[NWMAX], v[NWMAX];
 #pragma unroll
@@ -576,7 +621,7 @@ __device__ __forceinline__ void decode_entries(const Src& src, u32 n, u64 gp, u6
     // shared may not exceed the previous key (DecodeEntry's key_.size() >= shared)
     const u32 up = dpp_u32<0x138, 0xf>(ilen);  // wave_shr:1: the previous lane's
     const u32 prev_ilen = lane == 0 ? cilen : up;
-    if (ok && r > 0 && sh > prev_ilen) ok = false;
+    if (ok && !first && sh > prev_ilen) ok = false;  // a group's first entry has sh = 0
     if (ok && ilen < 8) {
       set_err(st, ERR_UNSUPPORTED, SITE_SHORT_KEY);
       ok = false;
@@ -633,9 +678,11 @@ __device__ __forceinline__ void decode_entries(const Src& src, u32 n, u64 gp, u6
 // restart groups' entry prefix follows the stage.
 template <u32 NWMAX>
 __global__ __launch_bounds__(64) void k_decode_entries(const BlockDesc* blks, u32 nb, const u32* blk_ebase, Planes K,
-                                                       u64* vptr, u32* vlen, DevStatus* st, u32 stage, u32 max_nr) {
+                                                       u64* vptr, u32* vlen, DevStatus* st, u32 stage, u32 max_nr,
+                                                       u32 ecap) {
   extern __shared__ uint4 smem[];
   u32* gpre = reinterpret_cast<u32*>(smem + stage / 16);
+  u16* eoff = reinterpret_cast<u16*>(gpre + max_nr + 1);
   const u32 lane = threadIdx.x;
   u32 maxu = 0;
   u32 b = blockIdx.x;
@@ -687,10 +734,11 @@ __global__ __launch_bounds__(64) void k_decode_entries(const BlockDesc* blks, u3
     }
     __syncthreads();
     if (staged) {
-      const LdsWin src{reinterpret_cast<const u64*>(smem), (u32)((u64)gp - a0)};
-      decode_entries<NWMAX>(src, n, (u64)gp, ebase, gpre, max_nr, K, vptr, vlen, st, maxu);
+      const LdsWin src{reinterpret_cast<const u8*>(smem), (u32)((u64)gp - a0)};
+      decode_entries<NWMAX>(src, n, (u64)gp, ebase, gpre, max_nr, eoff, ecap, K, vptr, vlen, st, maxu);
     } else {
-      decode_entries<NWMAX>(GlobalWin{gbytes((u64)gp), n}, n, (u64)gp, ebase, gpre, max_nr, K, vptr, vlen, st, maxu);
+      decode_entries<NWMAX>(GlobalWin{gbytes((u64)gp), n}, n, (u64)gp, ebase, gpre, max_nr, eoff, 0, K, vptr, vlen, st,
+                            maxu);
     }
     __syncthreads();
   }
@@ -797,22 +845,29 @@ void launch_file_ebase(KTimer& T, const u32* blk_base, const u32* blk_ebase, u32
           file_ebase);
 }
 void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 nb, const u32* blk_ebase, Planes K,
-                   u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr) {
+                   u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr, u32 max_ent) {
   if (!nb) return;
   if (K.W <= 8) {
     // stage: the largest block, its misalignment, and the 16 bytes an 8-byte
-    // window may read past it; blocks over 16 KiB are read in place
+    // window may read past it; blocks over 16 KiB are read in place.  Then
+    // the restart groups' entry prefix and, as far as 64 KiB allow, the
+    // entry offsets.
     const u32 stage = (u32)((std::min<u64>(max_blk, 16384) + 15 + 16 + 15) & ~15ull);
-    const size_t lds = stage + 4ull * (max_nr + 1);
+    size_t lds = stage + 4ull * (max_nr + 1);
     if (lds > 65536) launch_refused("decode (restart arrays too long for LDS)", hipErrorInvalidValue);
+    const u32 ecap = (u32)std::min<u64>((u64)max_ent, (65536 - lds) / 2);
+    lds += ((2ull * ecap + 3) & ~3ull);
     // a few blocks per wavefront, so that the next descriptor's load overlaps
     const u32 grid = std::max<u32>(1, std::min<u32>((nb + 3) / 4, 4096));
     if (K.W <= 2)
-      KLAUNCH(T, "decode", k_decode_entries<3>, grid, 64, lds, blks, nb, blk_ebase, K, vptr, vlen, st, stage, max_nr);
+      KLAUNCH(T, "decode", k_decode_entries<3>, grid, 64, lds, blks, nb, blk_ebase, K, vptr, vlen, st, stage, max_nr,
+              ecap);
     else if (K.W <= 4)
-      KLAUNCH(T, "decode", k_decode_entries<5>, grid, 64, lds, blks, nb, blk_ebase, K, vptr, vlen, st, stage, max_nr);
+      KLAUNCH(T, "decode", k_decode_entries<5>, grid, 64, lds, blks, nb, blk_ebase, K, vptr, vlen, st, stage, max_nr,
+              ecap);
     else
-      KLAUNCH(T, "decode", k_decode_entries<9>, grid, 64, lds, blks, nb, blk_ebase, K, vptr, vlen, st, stage, max_nr);
+      KLAUNCH(T, "decode", k_decode_entries<9>, grid, 64, lds, blks, nb, blk_ebase, K, vptr, vlen, st, stage, max_nr,
+              ecap);
     return;
   }
   const u32 grid = nb < 65536 ? nb : 65536;

@@ -232,25 +232,44 @@ __global__ __launch_bounds__(256) void k_bnext(Kept k, const u32* m_dev, EncPara
     }
     __syncthreads();
     u32 ub = 0;  // the upper bound of the previous start
+    // entries per block at the tile's mean entry size: the first search's guess
+    const u64 tbytes = sP[lim] & kByteMask;
+    const u32 est = tbytes ? (u32)min<u64>((u64)lim, (bs * lim) / tbytes) : 1;
     for (u32 s = 0; s < 4; s++) {
       const u32 x = 4 * tid + s;
       const u64 q = q0 + x;
       if (x >= kBnStarts || q >= m) break;
       const u64 px = sP[x];
       const u64 bx = px & kByteMask;
-      // smallest y in (x, lim] with bytes(x, y) + 8 >= bs, else lim + 1
-      u32 y;
-      if (s == 0) {
-        u32 lo = x + 1, hi = lim + 1;
+      // smallest y in (x, lim] with bytes(x, y) + 8 >= bs, else lim + 1:
+      // for the thread's first start from the tile's mean entry size, for
+      // the others from the previous start's bound, stepping at most 16
+      // entries before a binary search takes over
+      auto reaches = [&](u32 yy) { return (sP[yy] & kByteMask) - bx + 8 >= bs; };
+      u32 y = s == 0 ? min(lim, x + 1 + est) : max(ub, x + 1);
+      u32 lo = x + 1, hi = lim + 1;  // the answer lies in [lo, hi]
+      bool done = false;
+      for (u32 step = 0; step < 16 && !done; step++) {
+        if (y > lim || reaches(y)) {
+          hi = min(hi, y);
+          if (y == lo || (y <= lim && !reaches(y - 1))) done = true;
+          else y--;
+        } else {
+          lo = y + 1;
+          y++;
+          if (y > lim) {
+            y = lim + 1;
+            done = true;
+          }
+        }
+      }
+      if (!done) {
         while (lo < hi) {
           const u32 mid = (lo + hi) >> 1;
-          if ((sP[mid] & kByteMask) - bx + 8 >= bs) hi = mid;
+          if (reaches(mid)) hi = mid;
           else lo = mid + 1;
         }
         y = lo;
-      } else {
-        y = max(ub, x + 1);
-        while (y <= lim && (sP[y] & kByteMask) - bx + 8 < bs) y++;
       }
       ub = y;
       // exact S(x, y0) for y0 = min(y, lim); g restart points
@@ -264,14 +283,12 @@ __global__ __launch_bounds__(256) void k_bnext(Kept k, const u32* m_dev, EncPara
       if (found) {
         // walk back while the block one entry shorter still reaches bs
         end = y0;
-        u32 gg = g;
         while (end > x + 1) {
           const u32 e = end - 1;  // the entry that would be dropped
           const bool rs = (e - x) % R == 0;
           const u64 sz = (sP[e + 1] & kByteMask) - (sP[e] & kByteMask) + (rs ? (u64)sD[e] + 4 : 0);
           if (S - sz < bs) break;
           S -= sz;
-          gg -= rs;
           end = e;
         }
       } else {
@@ -502,12 +519,6 @@ __global__ void k_file_sizes(const u32* nf_dev, EncParams ep, EncBufs b) {
 // ---------------------------------------------------------------------------
 // writers
 
-__device__ __forceinline__ void load_crc_tables(u32 (*sl)[256], u32 (*z)[256]) {
-  for (u32 i = threadIdx.x; i < 4 * 256; i += blockDim.x) {
-    sl[i >> 8][i & 255] = g_crc.slice[i >> 8][i & 255];
-    z[i >> 8][i & 255] = g_crc.zeros[6][i >> 8][i & 255];
-  }
-}
 
 
 // The internal key user_key || fixed64(tag) of entry q as NWMAX big-endian
@@ -1650,7 +1661,7 @@ void launch_entry_sizes(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncBufs 
   KLAUNCH(T, "entry_sizes", k_entry_sizes, grid_for(m_cap, 256), 256, 0, k, m_dev, b, st);
 }
 void launch_bnext(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncParams ep, EncBufs b, DevStatus* st) {
-  KLAUNCH(T, "bnext", k_bnext, std::min<u32>(grid_for(m_cap, kBnStarts), 2048), 256, 0, k, m_dev, ep, b, st);
+  KLAUNCH(T, "bnext", k_bnext, std::min<u32>(grid_for(m_cap, kBnStarts), 8192), 256, 0, k, m_dev, ep, b, st);
 }
 void launch_block_info(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncBufs b, DevStatus* st) {
   KLAUNCH(T, "block_info", k_block_info, grid_for(m_cap, 256), 256, 0, k, m_dev, b, st);

@@ -26,6 +26,25 @@ __device__ __forceinline__ u64 level_items(u64 n, u32 C, u32 G, u32 level) {
   return it;
 }
 
+// dst[i] = src[base + i] (fill past n) for i < cnt, by the whole workgroup,
+// eight loads in flight per thread before any is stored.
+__device__ __forceinline__ void stage_u32(u32* dst, const u32* src, u64 base, u32 cnt, u64 n, u32 fill) {
+  constexpr u32 kBatch = 8;
+  for (u32 i0 = 0; i0 < cnt; i0 += blockDim.x * kBatch) {
+    u32 v[kBatch];
+#pragma unroll
+    for (u32 t = 0; t < kBatch; t++) {
+      const u32 i = i0 + t * blockDim.x + threadIdx.x;
+      v[t] = (i < cnt && base + i < n) ? src[base + i] : fill;
+    }
+#pragma unroll
+    for (u32 t = 0; t < kBatch; t++) {
+      const u32 i = i0 + t * blockDim.x + threadIdx.x;
+      if (i < cnt) dst[i] = v[t];
+    }
+  }
+}
+
 __global__ __launch_bounds__(kThreads) void k_orbit_chunk(const u32* nxt, const u32* n_dev, const u32* step_dev,
                                                            u32 C, u32 Wc, u16* f, DevStatus* st) {
   extern __shared__ u32 ex[];
@@ -40,7 +59,7 @@ __global__ __launch_bounds__(kThreads) void k_orbit_chunk(const u32* nxt, const 
   for (u64 c = blockIdx.x; c < nch; c += gridDim.x) {
     const u64 c0 = c * C;
     const u64 cend = c0 + C;
-    for (u32 i = threadIdx.x; i < C; i += blockDim.x) ex[i] = c0 + i < n ? nxt[c0 + i] : (u32)n;
+    stage_u32(ex, nxt, c0, C, n, (u32)n);
     __syncthreads();
     // pointer jumping until every pointer has left the chunk (at most
     // log2(C) + 1 rounds; fewer when the paths through the chunk are short)
@@ -162,10 +181,21 @@ __global__ __launch_bounds__(1024) void k_orbit_top(u16* ftab, u32* entries, con
   }
   const bool staged = loff[L.levels + 1] <= lds_cap;
   if (staged) {
+    // eight loads in flight per thread before any is stored
     const u32 t0 = items[0] * Wd;
-    for (u32 i = tid; i < t0; i += blockDim.x) {
-      const u32 o = i / Wd;
-      lt[i] = ftab[(u64)o * Wc + (i - o * Wd)];
+    for (u32 i0 = 0; i0 < t0; i0 += 8 * blockDim.x) {
+      u16 v[8];
+#pragma unroll
+      for (u32 t = 0; t < 8; t++) {
+        const u32 i = i0 + t * blockDim.x + tid;
+        const u32 o = i / Wd;
+        v[t] = i < t0 ? ftab[(u64)o * Wc + (i - o * Wd)] : (u16)0;
+      }
+#pragma unroll
+      for (u32 t = 0; t < 8; t++) {
+        const u32 i = i0 + t * blockDim.x + tid;
+        if (i < t0) lt[i] = v[t];
+      }
     }
     __syncthreads();
   }
@@ -224,10 +254,8 @@ __global__ __launch_bounds__(kThreads) void k_orbit_mark(const u32* nxt, const u
   for (u64 c = blockIdx.x; c < nch; c += gridDim.x) {
     const u64 c0 = c * C;
     const u64 c1 = min(c0 + C, n);
-    for (u32 i = threadIdx.x; i < C; i += blockDim.x) {
-      loc[i] = c0 + i < n ? nxt[c0 + i] : (u32)n;
-      fl[i] = 0;
-    }
+    stage_u32(loc, nxt, c0, C, n, (u32)n);
+    for (u32 i = threadIdx.x; i < C; i += blockDim.x) fl[i] = 0;
     __syncthreads();
     if (threadIdx.x == 0) {
       const u32 y = e0[c];
