@@ -107,6 +107,101 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
 
 
+def mix_jobs(eng, opts, n, seed):
+    """n compaction jobs of log-uniform input sizes in [4, 256] MiB, inputs
+    built on the GPU and resident in HBM: Ln->Ln+1 jobs with the 256 MiB
+    job's key/value distribution, bottommost jobs with the 512 MiB job's,
+    and L0->L1 jobs (4 overlapping L0 tables + L1 tables, up to 40 MiB)."""
+    import random
+    rng = random.Random(seed)
+    out = []
+    for k in range(n):
+        mib = 4.0 * (64.0 ** rng.random())  # log-uniform 4 .. 256
+        shape = [1, 2, 0][k % 3]
+        if shape == 0 and mib > 40:
+            shape = 1
+        scale = mib / {0: 20.0, 1: 256.0, 2: 512.0}[shape]
+        wl = jobs.Workload(shape, seed=0x4C554441 + 1000 + k, scale=scale)
+        dev, keep = build_device_inputs(eng, wl, opts)
+        job = jobs.make_job(wl, opts, None, flags=ffi.LUDA_JOB_KEEP_ON_DEVICE, device_inputs=dev)
+        out.append((shape, sum(s for which in dev for _, s in which), job, keep, wl))
+    return out
+
+
+def run_mix(args):
+    """Batched form against the plain one-job-at-a-time path on the same jobs:
+    aggregate input GB/s and job latency (from the moment all jobs are
+    queued), plus whether every batched result equals the plain one."""
+    opts = ffi.default_options()
+    eng = jobs.Engine(0)
+    t = time.time()
+    mix = mix_jobs(eng, opts, args.mix, args.mix_seed)
+    total = sum(b for _, b, _, _, _ in mix)
+    log(f"{len(mix)} jobs, {total / 2**30:.2f} GiB of inputs built in {time.time() - t:.1f}s")
+
+    def summary(res):
+        r = res.contents
+        return [eng.d2h(r.device_data + r.outputs[i].offset, r.outputs[i].file_size)
+                for i in range(r.num_outputs)]
+    # plain: one luda_compact after another
+    for shape, b, job, keep, wl in mix[:2]:  # warm-up
+        rc, o, msg = eng.compact(job.job, keep_on_device=True)
+        eng.free(o[2])
+    plain_hash, plain_lat = [], []
+    t0 = time.perf_counter()
+    for shape, b, job, keep, wl in mix:
+        rc, o, msg = eng.compact(job.job, keep_on_device=True)
+        if rc != ffi.LUDA_OK:
+            raise RuntimeError(f"luda_compact failed ({rc}): {msg}")
+        plain_lat.append((time.perf_counter() - t0) * 1e3)
+        plain_hash.append(o[2])
+    t_plain = time.perf_counter() - t0
+    plain_hash = [[sha(x) for x in summary(res)] for res in plain_hash_free(eng, plain_hash, summary)]
+    # batched: everything queued at once on the worker slots
+    beng = jobs.Engine(0)
+    beng.set_slots(args.slots)
+    for shape, b, job, keep, wl in mix[:2]:  # warm-up of every slot's buffers
+        beng.submit(job.job, lambda rc, o, msg: beng.free(o[2]), keep_on_device=True)
+    beng.drain()
+    got = {}
+    t0 = time.perf_counter()
+    for i, (shape, b, job, keep, wl) in enumerate(mix):
+        beng.submit(job.job, lambda rc, o, msg, i=i: got.__setitem__(i, (rc, o, (time.perf_counter() - t0) * 1e3)),
+                    keep_on_device=True)
+    beng.drain()
+    t_batch = time.perf_counter() - t0
+    same = True
+    batch_lat = []
+    for i in range(len(mix)):
+        rc, o, ms = got[i]
+        batch_lat.append(ms)
+        same &= rc == ffi.LUDA_OK and [sha(x) for x in summary(o[2])] == plain_hash[i]
+        beng.free(o[2])
+    line = {
+        "metric": "batched compaction input GB/s vs the plain path; p50/p99 job latency",
+        "jobs": len(mix), "input_bytes": total, "slots": args.slots,
+        "plain_gbps": round(total / t_plain / 1e9, 3), "batched_gbps": round(total / t_batch / 1e9, 3),
+        "speedup": round(t_plain / t_batch, 3),
+        "plain_p50_ms": round(percentile(plain_lat, 50), 3), "plain_p99_ms": round(percentile(plain_lat, 99), 3),
+        "batched_p50_ms": round(percentile(batch_lat, 50), 3), "batched_p99_ms": round(percentile(batch_lat, 99), 3),
+        "same_tables": same,
+        "workload": "log-uniform 4-256 MiB jobs: Ln->Ln+1 (16-byte keys, 100-byte values), bottommost "
+                    "(Zipf 8-64-byte keys, 16-1024-byte values) and L0->L1 (up to 40 MiB), inputs in HBM",
+    }
+    print(json.dumps(line), flush=True)
+    beng.close()
+    for shape, b, job, keep, wl in mix:
+        for res in keep:
+            eng.free(res)
+    eng.close()
+
+
+def plain_hash_free(eng, results, summary):
+    for res in results:
+        yield res
+        eng.free(res)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,12 +211,20 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mix", type=int, default=0, metavar="JOBS",
+                    help="instead: JOBS jobs of 4-256 MiB through the batched form (luda_submit) against the "
+                         "plain path, one JSON line of aggregate GB/s and job latency")
+    ap.add_argument("--mix-seed", type=int, default=3)
+    ap.add_argument("--slots", type=int, default=3)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (the known-size kernel that rocprofv3's "
                          "FETCH_SIZE/WRITE_SIZE are scaled against) and exit")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.mix:
+        run_mix(args)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: a compaction job runs whole on one device and this build has no "

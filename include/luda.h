@@ -168,6 +168,30 @@ int luda_build_tables(luda_ctx* ctx, const luda_options* opts, uint64_t n, const
 
 void luda_free_result(luda_ctx* ctx, luda_result* res);
 
+/* Batched and served compaction: jobs queued on the context run on worker
+ * slots, each with its own HIP stream and working memory, so that the
+ * kernels of several jobs overlap on the device.  A queued job starts when a
+ * slot is free and the device memory the running jobs are estimated to hold
+ * (about 4.5x their input bytes each) leaves room for it under the context's
+ * budget; a job is always admitted when no other job runs.  Replaces the
+ * one-compaction-at-a-time gate of DBImpl::BackgroundCompaction
+ * (db/db_impl.cc:865-871) and the GPU flush gate (:94-97); the reference has
+ * no queued form.
+ *
+ * `done` runs on the thread that calls luda_poll or luda_drain, once per job,
+ * with the job's status, its result (LUDA_OK only; free it with
+ * luda_free_result on this context) and an error message.  The job, its
+ * inputs and `user` must stay valid until then.  Results are the same as
+ * luda_compact's for the same job. */
+typedef void (*luda_done_fn)(void* user, int status, luda_result* result, const char* err);
+/* Worker slots (default 3); only before the first luda_submit. */
+int luda_set_slots(luda_ctx* ctx, uint32_t slots);
+int luda_submit(luda_ctx* ctx, const luda_job* job, luda_done_fn done, void* user);
+/* Runs the callbacks of finished jobs; returns how many (never blocks). */
+int luda_poll(luda_ctx* ctx);
+/* Waits for every submitted job and runs its callback; returns how many. */
+int luda_drain(luda_ctx* ctx);
+
 /* Device-buffer helpers so that a caller with no HIP code of its own can keep
  * inputs resident in HBM (LUDA_JOB_INPUTS_ON_DEVICE) and read outputs back. */
 int luda_device_alloc(luda_ctx* ctx, uint64_t bytes, void** dptr);

@@ -145,7 +145,7 @@ __device__ __forceinline__ void st32(u8* p, u32 v) {
 // first four bytes (a zero-initialised register fed X equals a ~0-initialised
 // one fed X with its first 32 bits flipped), so zero padding is neutral.
 struct CrcTables {
-  u32 slice[4][256];                   // slice[k][b]: byte b followed by k zero bytes, from a zero register
+  u32 slice[8][256];                   // slice[k][b]: byte b followed by k zero bytes, from a zero register
   u32 zeros[CRC_ZERO_LEVELS][4][256];  // zeros[j]: "append 4*2^j zero bytes", one table per register byte
   constexpr CrcTables() : slice(), zeros() {
     for (u32 i = 0; i < 256; i++) {
@@ -153,7 +153,7 @@ struct CrcTables {
       for (int k = 0; k < 8; k++) c = (c & 1) ? (c >> 1) ^ 0x82F63B78u : (c >> 1);
       slice[0][i] = c;
     }
-    for (int k = 1; k < 4; k++)
+    for (int k = 1; k < 8; k++)
       for (u32 i = 0; i < 256; i++) {
         u32 s = slice[k - 1][i];
         slice[k][i] = slice[0][s & 0xff] ^ (s >> 8);

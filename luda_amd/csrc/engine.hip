@@ -16,8 +16,11 @@
 #include <algorithm>
 #include <chrono>
 #include <map>
+#include <condition_variable>
+#include <deque>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "engine.h"
@@ -159,8 +162,12 @@ struct EncodeSetup {
 
 }  // namespace
 
+struct JobPool;
+
 struct luda_ctx {
   int device = 0;
+  JobPool* pool = nullptr;  // worker slots of luda_submit, created on first use
+  u32 slots = 3;
   hipStream_t s = nullptr;
   u64 budget = 0;
   std::mutex mu;
@@ -912,11 +919,14 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     float dms = 0;
     hipEventElapsedTime(&dms, c->ev0, c->ev1);
     stats.device_ms = dms;
-    if (status_code_of(stB) != LUDA_OK) {
+    // An overflow (keys wider than the planes, a capacity guessed too low)
+    // is retried first: what the kernels after it reported may come from the
+    // truncated keys or clipped counts; a real fault shows again next time.
+    const bool overflow = (stB.err & ERR_OVERFLOW) || stB.out_bytes > out_cap;
+    if (status_code_of(stB) != LUDA_OK && (!overflow || attempt >= kMaxAttempts)) {
       c->give_out(out);
       throw Fail{status_code_of(stB), std::string("compaction rejected: ") + site_name(stB.site)};
     }
-    const bool overflow = (stB.err & ERR_OVERFLOW) || stB.out_bytes > out_cap;
     if (overflow) {
       c->give_out(out);
       if (attempt >= kMaxAttempts) throw Fail{LUDA_NOT_SUPPORTED, std::string("capacity: ") + site_name(stB.site)};
@@ -941,7 +951,10 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
         nbo_min = (u64)stB.need_blocks + 16;
         grew = true;
       }
-      if (!grew) throw Fail{LUDA_NOT_SUPPORTED, std::string("capacity: ") + site_name(stB.site)};
+      if (!grew) {
+        const int code = status_code_of(stB);
+        throw Fail{code != LUDA_OK ? code : LUDA_NOT_SUPPORTED, std::string("capacity: ") + site_name(stB.site)};
+      }
       continue;
     }
     c->kw_hint = std::max<u32>(c->kw_hint, (stB.max_ulen + 7) / 8);
@@ -1057,11 +1070,12 @@ luda_result* build_tables(luda_ctx* c, const luda_options* opts, u64 n, const u8
     float dms = 0;
     hipEventElapsedTime(&dms, c->ev0, c->ev1);
     stats.device_ms = dms;
-    if (status_code_of(stB) != LUDA_OK) {
+    const bool overflow = (stB.err & ERR_OVERFLOW) || stB.out_bytes > out_cap;
+    if (status_code_of(stB) != LUDA_OK && (!overflow || attempt >= kMaxAttempts)) {
       c->give_out(out);
       throw Fail{status_code_of(stB), std::string("build rejected: ") + site_name(stB.site)};
     }
-    if ((stB.err & ERR_OVERFLOW) || stB.out_bytes > out_cap) {
+    if (overflow) {
       c->give_out(out);
       if (attempt >= kMaxAttempts) throw Fail{LUDA_NOT_SUPPORTED, std::string("capacity: ") + site_name(stB.site)};
       if (stB.out_bytes > out_cap) out_cap = stB.out_bytes + (1 << 20);
@@ -1096,6 +1110,138 @@ int guarded(luda_ctx* c, char* err, size_t errlen, F fn) {
   }
 }
 
+// The worker slots of luda_submit.  Each slot is a context of its own
+// (stream, working memory, output buffers) driven by one host thread; the
+// queue admits a job when a slot is idle and the estimated device memory of
+// the running jobs plus this one fits the budget (or nothing runs).
+struct QueuedJob {
+  const luda_job* job;
+  luda_done_fn done;
+  void* user;
+  u64 est;  // estimated device bytes while it runs
+};
+struct DoneJob {
+  luda_done_fn done;
+  void* user;
+  int status;
+  luda_result* result;
+  std::string err;
+  luda_ctx* owner;
+};
+
+u64 job_footprint(const luda_job* j) {
+  u64 in = 0;
+  for (int w = 0; w < 2; w++)
+    for (u32 i = 0; i < j->num_inputs[w]; i++) in += j->inputs[w][i].size;
+  return in * 9 / 2 + (64ull << 20);
+}
+
+}  // namespace
+
+struct JobPool {
+  std::vector<luda_ctx*> ctxs;
+  std::vector<std::thread> threads;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<QueuedJob> queue;
+  std::deque<DoneJob> done;
+  std::map<void*, luda_ctx*> owner;  // device output buffer -> the slot that owns it
+  u64 budget = 0, inflight = 0;
+  u32 running = 0, submitted = 0, delivered = 0;
+  bool stop = false;
+};
+
+namespace {
+
+int ctx_open(int device, u64 budget, luda_ctx** out, char* err, size_t errlen);
+void ctx_close(luda_ctx* c);
+int ctx_compact(luda_ctx* c, const luda_job* job, luda_result** out, char* err, size_t errlen);
+
+void pool_worker(JobPool* P, luda_ctx* w) {
+  hipSetDevice(w->device);
+  std::unique_lock<std::mutex> lk(P->mu);
+  for (;;) {
+    P->cv_work.wait(lk, [&] {
+      return P->stop || (!P->queue.empty() && (P->running == 0 || P->inflight + P->queue.front().est <= P->budget));
+    });
+    if (P->stop && P->queue.empty()) return;
+    QueuedJob q = P->queue.front();
+    P->queue.pop_front();
+    P->running++;
+    P->inflight += q.est;
+    lk.unlock();
+    luda_result* r = nullptr;
+    char err[256] = {0};
+    const int rc = ctx_compact(w, q.job, &r, err, sizeof(err));
+    lk.lock();
+    P->running--;
+    P->inflight -= q.est;
+    if (r && r->device_data) P->owner[r->device_data] = w;
+    P->done.push_back(DoneJob{q.done, q.user, rc, r, err, w});
+    P->cv_done.notify_all();
+    P->cv_work.notify_all();
+  }
+}
+
+JobPool* pool_of(luda_ctx* c) {
+  if (c->pool) return c->pool;
+  JobPool* P = new JobPool;
+  P->budget = c->budget;
+  for (u32 i = 0; i < c->slots; i++) {
+    luda_ctx* w = nullptr;
+    char err[256];
+    if (ctx_open(c->device, c->budget, &w, err, sizeof(err)) != LUDA_OK) break;
+    P->ctxs.push_back(w);
+  }
+  if (P->ctxs.empty()) {
+    delete P;
+    return nullptr;
+  }
+  for (luda_ctx* w : P->ctxs) P->threads.emplace_back(pool_worker, P, w);
+  c->pool = P;
+  return P;
+}
+
+void pool_close(JobPool* P) {
+  {
+    std::lock_guard<std::mutex> lk(P->mu);
+    P->stop = true;
+  }
+  P->cv_work.notify_all();
+  for (auto& t : P->threads) t.join();
+  for (auto& d : P->done)
+    if (d.result) {
+      // undelivered results: return their buffers before the slots go
+      if (d.result->device_data) {
+        std::lock_guard<std::mutex> lk(d.owner->mu);
+        d.owner->give_out(d.result->device_data);
+      }
+      if (d.result->data) hipHostFree(d.result->data);
+      free(d.result->outputs);
+      free(d.result->keys);
+      free(d.result);
+    }
+  for (luda_ctx* w : P->ctxs) ctx_close(w);
+  delete P;
+}
+
+int pool_deliver(JobPool* P, bool wait_all) {
+  int n = 0;
+  for (;;) {
+    DoneJob d;
+    {
+      std::unique_lock<std::mutex> lk(P->mu);
+      if (wait_all) P->cv_done.wait(lk, [&] { return !P->done.empty() || P->delivered + P->done.size() == P->submitted; });
+      if (P->done.empty()) return n;
+      d = std::move(P->done.front());
+      P->done.pop_front();
+      P->delivered++;
+    }
+    if (d.done) d.done(d.user, d.status, d.result, d.err.c_str());
+    n++;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1103,6 +1249,51 @@ extern "C" {
 const char* luda_version(void) { return "luda-mi355x 0.1 (gfx950)"; }
 
 int luda_init(int device, uint64_t hbm_budget_bytes, luda_ctx** out, char* err, size_t errlen) {
+  return ctx_open(device, hbm_budget_bytes, out, err, errlen);
+}
+
+void luda_shutdown(luda_ctx* c) {
+  if (!c) return;
+  if (c->pool) pool_close(c->pool);
+  c->pool = nullptr;
+  ctx_close(c);
+}
+
+int luda_set_slots(luda_ctx* c, uint32_t slots) {
+  if (!c || slots < 1 || slots > 16) return LUDA_INVALID_ARGUMENT;
+  if (c->pool) return LUDA_INVALID_ARGUMENT;
+  c->slots = slots;
+  return LUDA_OK;
+}
+
+int luda_submit(luda_ctx* c, const luda_job* job, luda_done_fn done, void* user) {
+  if (!c || !job) return LUDA_INVALID_ARGUMENT;
+  JobPool* P = pool_of(c);
+  if (!P) return LUDA_IO_ERROR;
+  {
+    std::lock_guard<std::mutex> lk(P->mu);
+    P->queue.push_back(QueuedJob{job, done, user, job_footprint(job)});
+    P->submitted++;
+  }
+  P->cv_work.notify_one();
+  return LUDA_OK;
+}
+
+int luda_poll(luda_ctx* c) {
+  if (!c || !c->pool) return 0;
+  return pool_deliver(c->pool, false);
+}
+
+int luda_drain(luda_ctx* c) {
+  if (!c || !c->pool) return 0;
+  return pool_deliver(c->pool, true);
+}
+
+}  // extern "C"
+
+namespace {
+
+int ctx_open(int device, uint64_t hbm_budget_bytes, luda_ctx** out, char* err, size_t errlen) {
   if (!out) return LUDA_INVALID_ARGUMENT;
   *out = nullptr;
   int ndev = 0;
@@ -1123,14 +1314,14 @@ int luda_init(int device, uint64_t hbm_budget_bytes, luda_ctx** out, char* err, 
     c->T.s = c->s;
   });
   if (rc != LUDA_OK) {
-    luda_shutdown(c);
+    ctx_close(c);
     return rc;
   }
   *out = c;
   return LUDA_OK;
 }
 
-void luda_shutdown(luda_ctx* c) {
+void ctx_close(luda_ctx* c) {
   if (!c) return;
   if (c->s) hipStreamSynchronize(c->s);
   c->in_buf.release();
@@ -1147,12 +1338,20 @@ void luda_shutdown(luda_ctx* c) {
   delete c;
 }
 
-int luda_compact(luda_ctx* c, const luda_job* job, luda_result** out, char* err, size_t errlen) {
+int ctx_compact(luda_ctx* c, const luda_job* job, luda_result** out, char* err, size_t errlen) {
   if (!c || !job || !out) return LUDA_INVALID_ARGUMENT;
   *out = nullptr;
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);
   return guarded(c, err, errlen, [&] { *out = compact(c, job); });
+}
+
+}  // namespace
+
+extern "C" {
+
+int luda_compact(luda_ctx* c, const luda_job* job, luda_result** out, char* err, size_t errlen) {
+  return ctx_compact(c, job, out, err, errlen);
 }
 
 int luda_build_tables(luda_ctx* c, const luda_options* opts, uint64_t n, const uint8_t* keys, const uint64_t* key_off,
@@ -1168,8 +1367,18 @@ int luda_build_tables(luda_ctx* c, const luda_options* opts, uint64_t n, const u
 void luda_free_result(luda_ctx* c, luda_result* r) {
   if (!r) return;
   if (c && r->device_data) {
-    std::lock_guard<std::mutex> lk(c->mu);
-    c->give_out(r->device_data);
+    // a result of luda_submit returns its buffer to the slot that made it
+    luda_ctx* owner = c;
+    if (c->pool) {
+      std::lock_guard<std::mutex> lk(c->pool->mu);
+      auto it = c->pool->owner.find(r->device_data);
+      if (it != c->pool->owner.end()) {
+        owner = it->second;
+        c->pool->owner.erase(it);
+      }
+    }
+    std::lock_guard<std::mutex> lk(owner->mu);
+    owner->give_out(r->device_data);
   }
   if (r->data) hipHostFree(r->data);
   free(r->outputs);

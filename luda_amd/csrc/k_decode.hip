@@ -16,6 +16,7 @@
 namespace {
 
 constexpr u32 kStage = 8192;  // bytes of a data block staged in LDS
+typedef u32 u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ u32 wave_incl_scan_u32(u32 v) { return wave_incl_sum(v); }
 
@@ -650,7 +651,10 @@ __device__ __forceinline__ void decode_entries(const Src& src, u32 n, u64 gp, u6
         K.klen[q] = ulen;
         K.tag[q] = __builtin_bswap64(be);
       } else {
-        if (ok) set_err(st, ERR_OVERFLOW, SITE_KEY_WIDTH);  // the host retries with wider planes
+        if (ok) {  // the host retries with planes as wide as this key
+          set_err(st, ERR_OVERFLOW, SITE_KEY_WIDTH);
+          maxu = max(maxu, ilen - 8);
+        }
 #pragma unroll
         for (u32 w = 0; w + 1 < NWMAX; w++)
           if (w < K.W) K.P[w * K.stride + q] = 0;
@@ -702,9 +706,10 @@ __global__ __launch_bounds__(64) void k_decode_entries(const BlockDesc* blks, u3
     const u64 a1 = ((u64)gp + n + 15) & ~15ull;
     const bool staged = a1 - a0 + 16 <= stage;
     const u32 n16 = staged ? (u32)((a1 - a0) / 16) : 0;
-    const GMEM u64* g8 = (const GMEM u64*)a0;
+    const GMEM u32x4* g16 = (const GMEM u32x4*)a0;
     u64* s8 = reinterpret_cast<u64*>(smem);
-    // every staging load of a batch in flight before the first LDS store
+    // every staging load of a batch (16 bytes per lane, a wave covering 1 KiB
+    // contiguously) in flight before the first LDS store
     constexpr u32 kBatch = 8;
     for (u32 x0 = 0; x0 < n16; x0 += 64 * kBatch) {
       u64 lo[kBatch], hi[kBatch];
@@ -712,8 +717,9 @@ __global__ __launch_bounds__(64) void k_decode_entries(const BlockDesc* blks, u3
       for (u32 t = 0; t < kBatch; t++) {
         const u32 x = x0 + 64 * t + lane;
         const u32 xc = x < n16 ? x : 0;  // in-bounds index for idle lanes
-        lo[t] = g8[2 * xc];
-        hi[t] = g8[2 * xc + 1];
+        const u32x4 v = g16[xc];
+        lo[t] = (u64)v[0] | ((u64)v[1] << 32);
+        hi[t] = (u64)v[2] | ((u64)v[3] << 32);
       }
       if (x0 == 0 && b + gridDim.x < nb) {
         nd = blks[b + gridDim.x];

@@ -192,35 +192,43 @@ __global__ __launch_bounds__(256) void k_bnext(Kept k, const u32* m_dev, EncPara
   for (u64 q0 = (u64)blockIdx.x * kBnStarts; q0 < m; q0 += (u64)gridDim.x * kBnStarts) {
     const u32 lim = (u32)min((u64)kBnTile, m - q0);  // staged entries [q0, q0 + lim)
     // stage: thread t owns staged entries [kBnPer t, kBnPer t + kBnPer)
-    // all loads first: a possible error report between them would order them
-    u32 r[kBnPer], nr[kBnPer], fo[kBnPer];
+    // coalesced loads (all issued before any is used) into LDS: the
+    // surcharges to sD, the packed sizes to sP (shifted by one); then each
+    // thread takes its kBnPer consecutive entries from there
+    {
+      u32 r[kBnPer], nr[kBnPer], fo[kBnPer];
 #pragma unroll
-    for (u32 i = 0; i < kBnPer; i++) {
-      const u32 x = kBnPer * tid + i;
-      const u64 qx = q0 + (x < lim ? x : 0);
-      r[i] = b.szr[qx];
-      nr[i] = b.sznr[qx];
-      fo[i] = k.forced[qx];
+      for (u32 i = 0; i < kBnPer; i++) {
+        const u32 x = 256 * i + tid;
+        const u64 qx = q0 + (x < lim ? x : 0);
+        r[i] = b.szr[qx];
+        nr[i] = b.sznr[qx];
+        fo[i] = k.forced[qx];
+      }
+      bool bad = false;
+#pragma unroll
+      for (u32 i = 0; i < kBnPer; i++) {
+        const u32 x = 256 * i + tid;
+        if (x < lim) {
+          bad |= r[i] >= (1u << 31) || nr[i] >= (1u << 31) || r[i] < nr[i];
+          sD[x] = r[i] - nr[i];
+          sP[x + 1] = (u64)nr[i] + (fo[i] ? kCfOne : 0);
+        }
+      }
+      if (bad) set_err(st, ERR_UNSUPPORTED, SITE_BLOCK_ENTRIES);
     }
+    __syncthreads();
     u64 v[kBnPer];
     u64 tot = 0;
-    bool bad = false;
 #pragma unroll
     for (u32 i = 0; i < kBnPer; i++) {
       const u32 x = kBnPer * tid + i;
-      u64 e = 0;
-      if (x < lim) {
-        bad |= r[i] >= (1u << 31) || nr[i] >= (1u << 31) || r[i] < nr[i];
-        e = (u64)nr[i] + (fo[i] ? kCfOne : 0);
-        sD[x] = r[i] - nr[i];
-      }
-      v[i] = e;
-      tot += e;
+      v[i] = x < lim ? sP[x + 1] : 0;
+      tot += v[i];
     }
-    if (bad) set_err(st, ERR_UNSUPPORTED, SITE_BLOCK_ENTRIES);
     const u64 incl = wave_incl_sum(tot);
     if ((tid & 63) == 63) s_wsum[wave] = incl;
-    __syncthreads();
+    __syncthreads();  // also: every thread has read its sizes before they are overwritten
     u64 run = incl - tot;
     for (u32 w = 0; w < wave; w++) run += s_wsum[w];
     if (tid == 0) sP[0] = 0;
@@ -331,7 +339,9 @@ __global__ __launch_bounds__(256) void k_bnext(Kept k, const u32* m_dev, EncPara
       b.bbytes[q] = (u32)S;
       smax = max(smax, (u32)(j - q));
     }
-    __syncthreads();
+    // the LDS is reused by the next tile (a barrier waits for this tile's
+    // stores too, so it is skipped after the last one)
+    if (q0 + (u64)gridDim.x * kBnStarts < m) __syncthreads();
   }
   smax = wave_max(smax);
   if ((threadIdx.x & 63) == 0 && smax) atomicMax(&st->blk_step_max, smax);
@@ -1133,16 +1143,30 @@ __device__ __forceinline__ u32 slice4(const u32 (*sl)[256], u32 x) {
   return sl[3][x & 0xff] ^ sl[2][(x >> 8) & 0xff] ^ sl[1][(x >> 16) & 0xff] ^ sl[0][x >> 24];
 }
 
+// One CRC-32C slicing-by-8 step: the raw register after 8 bytes, given
+// x = register ^ bytes 0..3 and y = bytes 4..7 (little-endian words).
+__device__ __forceinline__ u32 slice8(const u32 (*sl)[256], u32 x, u32 y) {
+  return sl[7][x & 0xff] ^ sl[6][(x >> 8) & 0xff] ^ sl[5][(x >> 16) & 0xff] ^ sl[4][x >> 24] ^ sl[3][y & 0xff] ^
+         sl[2][(y >> 8) & 0xff] ^ sl[1][(y >> 16) & 0xff] ^ sl[0][y >> 24];
+}
+// The raw register of 16 bytes from a zero register.
+__device__ __forceinline__ u32 crc16(const u32 (*sl)[256], const u32 (&w)[4]) {
+  return slice8(sl, slice8(sl, w[0], w[1]) ^ w[2], w[3]);
+}
+
 // The message of L bytes at stage byte mis (mis < 16; the stage is 16-byte
 // aligned and dst is congruent to it mod 16) copied to dst, followed by its
 // masked crc32c::Value.  The message is cut into the stage's 16-byte pieces:
 // the F whole ones below the message end are right-aligned into rows of 64
-// (1024 bytes), each lane folding one piece per row into its register and
-// writing it to dst with one 16-byte store; the lanes are then joined, and
-// the last e < 16 bytes appended.  The ~0 initial value is folded into the
-// message's first four bytes and the stage bytes before mis are zeroed.
-// zt[j] are copies of g_crc.zeros[j + 2] (the joins of 2^j pieces).  Needs
-// F >= 2.
+// (1024 bytes), each lane folding one piece per row (every row's piece on its
+// own, then the rows joined) and writing it to dst with one 16-byte store;
+// the lanes are then joined (within rows of 16 lanes by DPP, the four rows by
+// their first lanes), and the last e < 16 bytes appended.  The ~0 initial
+// value is folded into the message's first four bytes and the stage bytes
+// before mis are zeroed.  sl holds the slicing-by-8 tables, z1024 and zt[j]
+// copies of g_crc.zeros[8] and g_crc.zeros[j + 2].  Needs F >= 2 and
+// mis + L + 16 <= 1024 * kCrcRows.
+constexpr u32 kCrcRows = (kStage + 32 + 1023) / 1024;
 __device__ __forceinline__ void crc_copy_out(const uint4* stage4, u32 mis, u32 L, u8* dst, const u32 (*sl)[256],
                                              const u32 (*z1024)[256], const u32 (*zt)[4][256]) {
   const u32 lane = threadIdx.x & 63;
@@ -1151,37 +1175,32 @@ __device__ __forceinline__ void crc_copy_out(const uint4* stage4, u32 mis, u32 L
   GMEM u8* gd = (GMEM u8*)dst;
   GMEM u32x4* d4 = (GMEM u32x4*)(dst - mis);
   const u32 rows = (F + 63) >> 6, slot0 = 64 * rows - F;
-  u32 s = 0;
-  for (u32 r = 0; r < rows; r++) {
-    const int f = (int)(64 * r + lane) - (int)slot0;
-    uint4 v = {0, 0, 0, 0};
-    if (f >= 0) {
-      v = stage4[f];
-      if (f > 0 || mis == 0) d4[f] = u32x4{v.x, v.y, v.z, v.w};
-    }
-    u32 w[4] = {v.x, v.y, v.z, v.w};
-    if (r < 2) {  // the first two pieces are in rows 0 and 1
+  u32 c[kCrcRows];
 #pragma unroll
-      for (u32 t = 0; t < 4; t++) {
-        const int lo = 4 * (int)t + (f == 1 ? 16 : 0);
-        if (f == 0) w[t] &= byte_span_mask((int)mis - lo, 4);
-        if (f == 0 || f == 1) w[t] ^= byte_span_mask((int)mis - lo, (int)mis + 4 - lo);
+  for (u32 r = 0; r < kCrcRows; r++) {
+    c[r] = 0;
+    if (r < rows) {
+      const int f = (int)(64 * r + lane) - (int)slot0;
+      uint4 v = {0, 0, 0, 0};
+      if (f >= 0) {
+        v = stage4[f];
+        if (f > 0 || mis == 0) d4[f] = u32x4{v.x, v.y, v.z, v.w};
       }
+      u32 w[4] = {v.x, v.y, v.z, v.w};
+      if (r < 2) {  // the first two pieces are in rows 0 and 1
+#pragma unroll
+        for (u32 t = 0; t < 4; t++) {
+          const int lo = 4 * (int)t + (f == 1 ? 16 : 0);
+          if (f == 0) w[t] &= byte_span_mask((int)mis - lo, 4);
+          if (f == 0 || f == 1) w[t] ^= byte_span_mask((int)mis - lo, (int)mis + 4 - lo);
+        }
+      }
+      c[r] = crc16(sl, w);
     }
-    u32 c = 0;
-#pragma unroll
-    for (u32 t = 0; t < 4; t++) c = slice4(sl, c ^ w[t]);
-    s = crc_shift(z1024, s) ^ c;
   }
-  // lane l's pieces end 16 (63 - l) bytes before the last whole piece does
-#pragma unroll
-  for (int j = 0; j < 6; j++) {
-    const u32 other = __shfl_down(s, 1u << j, 64);
-    if ((lane & ((2u << j) - 1)) == 0) s = crc_shift(zt[j], s) ^ other;
-  }
-  s = (u32)__builtin_amdgcn_readlane((int)s, 0);
+  // the last e bytes, right-aligned in a zero-filled piece (all lanes alike)
+  u32 clast = 0;
   if (e) {
-    // the last e bytes, right-aligned in a zero-filled piece, then folded
     const uint4 t = stage4[F];
     u32 w[4] = {t.x, t.y, t.z, t.w};
 #pragma unroll
@@ -1195,12 +1214,32 @@ __device__ __forceinline__ void crc_copy_out(const uint4* stage4, u32 mis, u32 L
       const u32 lo = u >= 1 ? sel4(w, (u32)(u - 1)) : 0u;
       o[i] = bsh ? __builtin_amdgcn_alignbyte(hi, lo, 4 - bsh) : hi;
     }
-    u32 c = 0;
+    clast = crc16(sl, o);
+  }
+  u32 s = c[0];
 #pragma unroll
-    for (u32 i = 0; i < 4; i++) c = slice4(sl, c ^ o[i]);
+  for (u32 r = 1; r < kCrcRows; r++)
+    if (r < rows) s = crc_shift(z1024, s) ^ c[r];
+  // lane l's pieces end 16 (63 - l) bytes before the last whole piece does:
+  // join pairs 1, 2, 4, 8 lanes apart within rows of 16 (DPP row_shl), ...
+  {
+    u32 o = dpp_u32<0x101, 0xf>(s);
+    if ((lane & 1) == 0) s = crc_shift(zt[0], s) ^ o;
+    o = dpp_u32<0x102, 0xf>(s);
+    if ((lane & 3) == 0) s = crc_shift(zt[1], s) ^ o;
+    o = dpp_u32<0x104, 0xf>(s);
+    if ((lane & 7) == 0) s = crc_shift(zt[2], s) ^ o;
+    o = dpp_u32<0x108, 0xf>(s);
+    if ((lane & 15) == 0) s = crc_shift(zt[3], s) ^ o;
+  }
+  // ... then the four rows from their first lanes
+  const u32 s0 = (u32)__builtin_amdgcn_readlane((int)s, 0), s1 = (u32)__builtin_amdgcn_readlane((int)s, 16);
+  const u32 s2 = (u32)__builtin_amdgcn_readlane((int)s, 32), s3 = (u32)__builtin_amdgcn_readlane((int)s, 48);
+  s = crc_shift(zt[5], crc_shift(zt[4], s0) ^ s1) ^ (crc_shift(zt[4], s2) ^ s3);
+  if (e) {
     for (u32 i = 0; i < (e >> 2); i++) s = slice4(sl, s);
     for (u32 i = 0; i < (e & 3); i++) s = (s >> 8) ^ sl[0][s & 0xff];
-    s ^= c;
+    s ^= clast;
   }
   const u32 crc = crc_mask(s ^ 0xffffffffu);
   if (mis && lane < 16 - mis) gd[lane] = stage[mis + lane];
@@ -1260,9 +1299,9 @@ template <u32 NWMAX>
 __global__ __launch_bounds__(kWbThreads) void k_write_blocks(Kept k, const u32* nbo_dev, EncParams ep, EncBufs b,
                                                               u8* out, u64 out_cap, DevStatus* st) {
   constexpr bool kFast = NWMAX > 0;
-  constexpr bool kPre = kFast && NWMAX <= 5;  // entries loaded a block ahead (registers allow keys <= 32 bytes)
+  constexpr bool kPre = kFast && NWMAX <= 3;  // entries loaded a block ahead (registers allow keys <= 16 bytes)
   constexpr u32 NWP = kFast ? NWMAX : 1;
-  __shared__ u32 sl[4][256];
+  __shared__ u32 sl[8][256];         // slicing tables (the fast path uses all eight, the others four)
   __shared__ u32 zr[4][256];         // 1024 zero bytes (the fast path) or 256 (the others)
   __shared__ u32 zt[6][4][256];      // 16 << j zero bytes: the joins of crc_copy_out
   __shared__ uint4 stage_all[kWbWaves][(kStage + 32) / 16];
@@ -1272,10 +1311,8 @@ __global__ __launch_bounds__(kWbThreads) void k_write_blocks(Kept k, const u32* 
   };
   __shared__ WaveTab wtab[kWbWaves];
   if (st->out_bytes > out_cap || st->err) return;
-  for (u32 i = threadIdx.x; i < 4 * 256; i += blockDim.x) {
-    sl[i >> 8][i & 255] = g_crc.slice[i >> 8][i & 255];
-    zr[i >> 8][i & 255] = g_crc.zeros[kFast ? 8 : 6][i >> 8][i & 255];
-  }
+  for (u32 i = threadIdx.x; i < 8 * 256; i += blockDim.x) sl[i >> 8][i & 255] = g_crc.slice[i >> 8][i & 255];
+  for (u32 i = threadIdx.x; i < 4 * 256; i += blockDim.x) zr[i >> 8][i & 255] = g_crc.zeros[kFast ? 8 : 6][i >> 8][i & 255];
   if (kFast)
     for (u32 i = threadIdx.x; i < 6 * 4 * 256; i += blockDim.x)
       zt[i >> 10][(i >> 8) & 3][i & 255] = g_crc.zeros[2 + (i >> 10)][(i >> 8) & 3][i & 255];

@@ -176,7 +176,14 @@ ENGINE_SIGNATURES = {
     "luda_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     "luda_memcpy_d2h": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
     "luda_version": (C.c_char_p, []),
+    "luda_set_slots": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "luda_submit": (C.c_int, [C.c_void_p, C.POINTER(luda_job), C.c_void_p, C.c_void_p]),
+    "luda_poll": (C.c_int, [C.c_void_p]),
+    "luda_drain": (C.c_int, [C.c_void_p]),
 }
+
+# void (*luda_done_fn)(void* user, int status, luda_result* result, const char* err)
+DONE_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, RESULT_P, C.c_char_p)
 
 
 def _bind(lib, table):

@@ -202,6 +202,49 @@ class Engine:
     def free(self, res):
         self.lib.luda_free_result(self.ctx, res)
 
+    # -- batched / served form (luda_submit, luda_poll, luda_drain) --------
+    def set_slots(self, slots):
+        rc = self.lib.luda_set_slots(self.ctx, slots)
+        if rc != ffi.LUDA_OK:
+            raise RuntimeError(f"luda_set_slots failed ({rc})")
+
+    def submit(self, job, on_done, keep_on_device=False):
+        """Queues `job`; on_done(status, (tables, stats, result_ptr) | None,
+        message) runs from poll()/drain().  The job must stay alive until then."""
+        if keep_on_device:
+            job.flags |= ffi.LUDA_JOB_KEEP_ON_DEVICE
+        if not hasattr(self, "_pending"):
+            self._pending = {}
+            self._next_id = 1
+
+            def trampoline(user, status, res, err):
+                cb, jb, kod = self._pending.pop(user)
+                if status != ffi.LUDA_OK:
+                    cb(status, None, (err or b"").decode())
+                    return
+                try:
+                    stats = res.contents.stats
+                    tables = [] if kod else result_tables(res)
+                    cb(status, (tables, _stats_dict(stats), res), "")
+                finally:
+                    if not kod:
+                        self.lib.luda_free_result(self.ctx, res)
+            self._done_fn = ffi.DONE_FN(trampoline)
+        uid = self._next_id
+        self._next_id += 1
+        self._pending[uid] = (on_done, job, keep_on_device)
+        rc = self.lib.luda_submit(self.ctx, C.byref(job), C.cast(self._done_fn, C.c_void_p), C.c_void_p(uid))
+        if rc != ffi.LUDA_OK:
+            self._pending.pop(uid)
+            raise RuntimeError(f"luda_submit failed ({rc})")
+        return uid
+
+    def poll(self):
+        return self.lib.luda_poll(self.ctx)
+
+    def drain(self):
+        return self.lib.luda_drain(self.ctx)
+
     def builder(self):
         """A table builder for build_group_tables backed by the engine."""
         def build(opts_p, n, keys, key_off, vals, val_off):

@@ -134,6 +134,22 @@ def test_tiny_jobs_match_oracle(eng, block):
         assert summary(out[0]) == summary(o1), seed
 
 
+def test_fresh_context_first_job_wider_keys_than_guessed():
+    """Each job on a context of its own: the key planes are first sized from
+    the keys at restart points only, so a longer key further in overflows
+    them and the job must be retried wider, not rejected (kernels after the
+    overflow see truncated keys and may report spurious faults)."""
+    for seed in range(500, 524):
+        tj = TinyJob(seed)
+        job, keep = tj.make_job(lambda e, o: [t.data for t in oracle.build_tables(e, o, "oracle")])
+        rc1, want = oracle.compact(job, "oracle")
+        e = jobs.Engine(0)
+        rc2, out, msg = e.compact(job)
+        e.close()
+        assert rc1 == 0 and rc2 == 0, (seed, msg)
+        assert summary(out[0]) == summary(want), seed
+
+
 with open(os.path.join(GOLD, "jobs.json")) as f:
     GOLD_JOBS = json.load(f)
 
