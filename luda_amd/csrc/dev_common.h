@@ -40,6 +40,7 @@ enum ErrSite : u32 {
   SITE_COMPRESSED = 10,
   SITE_BLOCK_ENTRIES = 11,
   SITE_FILE_BLOCKS = 12,
+  SITE_SCAN = 13,
 };
 
 // Device-resident counters and flags of one job.  The host reads this struct

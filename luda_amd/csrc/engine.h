@@ -152,26 +152,22 @@ struct EncBufs {
   u32* bnext;    // first entry after the block that would start here
   u32* bbytes;   // raw size of that block
   u32* bs;       // 1 where a block starts
-  u32* bid;      // exclusive scan of bs: block id
   // per output block (nbo + 1 entries where a sentinel is needed)
   u32* ob_first;
   u32* ob_n;
   u32* ob_bytes;
   u32* ob_forced;  // 1 where ShouldStopBefore starts a table at this block
-  u32* ob_tot;     // ob_bytes + 5, the scan input of O
-  u64* O;          // exclusive scan of ob_tot
+  u64* O;          // exclusive scan of ob_bytes + 5: block offset in the output's data
   u32* ff;         // exclusive scan of ob_forced
   u32* flist;      // forced blocks in order
   u32* fnext;      // first block after the table that would start here
   u32* fs;         // 1 where a table starts
-  u32* fid;        // exclusive scan of fs: table id
   u32* ob_file;
   u64* ob_off;     // offset inside its table
   u64* ob_isz;     // index entry size, then (scanned) its offset in the index block
   // per output table (nf + 1 entries)
   u32* f_first;    // first block; f_first[nf] = nbo
-  u32* f_nwin;     // filter windows, then (scanned) f_win
-  u32* f_win;
+  u32* f_win;      // first filter window (exclusive scan of window counts)
   u64* f_dend;     // data end offset
   u64* f_fbytes;   // filter data bytes
   u64* f_ibytes;   // index block size
@@ -189,16 +185,40 @@ struct EncBufs {
 // ---------------------------------------------------------------------------
 // launchers
 
-struct ScanTmp {
-  void* tiles = nullptr;  // device, >= scan_tmp_bytes(n_cap)
+// A context's single-pass scans: one look-back word per tile of the largest
+// scan (zeroed when allocated, never shared with other data) and the epoch of
+// the last launch, which tags the words it writes.
+struct ScanState {
+  u64* words = nullptr;
+  u64 tiles = 0;
+  u32 epoch = 0;  // 1 .. 2^14 - 1; the words are zeroed again when it wraps
 };
-size_t scan_tmp_bytes(u64 n_cap);
+struct ScanArgs {
+  u64* words;
+  u32 epoch;
+  u32 cap;    // with need: a total above cap is stored to *need and replaced by 0
+  u32* need;
+  DevStatus* st;
+};
+struct ScanGuard {
+  u32 cap;
+  u32* need;
+};
+u64 scan_tiles_for(u64 n_cap);
+// The words for scans of up to n_cap items (the stream is idle when it grows).
+hipError_t scan_reserve(ScanState& S, u64 n_cap, hipStream_t s);
+void scan_release(ScanState& S);
+u32 scan_next_epoch(ScanState& S, hipStream_t s);
+[[noreturn]] void throw_scan_capacity(u64 need, u64 have);
 // Exclusive scan of n = (n_dev ? *n_dev : 0) + n_add items (n <= n_cap);
-// total to *total (may be null).  In place is allowed.
-void scan_u32(KTimer& T, const u32* in, u32* out, const u32* n_dev, u32 n_add, u32 n_cap, u32* total, void* tmp);
-void scan_u64(KTimer& T, const u64* in, u64* out, const u32* n_dev, u32 n_add, u32 n_cap, u64* total, void* tmp);
-void scan_u32_to_u64(KTimer& T, const u32* in, u64* out, const u32* n_dev, u32 n_add, u32 n_cap, u64* total,
-                     void* tmp);
+// total to *total (may be null).  In place is allowed.  With a guard, a
+// total above g->cap is reported (ERR_OVERFLOW, *g->need) and stored as 0.
+void scan_u32(KTimer& T, ScanState& S, const u32* in, u32* out, const u32* n_dev, u32 n_add, u32 n_cap, u32* total,
+              DevStatus* st, const ScanGuard* g = nullptr);
+void scan_u64(KTimer& T, ScanState& S, const u64* in, u64* out, const u32* n_dev, u32 n_add, u32 n_cap, u64* total,
+              DevStatus* st);
+void scan_u32_to_u64(KTimer& T, ScanState& S, const u32* in, u64* out, const u32* n_dev, u32 n_add, u32 n_cap,
+                     u64* total, DevStatus* st);
 void fill_u32(KTimer& T, u32* p, u32 v, u64 n);
 void fill_u8(KTimer& T, u8* p, u8 v, u64 n);
 void iota_u32(KTimer& T, u32* p, u64 n);
@@ -207,8 +227,10 @@ void iota_u32(KTimer& T, u32* p, u64 n);
 void launch_files(KTimer& T, const FileDesc* files, u32 nfiles, FileInfo* info, u32* nblk, DevStatus* st);
 void launch_index(KTimer& T, const FileDesc* files, const FileInfo* info, const u32* blk_base, u32 nfiles,
                   u32 nb_cap, BlockDesc* blks, u32* blk_nent, DevStatus* st);
-void launch_file_ebase(KTimer& T, const u32* blk_base, const u32* blk_ebase, u32 nfiles, const u32* n_entries,
-                       u32* file_ebase);
+// blk_ebase = exclusive scan of blk_nent over st->n_blocks_in blocks,
+// st->n_entries, and file_ebase[f] = the first entry of table f (f <= nfiles)
+void scan_entries(KTimer& T, ScanState& S, const BlockDesc* blks, const u32* blk_nent, u32* blk_ebase, u32 nb_cap,
+                  u32 nfiles, u32* file_ebase, DevStatus* st);
 void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 nb, const u32* blk_ebase, Planes K,
                    u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr, u32 max_ent);
 void launch_ingest(KTimer& T, u64 n, const u8* keys, const u64* key_off, const u8* vals, const u64* val_off, Kept out,
@@ -217,11 +239,17 @@ void launch_ingest(KTimer& T, u64 n, const u8* keys, const u64* key_off, const u
 // k_merge.hip
 void launch_merge(KTimer& T, const MergePair* pairs, u32 npairs, u32 total_tiles, const u32* in, u32* out, Planes K);
 u32 merge_items_per_tile(u32 W);  // outputs per merge tile for key planes of W words
-void launch_check_sorted(KTimer& T, Planes K, u64 n, const u32* child_start, u32 nchild, DevStatus* st);
+// also writes iota[e] = e, the merge's starting permutation
+void launch_check_sorted(KTimer& T, Planes K, u64 n, const u32* child_start, u32 nchild, u32* iota, DevStatus* st);
 
 // k_select.hip
-void launch_drop(KTimer& T, const u32* perm, u32 n, Planes K, const DropParams& dp, u32* keep, u32* gcnt,
+// events[0]: the grandparent-crossing count that the keep scan appends to
+void launch_drop(KTimer& T, const u32* perm, u32 n, Planes K, const DropParams& dp, u32* keep, u32* gcnt, u32* events,
                  DevStatus* st);
+// kpos = exclusive scan of keep, n_kept; with grandparents also the crossings
+// (events[1..]) that launch_stops walks
+void scan_keep(KTimer& T, ScanState& S, const u32* keep, u32* kpos, u32 n, const DropParams& dp, const u32* gcnt,
+               u32* events, u32 ev_cap, DevStatus* st);
 void launch_gather(KTimer& T, const u32* perm, u32 n, const u32* keep, const u32* kpos, Planes K, const u64* vptr,
                    const u32* vlen, Kept out);
 void launch_stops(KTimer& T, const u32* gcnt, const u32* kpos, const u32* keep, u32 n, const DropParams& dp,
@@ -246,20 +274,24 @@ void run_orbit(KTimer& T, const OrbitPlan& p, const u32* nxt, const u32* n_dev, 
 // k_encode.hip
 void launch_entry_sizes(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncBufs b, DevStatus* st);
 void launch_bnext(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncParams ep, EncBufs b, DevStatus* st);
-void launch_block_info(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncBufs b, DevStatus* st);
-void launch_block_tot(KTimer& T, const u32* nbo_dev, u32 nbo_cap, EncBufs b, u32* tot);
-void launch_block_forced(KTimer& T, const u32* nbo_dev, u32 nbo_cap, EncBufs b);
+// scans with the per-block and per-table steps fused in (n_blocks_out with
+// the block table, block offsets O, the forced-block list, n_files_out with
+// f_first, the window offsets f_win, the table offsets f_off)
+void scan_blocks(KTimer& T, ScanState& S, Kept k, const u32* m_dev, u32 m_cap, EncBufs b, u32 nbo_cap,
+                 DevStatus* st);
+void scan_block_offsets(KTimer& T, ScanState& S, const u32* nbo_dev, u32 nbo_cap, EncBufs b, DevStatus* st);
+void scan_forced(KTimer& T, ScanState& S, const u32* nbo_dev, u32 nbo_cap, EncBufs b, DevStatus* st);
+void scan_files(KTimer& T, ScanState& S, const u32* nbo_dev, u32 nbo_cap, EncBufs b, u32 nf_cap, DevStatus* st);
+void scan_windows(KTimer& T, ScanState& S, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b, u32 nw_cap,
+                  DevStatus* st);
+void scan_file_offsets(KTimer& T, ScanState& S, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b,
+                       DevStatus* st);
 void launch_fnext(KTimer& T, const u32* nbo_dev, u32 nbo_cap, const u32* nforced_dev, EncParams ep, EncBufs b,
                   DevStatus* st);
-void launch_file_first(KTimer& T, const u32* nbo_dev, u32 nbo_cap, EncBufs b, const u32* nf_dev);
 void launch_block_layout(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, const u32* nf_dev, EncParams ep,
                          EncBufs b);
-void launch_file_layout(KTimer& T, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b);
-void launch_window_init(KTimer& T, const u32* nw_dev, u32 nw_cap, EncBufs b);
-void launch_window_counts(KTimer& T, const u32* nbo_dev, u32 nbo_cap, const u32* nf_dev, const u32* nw_dev,
-                          EncBufs b);
-void launch_window_sizes(KTimer& T, const u32* nw_dev, u32 nw_cap, EncParams ep, EncBufs b);
-void launch_file_sizes(KTimer& T, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b);
+void launch_windows(KTimer& T, const u32* nbo_dev, u32 nbo_cap, const u32* nf_dev, const u32* nw_dev, EncParams ep,
+                    EncBufs b);
 void launch_write_blocks(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u8* out,
                          u64 out_cap, DevStatus* st);
 void launch_write_filters(KTimer& T, Kept k, const u32* nw_dev, u32 nw_cap, const u32* nf_dev, EncParams ep, EncBufs b,

@@ -111,6 +111,7 @@ const char* site_name(u32 s) {
     case SITE_COMPRESSED: return "compressed block";
     case SITE_BLOCK_ENTRIES: return "block entries";
     case SITE_FILE_BLOCKS: return "table blocks";
+    case SITE_SCAN: return "prefix sum";
     default: return "unknown";
   }
 }
@@ -174,6 +175,7 @@ struct luda_ctx {
   DevBuf in_buf;     // uploaded host inputs
   DevBuf a_buf;      // phase A working memory
   DevBuf b_buf;      // phase B working memory
+  ScanState scans;   // look-back words of the single-pass scans
   DevStatus* st = nullptr;
   PinnedBuf pin;     // staging for small host<->device copies
   PinnedBuf pin2;
@@ -228,6 +230,11 @@ void launch_refused(const char* name, hipError_t e) {
   throw Fail{LUDA_IO_ERROR, std::string("HIP refused the launch of ") + name + ": " + hipGetErrorString(e)};
 }
 
+void throw_scan_capacity(u64 need, u64 have) {
+  throw Fail{LUDA_IO_ERROR, "prefix sum of " + std::to_string(need) + " tiles, words reserved for " +
+                                std::to_string(have)};
+}
+
 namespace {
 
 int status_code_of(const DevStatus& st) {
@@ -257,7 +264,6 @@ struct EncodeArrays {
   u32* ent_b;
   u16* ftab_f;
   u32* ent_f;
-  void* scan_tmp;
 };
 
 // Carves the encoder's arrays for a setup out of an arena (dry or real).
@@ -282,23 +288,19 @@ EncodeArrays carve_encode(Arena& a, const EncodeSetup& s, u32 W, bool with_kept)
   b.bnext = a.take<u32>(m);
   b.bbytes = a.take<u32>(m);
   b.bs = a.take<u32>(m);
-  b.bid = a.take<u32>(m);
   b.ob_first = a.take<u32>(nb + 1);
   b.ob_n = a.take<u32>(nb + 1);
   b.ob_bytes = a.take<u32>(nb + 1);
   b.ob_forced = a.take<u32>(nb + 1);
-  b.ob_tot = a.take<u32>(nb + 1);
   b.O = a.take<u64>(nb + 1);
   b.ff = a.take<u32>(nb + 1);
   b.flist = a.take<u32>(nb + 1);
   b.fnext = a.take<u32>(nb + 1);
   b.fs = a.take<u32>(nb + 1);
-  b.fid = a.take<u32>(nb + 1);
   b.ob_file = a.take<u32>(nb + 1);
   b.ob_off = a.take<u64>(nb + 1);
   b.ob_isz = a.take<u64>(nb + 1);
   b.f_first = a.take<u32>(nf + 1);
-  b.f_nwin = a.take<u32>(nf + 1);
   b.f_win = a.take<u32>(nf + 1);
   b.f_dend = a.take<u64>(nf + 1);
   b.f_fbytes = a.take<u64>(nf + 1);
@@ -317,18 +319,7 @@ EncodeArrays carve_encode(Arena& a, const EncodeSetup& s, u32 W, bool with_kept)
     r.ftab_f = a.take<u16>(s.of.table_entries);
     r.ent_f = a.take<u32>(s.of.entry_entries);
   }
-  const u64 maxn = std::max<u64>(std::max<u64>(m, nb), std::max<u64>(nf, nw)) + 2;
-  r.scan_tmp = a.take<u8>(scan_tmp_bytes(maxn));
   return r;
-}
-
-__global__ void k_guard(u32* count, u32 cap, DevStatus* st, u32* need) {
-  if (*count > cap) {
-    *need = *count;
-    *count = 0;
-    atomicOr(&st->err, ERR_OVERFLOW);
-    atomicCAS(&st->site, 0u, (u32)SITE_CAPACITY);
-  }
 }
 
 __global__ void k_table_starts_forced(const u32* nbo_dev, EncBufs b) {
@@ -351,38 +342,30 @@ void run_encode(luda_ctx* c, const EncodeSetup& s, const EncodeArrays& A, u8* ou
   const u32* nbo_dev = &st->n_blocks_out;
   const u32* nf_dev = &st->n_files_out;
   const u32* nw_dev = &st->n_windows;
+  ScanState& SC = c->scans;
+  HIPCHK(scan_reserve(SC, std::max<u64>(std::max<u64>(s.m_cap, s.nbo_cap), std::max<u64>(s.nf_cap, s.nw_cap)) + 1,
+                      c->s));
   launch_entry_sizes(T, k, m_dev, s.m_cap, b, st);
   launch_bnext(T, k, m_dev, s.m_cap, s.ep, b, st);
   run_orbit(T, s.ob, b.bnext, m_dev, &st->blk_step_max, b.bs, A.ftab_b, A.ent_b, st);
-  scan_u32(T, b.bs, b.bid, m_dev, 0, s.m_cap, &st->n_blocks_out, A.scan_tmp);
-  KLAUNCH(T, "guard", k_guard, 1, 1, 0, &st->n_blocks_out, s.nbo_cap, st, &st->need_blocks);
-  launch_block_info(T, k, m_dev, s.m_cap, b, st);
-  launch_block_tot(T, nbo_dev, s.nbo_cap, b, b.ob_tot);
-  scan_u32_to_u64(T, b.ob_tot, b.O, nbo_dev, 1, s.nbo_cap + 1, &st->data_bytes, A.scan_tmp);
+  scan_blocks(T, SC, k, m_dev, s.m_cap, b, s.nbo_cap, st);
+  scan_block_offsets(T, SC, nbo_dev, s.nbo_cap, b, st);
   if (s.size_cuts) {
-    scan_u32(T, b.ob_forced, b.ff, nbo_dev, 0, s.nbo_cap, &st->n_forced, A.scan_tmp);
-    launch_block_forced(T, nbo_dev, s.nbo_cap, b);
+    scan_forced(T, SC, nbo_dev, s.nbo_cap, b, st);
     launch_fnext(T, nbo_dev, s.nbo_cap, &st->n_forced, s.ep, b, st);
     run_orbit(T, s.of, b.fnext, nbo_dev, &st->file_step_max, b.fs, A.ftab_f, A.ent_f, st);
   } else {
     KLAUNCH(T, "table_starts", k_table_starts_forced, grid_for(s.nbo_cap, 256), 256, 0, nbo_dev, b);
   }
-  scan_u32(T, b.fs, b.fid, nbo_dev, 0, s.nbo_cap, &st->n_files_out, A.scan_tmp);
-  KLAUNCH(T, "guard", k_guard, 1, 1, 0, &st->n_files_out, s.nf_cap, st, &st->need_files);
-  launch_file_first(T, nbo_dev, s.nbo_cap, b, nf_dev);
+  scan_files(T, SC, nbo_dev, s.nbo_cap, b, s.nf_cap, st);
   launch_block_layout(T, k, nbo_dev, s.nbo_cap, nf_dev, s.ep, b);
-  scan_u64(T, b.ob_isz, b.ob_isz, nbo_dev, 1, s.nbo_cap + 1, nullptr, A.scan_tmp);
-  launch_file_layout(T, nf_dev, s.nf_cap, s.ep, b);
-  scan_u32(T, b.f_nwin, b.f_win, nf_dev, 1, s.nf_cap + 1, &st->n_windows, A.scan_tmp);
+  scan_u64(T, SC, b.ob_isz, b.ob_isz, nbo_dev, 1, s.nbo_cap + 1, nullptr, st);
+  scan_windows(T, SC, nf_dev, s.nf_cap, s.ep, b, s.nw_cap, st);
   if (s.ep.bloom_bits) {
-    KLAUNCH(T, "guard", k_guard, 1, 1, 0, &st->n_windows, s.nw_cap, st, &st->need_windows);
-    launch_window_init(T, nw_dev, s.nw_cap, b);
-    launch_window_counts(T, nbo_dev, s.nbo_cap, nf_dev, nw_dev, b);
-    launch_window_sizes(T, nw_dev, s.nw_cap, s.ep, b);
-    scan_u64(T, b.w_fsz, b.w_fsz, nw_dev, 1, s.nw_cap + 1, nullptr, A.scan_tmp);
+    launch_windows(T, nbo_dev, s.nbo_cap, nf_dev, nw_dev, s.ep, b);
+    scan_u64(T, SC, b.w_fsz, b.w_fsz, nw_dev, 1, s.nw_cap + 1, nullptr, st);
   }
-  launch_file_sizes(T, nf_dev, s.nf_cap, s.ep, b);
-  scan_u64(T, b.f_off, b.f_off, nf_dev, 1, s.nf_cap + 1, &st->out_bytes, A.scan_tmp);
+  scan_file_offsets(T, SC, nf_dev, s.nf_cap, s.ep, b, st);
   launch_write_blocks(T, k, nbo_dev, s.nbo_cap, s.ep, b, out, s.out_cap, st);
   if (s.ep.bloom_bits) launch_write_filters(T, k, nw_dev, s.nw_cap, nf_dev, s.ep, b, out, s.out_cap, st);
   launch_write_tail(T, k, nf_dev, s.nf_cap, s.ep, b, out, s.out_cap, st);
@@ -664,7 +647,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
   for (auto* f : files) nb_cap += f->size / 41 + 1;
   if (nb_cap > 0xffffffffull / 2) throw Fail{LUDA_NOT_SUPPORTED, "inputs too large"};
   auto carve_a = [&](Arena& a, FileDesc*& d_files, FileInfo*& d_info, u32*& d_nblk, u32*& d_bbase, BlockDesc*& d_blks,
-                     u32*& d_nent, u32*& d_ebase, u32*& d_febase, void*& tmp) {
+                     u32*& d_nent, u32*& d_ebase, u32*& d_febase) {
     d_files = a.take<FileDesc>(nfiles);
     d_info = a.take<FileInfo>(nfiles);
     d_nblk = a.take<u32>(nfiles + 1);
@@ -673,34 +656,31 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     d_nent = a.take<u32>(nb_cap + 1);
     d_ebase = a.take<u32>(nb_cap + 1);
     d_febase = a.take<u32>(nfiles + 1);
-    tmp = a.take<u8>(scan_tmp_bytes(std::max<u64>(nb_cap, nfiles) + 2));
   };
   FileDesc* d_files;
   FileInfo* d_info;
   u32 *d_nblk, *d_bbase, *d_nent, *d_ebase, *d_febase;
   BlockDesc* d_blks;
-  void* tmpA;
   {
     Arena dry;
     dry.dry = true;
-    carve_a(dry, d_files, d_info, d_nblk, d_bbase, d_blks, d_nent, d_ebase, d_febase, tmpA);
+    carve_a(dry, d_files, d_info, d_nblk, d_bbase, d_blks, d_nent, d_ebase, d_febase);
     HIPCHK(c->a_buf.reserve(dry.used));
     Arena a;
     a.base = c->a_buf.p;
     a.cap = c->a_buf.cap;
-    carve_a(a, d_files, d_info, d_nblk, d_bbase, d_blks, d_nent, d_ebase, d_febase, tmpA);
+    carve_a(a, d_files, d_info, d_nblk, d_bbase, d_blks, d_nent, d_ebase, d_febase);
   }
   DevStatus st0;
   memset(&st0, 0, sizeof(st0));
   fill_status(c, st0);
   upload(c, d_files, fdesc.data(), nfiles * sizeof(FileDesc));
   KTimer& T = c->T;
-  fill_u32(T, d_nblk, 0, nfiles + 1);
   launch_files(T, d_files, nfiles, d_info, d_nblk, c->st);
-  scan_u32(T, d_nblk, d_bbase, nullptr, nfiles + 1, nfiles + 1, nullptr, tmpA);
+  HIPCHK(scan_reserve(c->scans, std::max<u64>(nb_cap, nfiles + 1), c->s));
+  scan_u32(T, c->scans, d_nblk, d_bbase, nullptr, nfiles + 1, nfiles + 1, nullptr, c->st);
   launch_index(T, d_files, d_info, d_bbase, nfiles, (u32)nb_cap, d_blks, d_nent, c->st);  // sets n_blocks_in
-  scan_u32(T, d_nent, d_ebase, &c->st->n_blocks_in, 0, (u32)nb_cap, &c->st->n_entries, tmpA);
-  launch_file_ebase(T, d_bbase, d_ebase, nfiles, &c->st->n_entries, d_febase);
+  scan_entries(T, c->scans, d_blks, d_nent, d_ebase, (u32)nb_cap, nfiles, d_febase, c->st);
   DevStatus stA;
   std::vector<u32> febase(nfiles + 1);
   HIPCHK(hipGetLastError());
@@ -877,10 +857,12 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
       sb.max_ulen = 0;
       fill_status(c, sb);
       // decode, check, merge, drop, gather, stops
+      HIPCHK(scan_reserve(c->scans,
+                          std::max<u64>(std::max<u64>(n, es.m_cap), std::max<u64>(es.nbo_cap, std::max(es.nf_cap, es.nw_cap))) + 1,
+                          c->s));
       launch_decode(T, d_files, d_blks, nb, d_ebase, K, vptr, vlen, c->st, stA.max_blk, stA.max_nr, stA.max_ent);
-      launch_check_sorted(T, K, n, d_cstart, (u32)cstart.size() - 1, c->st);
+      launch_check_sorted(T, K, n, d_cstart, (u32)cstart.size() - 1, permA, c->st);
       const u32* perm = permA;
-      iota_u32(T, permA, n);
       {
         u32* in = permA;
         u32* outp = permB;
@@ -896,8 +878,8 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
         }
         perm = in;
       }
-      launch_drop(T, perm, n, K, dp, keep, gcnt, c->st);
-      scan_u32(T, keep, kpos, nullptr, n, n, &c->st->n_kept, EA.scan_tmp);
+      launch_drop(T, perm, n, K, dp, keep, gcnt, events, c->st);
+      scan_keep(T, c->scans, keep, kpos, n, dp, gcnt, events, ngp, c->st);
       launch_gather(T, perm, n, keep, kpos, K, vptr, vlen, EA.kept);
       if (ngp) launch_stops(T, gcnt, kpos, keep, n, dp, events, ngp, EA.kept, c->st);
       run_encode(c, es, EA, (u8*)out);
@@ -1327,6 +1309,7 @@ void ctx_close(luda_ctx* c) {
   c->in_buf.release();
   c->a_buf.release();
   c->b_buf.release();
+  scan_release(c->scans);
   c->pin.release();
   c->pin2.release();
   for (auto& kv : c->out_pool) hipFree(kv.second);

@@ -12,6 +12,7 @@
 #include <algorithm>
 
 #include "engine.h"
+#include "scan.h"
 
 namespace {
 
@@ -22,6 +23,7 @@ __device__ __forceinline__ u32 wave_incl_scan_u32(u32 v) { return wave_incl_sum(
 
 // One thread per input table: footer -> index block handle.
 __global__ void k_files(const FileDesc* files, u32 nfiles, FileInfo* info, u32* nblk, DevStatus* st) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) nblk[nfiles] = 0;  // the end of the block-count scan
   GRID_STRIDE(f, nfiles) {
     nblk[f] = 0;
     const u8* base = files[f].base;
@@ -275,14 +277,25 @@ __global__ __launch_bounds__(256) void k_index(const FileDesc* files, const File
   if (maxe) atomicMax(&st->max_ent, maxe);
 }
 
-__global__ void k_file_ebase(const u32* blk_base, const u32* blk_ebase, u32 nfiles, const u32* n_entries,
-                             u32* file_ebase) {
-  GRID_STRIDE(f, nfiles + 1) {
-    const u32 nb = blk_base[nfiles];
-    const u32 b = blk_base[f];
-    file_ebase[f] = b < nb ? blk_ebase[b] : *n_entries;
+// The entry scan's epilogue (over the blocks' entry counts): each table's
+// first entry.  A table's first block writes it, for the empty tables just
+// before it too; the last tile writes the empty tables at the end and the
+// entry count after the last table.
+struct FileEbaseEpi {
+  const BlockDesc* blks;
+  u32* file_ebase;
+  u32 nfiles;
+  struct Pre {
+    u32 f, fp;
+  };
+  __device__ Pre pre(u64 b, u32) const { return {blks[b].file, b ? blks[b - 1].file + 1 : 0u}; }
+  __device__ void item(u64, u32, u32 ebase, const Pre& p) const {
+    for (u32 g = p.fp; g <= p.f; g++) file_ebase[g] = ebase;
   }
-}
+  __device__ void done(u32 n_entries, u64 nb) const {
+    for (u32 g = nb ? blks[nb - 1].file + 1 : 0u; g <= nfiles; g++) file_ebase[g] = n_entries;
+  }
+};
 
 // Byte sources for decode_block: the block staged in LDS, or read in place
 // from global memory.  Each holds one kind of pointer only, so every read is
@@ -842,13 +855,13 @@ void launch_files(KTimer& T, const FileDesc* files, u32 nfiles, FileInfo* info, 
 }
 void launch_index(KTimer& T, const FileDesc* files, const FileInfo* info, const u32* blk_base, u32 nfiles,
                   u32 nb_cap, BlockDesc* blks, u32* blk_nent, DevStatus* st) {
-  KLAUNCH(T, "index", k_index, grid_for(nb_cap, 256), 256, 0, files, info, blk_base, nfiles, nb_cap, blks, blk_nent,
+  KLAUNCH(T, "index", k_index, std::min<u32>(grid_for(nb_cap, 256), 1024), 256, 0, files, info, blk_base, nfiles, nb_cap, blks, blk_nent,
           st);
 }
-void launch_file_ebase(KTimer& T, const u32* blk_base, const u32* blk_ebase, u32 nfiles, const u32* n_entries,
-                       u32* file_ebase) {
-  KLAUNCH(T, "file_ebase", k_file_ebase, grid_for(nfiles + 1, 64), 64, 0, blk_base, blk_ebase, nfiles, n_entries,
-          file_ebase);
+void scan_entries(KTimer& T, ScanState& S, const BlockDesc* blks, const u32* blk_nent, u32* blk_ebase, u32 nb_cap,
+                  u32 nfiles, u32* file_ebase, DevStatus* st) {
+  lbscan::run<u32, u32>(T, "scan_entries", S, lbscan::ArrLoad<u32>{blk_nent}, blk_ebase, &st->n_blocks_in, 0, nb_cap,
+                        &st->n_entries, st, nullptr, FileEbaseEpi{blks, file_ebase, nfiles});
 }
 void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 nb, const u32* blk_ebase, Planes K,
                    u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr, u32 max_ent) {

@@ -13,6 +13,7 @@
 #include <algorithm>
 
 #include "engine.h"
+#include "scan.h"
 
 namespace {
 
@@ -347,36 +348,48 @@ __global__ __launch_bounds__(256) void k_bnext(Kept k, const u32* m_dev, EncPara
   if ((threadIdx.x & 63) == 0 && smax) atomicMax(&st->blk_step_max, smax);
 }
 
-__global__ void k_block_info(Kept k, const u32* m_dev, EncBufs b, DevStatus* st) {
-  const u64 m = *m_dev;
-  const u32 nbo = st->n_blocks_out;  // 0 when the count overflowed its capacity (reported)
-  GRID_STRIDE(q, m) {
-    if (!b.bs[q]) continue;
-    const u32 id = b.bid[q];
-    if (id >= nbo) continue;
-    b.ob_first[id] = (u32)q;
-    b.ob_n[id] = b.bnext[q] - (u32)q;
-    b.ob_bytes[id] = b.bbytes[q];
-    b.ob_forced[id] = (q > 0 && k.forced[q]) ? 1u : 0u;
+// The block scan's epilogue (over the block-start flags bs): output block id
+// = the exclusive count of block starts before entry q.
+struct BlockInfoEpi {
+  const u32* forced;
+  EncBufs b;
+  u32 cap;  // output blocks the arrays hold
+  struct Pre {
+    u32 next, bytes, forced;
+  };
+  __device__ Pre pre(u64 q, u32 start) const {
+    if (!start) return {0, 0, 0};
+    return {b.bnext[q], b.bbytes[q], forced[q]};
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  __device__ void item(u64 q, u32 start, u32 id, const Pre& p) const {
+    if (!start || id >= cap) return;
+    b.ob_first[id] = (u32)q;
+    b.ob_n[id] = p.next - (u32)q;
+    b.ob_bytes[id] = p.bytes;
+    b.ob_forced[id] = (q > 0 && p.forced) ? 1u : 0u;
+  }
+  __device__ void done(u32 nbo, u64) const {
     b.ob_forced[nbo] = 0;
     b.ob_bytes[nbo] = 0;
   }
-}
+};
 
-// ob_bytes + 5 in place of a separate array for the offset scan.
-__global__ void k_block_tot(const u32* nbo_dev, EncBufs b, u32* tot) {
-  const u64 nbo = *nbo_dev;
-  GRID_STRIDE(i, nbo + 1) tot[i] = i < nbo ? b.ob_bytes[i] + 5 : 0;
-}
+// Items of the block offset scan: block bytes with their 5-byte trailer, 0
+// for the end.
+struct BlockTotLoad {
+  const u32* bytes;
+  const u32* nbo_dev;
+  __device__ u32 operator()(u64 i) const { return i < *nbo_dev ? bytes[i] + 5 : 0u; }
+};
 
-__global__ void k_block_forced(const u32* nbo_dev, EncBufs b) {
-  const u64 nbo = *nbo_dev;
-  GRID_STRIDE(i, nbo) {
-    if (b.ob_forced[i]) b.flist[b.ff[i]] = (u32)i;
+// The forced-block scan's epilogue: the list of forced blocks in order.
+struct ForcedListEpi : lbscan::NoEpi {
+  u32* flist;
+  __device__ void item(u64 i, u32 forced, u32 before, const Pre&) const {
+    if (forced) flist[before] = (u32)i;
   }
-}
+  __device__ void done(u32, u64) const {}
+};
 
 // The table DoCompactionWork would build starting at block i: blocks are
 // added until FileSize() >= max_file_size (db/db_impl.cc:1173-1180), or up to
@@ -404,14 +417,16 @@ __global__ void k_fnext(const u32* nbo_dev, const u32* nforced_dev, EncParams ep
   if ((threadIdx.x & 63) == 0 && smax) atomicMax(&st->file_step_max, smax);
 }
 
-__global__ void k_file_first(const u32* nbo_dev, EncBufs b, const u32* nf_dev) {
-  const u64 nbo = *nbo_dev;
-  const u32 nf = *nf_dev;  // 0 when the table count overflowed its capacity
-  GRID_STRIDE(i, nbo) {
-    if (b.fs[i] && b.fid[i] < nf) b.f_first[b.fid[i]] = (u32)i;
+// The table scan's epilogue (over the table-start flags fs): each table's
+// first block, and the block count after the last table.
+struct FileFirstEpi : lbscan::NoEpi {
+  u32* f_first;
+  u32 cap;  // tables the arrays hold
+  __device__ void item(u64 i, u32 start, u32 f, const Pre&) const {
+    if (start && f < cap) f_first[f] = (u32)i;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) b.f_first[*nf_dev] = (u32)nbo;
-}
+  __device__ void done(u32 nf, u64 nbo) const { f_first[nf] = (u32)nbo; }
+};
 
 __device__ __forceinline__ u32 file_of_block(const EncBufs& b, u32 nf, u64 i) {
   u32 lo = 0, hi = nf;
@@ -426,6 +441,7 @@ __device__ __forceinline__ u32 file_of_block(const EncBufs& b, u32 nf, u64 i) {
 __global__ void k_block_layout(Kept k, const u32* nbo_dev, const u32* nf_dev, EncParams ep, EncBufs b) {
   const u64 nbo = *nbo_dev;
   const u32 nf = *nf_dev;
+  if (blockIdx.x == 0 && threadIdx.x == 0) b.ob_isz[nbo] = 0;  // the end of the index size scan
   if (nf == 0) return;  // no tables, or their count overflowed (reported)
   GRID_STRIDE(i, nbo) {
     const u32 f = file_of_block(b, nf, i);
@@ -440,58 +456,73 @@ __global__ void k_block_layout(Kept k, const u32* nbo_dev, const u32* nf_dev, En
   }
 }
 
-__global__ void k_isz_sentinel(const u32* nbo_dev, EncBufs b) { b.ob_isz[*nbo_dev] = 0; }
-
 // Filter windows of a table: FilterBlockBuilder emits one filter per 2 KiB of
 // block start offsets up to the data end, plus one for keys still pending at
-// Finish (table/filter_block.cc:21-49).
-__global__ void k_file_layout(const u32* nf_dev, EncParams ep, EncBufs b) {
-  const u64 nf = *nf_dev;
-  GRID_STRIDE(f, nf) {
+// Finish (table/filter_block.cc:21-49).  Items of the window scan: table f's
+// window count (its data end to f_dend), 0 for the end.
+struct FileWindowsLoad {
+  EncBufs b;
+  EncParams ep;
+  const u32* nf_dev;
+  __device__ u32 operator()(u64 f) const {
+    if (f >= *nf_dev) return 0;
     const u32 b0 = b.f_first[f], b1 = b.f_first[f + 1];
     const u64 dend = b.O[b1] - b.O[b0];
     const u64 last = b.O[b1 - 1] - b.O[b0];
     b.f_dend[f] = dend;
-    u64 nw = 0;
-    if (ep.bloom_bits) nw = max(dend >> 11, (last >> 11) + 1);
-    b.f_nwin[f] = (u32)nw;
+    return ep.bloom_bits ? (u32)max(dend >> 11, (last >> 11) + 1) : 0u;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) b.f_nwin[nf] = 0;
-}
-
-__global__ void k_window_init(const u32* nw_dev, EncBufs b) {
-  const u64 nw = *nw_dev;
-  GRID_STRIDE(w, nw + 1) {
-    b.w_keys[w] = 0;
-    b.w_first[w] = 0xffffffffu;
-  }
-}
-
-__global__ void k_window_counts(const u32* nbo_dev, const u32* nf_dev, const u32* nw_dev, EncBufs b) {
-  const u64 nbo = *nbo_dev;
-  const u32 nw = *nw_dev;  // 0 when the window count overflowed its capacity
-  if (*nf_dev == 0 || nw == 0) return;
-  GRID_STRIDE(i, nbo) {
-    const u32 w = b.f_win[b.ob_file[i]] + (u32)(b.ob_off[i] >> 11);
-    if (w >= nw) continue;
-    atomicAdd(&b.w_keys[w], b.ob_n[i]);
-    atomicMin(&b.w_first[w], b.ob_first[i]);
-    b.w_file[w] = b.ob_file[i];  // every block of the window writes the same table
-  }
-}
+};
 
 // BloomFilterPolicy::CreateFilter sizes (util/bloom.cc:30-38).
-__global__ void k_window_sizes(const u32* nw_dev, EncParams ep, EncBufs b) {
-  const u64 nw = *nw_dev;
-  GRID_STRIDE(w, nw + 1) {
-    const u64 n = w < nw ? b.w_keys[w] : 0;
-    u64 sz = 0;
-    if (n) {
-      u64 bits = n * ep.bloom_bits;
-      if (bits < 64) bits = 64;
-      sz = (bits + 7) / 8 + 1;
+__device__ __forceinline__ u64 filter_bytes(u64 keys, u32 bits_per_key) {
+  if (!keys) return 0;
+  u64 bits = keys * bits_per_key;
+  if (bits < 64) bits = 64;
+  return (bits + 7) / 8 + 1;
+}
+
+// Every filter window's key count, first entry, table and filter size, with
+// one writer per window and no atomics.  Block i lies in window
+// w(i) = f_win[f] + (ob_off[i] >> 11), which never decreases with i.  The
+// first block of a window writes the window (its keys run to the first block
+// of a later window); the last block of a window also writes the empty
+// windows up to the next block's window, or to the end of its table.
+__global__ void k_windows(const u32* nbo_dev, const u32* nf_dev, const u32* nw_dev, EncParams ep, EncBufs b) {
+  const u64 nbo = *nbo_dev;
+  const u32 nw = *nw_dev;  // 0 when the window count overflowed its capacity
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    b.w_keys[nw] = 0;
+    b.w_first[nw] = 0xffffffffu;
+    b.w_fsz[nw] = 0;
+  }
+  if (*nf_dev == 0 || nw == 0) return;
+  GRID_STRIDE(i, nbo) {
+    const u32 f = b.ob_file[i];
+    const u32 wf = b.f_win[f];
+    const u64 sw = b.ob_off[i] >> 11;
+    const u32 w = wf + (u32)sw;
+    const bool first = i == 0 || b.ob_file[i - 1] != f || (b.ob_off[i - 1] >> 11) != sw;
+    const bool more = i + 1 < nbo && b.ob_file[i + 1] == f;
+    const u32 wn = more ? wf + (u32)(b.ob_off[i + 1] >> 11) : b.f_win[f + 1];
+    if (first && w < nw) {
+      u64 j = i + 1;
+      while (j < nbo && b.ob_file[j] == f && (b.ob_off[j] >> 11) == sw) j++;
+      const u32 end = j < nbo ? b.ob_first[j] : b.ob_first[nbo - 1] + b.ob_n[nbo - 1];
+      const u32 keys = end - b.ob_first[i];
+      b.w_keys[w] = keys;
+      b.w_first[w] = b.ob_first[i];
+      b.w_file[w] = f;
+      b.w_fsz[w] = filter_bytes(keys, ep.bloom_bits);
     }
-    b.w_fsz[w] = sz;
+    if (wn != w) {
+      for (u32 e = w + 1; e < wn && e < nw; e++) {
+        b.w_keys[e] = 0;
+        b.w_first[e] = 0xffffffffu;
+        b.w_file[e] = f;
+        b.w_fsz[e] = 0;
+      }
+    }
   }
 }
 
@@ -501,13 +532,14 @@ __device__ __forceinline__ u64 meta_block_size(const EncParams& ep, u64 dend, u6
   return 3 + 34 + hl + 4 + 4;
 }
 
-__global__ void k_file_sizes(const u32* nf_dev, EncParams ep, EncBufs b) {
-  const u64 nf = *nf_dev;
-  GRID_STRIDE(f, nf + 1) {
-    if (f == nf) {
-      b.f_off[f] = 0;
-      continue;
-    }
+// Items of the table offset scan: table f's file size (its filter, index and
+// layout fields along the way), 0 for the end.
+struct FileSizesLoad {
+  EncBufs b;
+  EncParams ep;
+  const u32* nf_dev;
+  __device__ u64 operator()(u64 f) const {
+    if (f >= *nf_dev) return 0;
     const u32 b0 = b.f_first[f], b1 = b.f_first[f + 1];
     const u64 dend = b.f_dend[f];
     u64 fb = 0, size = dend;
@@ -521,10 +553,9 @@ __global__ void k_file_sizes(const u32* nf_dev, EncParams ep, EncBufs b) {
     }
     const u64 ib = b.ob_isz[b1] - b.ob_isz[b0] + 4ull * (b1 - b0) + 4;
     b.f_ibytes[f] = ib;
-    size += meta_block_size(ep, dend, fb) + 5 + ib + 5 + 48;
-    b.f_off[f] = size;
+    return size + meta_block_size(ep, dend, fb) + 5 + ib + 5 + 48;
   }
-}
+};
 
 // ---------------------------------------------------------------------------
 // writers
@@ -1700,42 +1731,47 @@ void launch_entry_sizes(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncBufs 
 void launch_bnext(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncParams ep, EncBufs b, DevStatus* st) {
   KLAUNCH(T, "bnext", k_bnext, std::min<u32>(grid_for(m_cap, kBnStarts), 8192), 256, 0, k, m_dev, ep, b, st);
 }
-void launch_block_info(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncBufs b, DevStatus* st) {
-  KLAUNCH(T, "block_info", k_block_info, grid_for(m_cap, 256), 256, 0, k, m_dev, b, st);
+void scan_blocks(KTimer& T, ScanState& S, Kept k, const u32* m_dev, u32 m_cap, EncBufs b, u32 nbo_cap,
+                 DevStatus* st) {
+  const ScanGuard g{nbo_cap, &st->need_blocks};
+  lbscan::run<u32, u32>(T, "scan_blocks", S, lbscan::ArrLoad<u32>{b.bs}, (u32*)nullptr, m_dev, 0, m_cap,
+                        &st->n_blocks_out, st, &g, BlockInfoEpi{k.forced, b, nbo_cap});
 }
-void launch_block_tot(KTimer& T, const u32* nbo_dev, u32 nbo_cap, EncBufs b, u32* tot) {
-  KLAUNCH(T, "block_tot", k_block_tot, grid_for(nbo_cap + 1, 256), 256, 0, nbo_dev, b, tot);
+void scan_block_offsets(KTimer& T, ScanState& S, const u32* nbo_dev, u32 nbo_cap, EncBufs b, DevStatus* st) {
+  lbscan::run<u32, u64>(T, "scan_offsets", S, BlockTotLoad{b.ob_bytes, nbo_dev}, b.O, nbo_dev, 1, nbo_cap + 1,
+                        &st->data_bytes, st);
 }
-void launch_block_forced(KTimer& T, const u32* nbo_dev, u32 nbo_cap, EncBufs b) {
-  KLAUNCH(T, "block_forced", k_block_forced, grid_for(nbo_cap, 256), 256, 0, nbo_dev, b);
+void scan_forced(KTimer& T, ScanState& S, const u32* nbo_dev, u32 nbo_cap, EncBufs b, DevStatus* st) {
+  lbscan::run<u32, u32>(T, "scan_forced", S, lbscan::ArrLoad<u32>{b.ob_forced}, b.ff, nbo_dev, 0, nbo_cap,
+                        &st->n_forced, st, nullptr, ForcedListEpi{{}, b.flist});
 }
 void launch_fnext(KTimer& T, const u32* nbo_dev, u32 nbo_cap, const u32* nforced_dev, EncParams ep, EncBufs b,
                   DevStatus* st) {
   KLAUNCH(T, "fnext", k_fnext, grid_for(nbo_cap, 256), 256, 0, nbo_dev, nforced_dev, ep, b, st);
 }
-void launch_file_first(KTimer& T, const u32* nbo_dev, u32 nbo_cap, EncBufs b, const u32* nf_dev) {
-  KLAUNCH(T, "file_first", k_file_first, grid_for(nbo_cap, 256), 256, 0, nbo_dev, b, nf_dev);
+void scan_files(KTimer& T, ScanState& S, const u32* nbo_dev, u32 nbo_cap, EncBufs b, u32 nf_cap, DevStatus* st) {
+  const ScanGuard g{nf_cap, &st->need_files};
+  lbscan::run<u32, u32>(T, "scan_files", S, lbscan::ArrLoad<u32>{b.fs}, (u32*)nullptr, nbo_dev, 0, nbo_cap,
+                        &st->n_files_out, st, &g, FileFirstEpi{{}, b.f_first, nf_cap});
 }
 void launch_block_layout(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, const u32* nf_dev, EncParams ep,
                          EncBufs b) {
-  KLAUNCH(T, "isz_sentinel", k_isz_sentinel, 1, 1, 0, nbo_dev, b);
   KLAUNCH(T, "block_layout", k_block_layout, grid_for(nbo_cap, 256), 256, 0, k, nbo_dev, nf_dev, ep, b);
 }
-void launch_file_layout(KTimer& T, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b) {
-  KLAUNCH(T, "file_layout", k_file_layout, grid_for(nf_cap, 256), 256, 0, nf_dev, ep, b);
+void scan_windows(KTimer& T, ScanState& S, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b, u32 nw_cap,
+                  DevStatus* st) {
+  const ScanGuard g{nw_cap, &st->need_windows};
+  lbscan::run<u32, u32>(T, "scan_windows", S, FileWindowsLoad{b, ep, nf_dev}, b.f_win, nf_dev, 1, nf_cap + 1,
+                        &st->n_windows, st, ep.bloom_bits ? &g : nullptr);
 }
-void launch_window_init(KTimer& T, const u32* nw_dev, u32 nw_cap, EncBufs b) {
-  KLAUNCH(T, "window_init", k_window_init, grid_for(nw_cap + 1, 256), 256, 0, nw_dev, b);
+void launch_windows(KTimer& T, const u32* nbo_dev, u32 nbo_cap, const u32* nf_dev, const u32* nw_dev, EncParams ep,
+                    EncBufs b) {
+  KLAUNCH(T, "windows", k_windows, grid_for(nbo_cap, 256), 256, 0, nbo_dev, nf_dev, nw_dev, ep, b);
 }
-void launch_window_counts(KTimer& T, const u32* nbo_dev, u32 nbo_cap, const u32* nf_dev, const u32* nw_dev,
-                          EncBufs b) {
-  KLAUNCH(T, "window_counts", k_window_counts, grid_for(nbo_cap, 256), 256, 0, nbo_dev, nf_dev, nw_dev, b);
-}
-void launch_window_sizes(KTimer& T, const u32* nw_dev, u32 nw_cap, EncParams ep, EncBufs b) {
-  KLAUNCH(T, "window_sizes", k_window_sizes, grid_for(nw_cap + 1, 256), 256, 0, nw_dev, ep, b);
-}
-void launch_file_sizes(KTimer& T, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b) {
-  KLAUNCH(T, "file_sizes", k_file_sizes, grid_for(nf_cap + 1, 256), 256, 0, nf_dev, ep, b);
+void scan_file_offsets(KTimer& T, ScanState& S, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b,
+                       DevStatus* st) {
+  lbscan::run<u64, u64>(T, "scan_files_off", S, FileSizesLoad{b, ep, nf_dev}, b.f_off, nf_dev, 1, nf_cap + 1,
+                        &st->out_bytes, st);
 }
 void launch_write_blocks(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u8* out,
                          u64 out_cap, DevStatus* st) {

@@ -178,8 +178,9 @@ __global__ __launch_bounds__(256) void k_merge_tiles(const MergePair* pairs, u32
 // Entries of each child (child c = [cs[c], cs[c+1])) must be strictly
 // increasing: that is what makes the stable merge equal MergingIterator's
 // output.  A table that is not sorted is left to the CPU path.
-__global__ void k_check_sorted(Planes K, u64 n, const u32* cs, u32 nchild, DevStatus* st) {
+__global__ void k_check_sorted(Planes K, u64 n, const u32* cs, u32 nchild, u32* iota, DevStatus* st) {
   GRID_STRIDE(e, n) {
+    iota[e] = (u32)e;  // the merge's starting permutation
     if (e == 0) continue;
     u32 lo = 0, hi = nchild;
     while (hi - lo > 1) {
@@ -217,7 +218,7 @@ void launch_merge(KTimer& T, const MergePair* pairs, u32 npairs, u32 total_tiles
   else KLAUNCH(T, "merge", k_merge, grid_for(total_tiles, 256), 256, 0, pairs, npairs, total_tiles, in, out, K);
 }
 
-void launch_check_sorted(KTimer& T, Planes K, u64 n, const u32* child_start, u32 nchild, DevStatus* st) {
-  if (n < 2) return;
-  KLAUNCH(T, "check_sorted", k_check_sorted, grid_for(n, 256), 256, 0, K, n, child_start, nchild, st);
+void launch_check_sorted(KTimer& T, Planes K, u64 n, const u32* child_start, u32 nchild, u32* iota, DevStatus* st) {
+  if (!n) return;
+  KLAUNCH(T, "check_sorted", k_check_sorted, grid_for(n, 256), 256, 0, K, n, child_start, nchild, iota, st);
 }

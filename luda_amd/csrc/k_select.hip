@@ -14,6 +14,7 @@
 // the key passes a grandparent's largest key; those crossings are found in
 // parallel and the short sequential budget walk runs over them alone.
 #include "engine.h"
+#include "scan.h"
 
 namespace {
 
@@ -41,7 +42,8 @@ __device__ bool is_base_level_for_key(const Planes& K, u64 e, const DropParams& 
   return true;
 }
 
-__global__ void k_drop(const u32* perm, u32 n, Planes K, DropParams dp, u32* keep, u32* gcnt) {
+__global__ void k_drop(const u32* perm, u32 n, Planes K, DropParams dp, u32* keep, u32* gcnt, u32* nev) {
+  if (dp.ngp && blockIdx.x == 0 && threadIdx.x == 0) *nev = 0;  // the crossing count of the keep scan
   GRID_STRIDE(p, n) {
     const u32 e = perm[p];
     const u64 tag = K.tag[e];
@@ -85,14 +87,24 @@ __global__ void k_gather(const u32* perm, u32 n, const u32* keep, const u32* kpo
   }
 }
 
-// Positions p >= 1 where the grandparent count grows (at most ngp of them).
-__global__ void k_stop_events(const u32* gcnt, u32 n, u32* events, u32* nev, u32 cap) {
-  GRID_STRIDE(p, n) {
-    if (p == 0 || gcnt[p] <= gcnt[p - 1]) continue;
+// The keep scan's epilogue: positions p >= 1 where the grandparent count
+// grows (at most ngp of them), in any order.
+struct StopEventsEpi {
+  const u32* gcnt;
+  u32* events;
+  u32* nev;
+  u32 cap;
+  struct Pre {
+    bool cross;
+  };
+  __device__ Pre pre(u64 p, u32) const { return {events && p > 0 && gcnt[p] > gcnt[p - 1]}; }
+  __device__ void item(u64 p, u32, u32, const Pre& c) const {
+    if (!c.cross) return;
     const u32 i = atomicAdd(nev, 1u);
     if (i < cap) events[i] = (u32)p;
   }
-}
+  __device__ void done(u32, u64) const {}
+};
 
 constexpr u32 kStopThreads = 1024;
 constexpr u32 kMaxEvents = 2048;
@@ -170,11 +182,18 @@ u32 grid_for(u64 n, u32 per_block) {
 
 }  // namespace
 
-void launch_drop(KTimer& T, const u32* perm, u32 n, Planes K, const DropParams& dp, u32* keep, u32* gcnt,
+void launch_drop(KTimer& T, const u32* perm, u32 n, Planes K, const DropParams& dp, u32* keep, u32* gcnt, u32* events,
                  DevStatus* st) {
   (void)st;
   if (!n) return;
-  KLAUNCH(T, "drop", k_drop, grid_for(n, 256), 256, 0, perm, n, K, dp, keep, gcnt);
+  KLAUNCH(T, "drop", k_drop, grid_for(n, 256), 256, 0, perm, n, K, dp, keep, gcnt, events);
+}
+
+void scan_keep(KTimer& T, ScanState& S, const u32* keep, u32* kpos, u32 n, const DropParams& dp, const u32* gcnt,
+               u32* events, u32 ev_cap, DevStatus* st) {
+  const bool stops = n && dp.ngp;
+  lbscan::run<u32, u32>(T, "scan_keep", S, lbscan::ArrLoad<u32>{keep}, kpos, nullptr, n, n, &st->n_kept, st, nullptr,
+                        StopEventsEpi{gcnt, stops ? events + 1 : nullptr, events, ev_cap});
 }
 
 void launch_gather(KTimer& T, const u32* perm, u32 n, const u32* keep, const u32* kpos, Planes K, const u64* vptr,
@@ -187,9 +206,7 @@ void launch_stops(KTimer& T, const u32* gcnt, const u32* kpos, const u32* keep, 
                   u32* events, u32 ev_cap, Kept out, DevStatus* st) {
   (void)keep;
   if (!n || !dp.ngp) return;
-  // events[0] is the counter, events + 1 the list.
-  fill_u32(T, events, 0, 1);
-  KLAUNCH(T, "stop_events", k_stop_events, grid_for(n, 256), 256, 0, gcnt, n, events + 1, events, ev_cap);
+  // events[0] is the counter (zeroed by the drop kernel), events + 1 the list (filled by the keep scan).
   KLAUNCH(T, "stop_apply", k_stop_apply, 1, kStopThreads, 0, gcnt, kpos, n, dp, (const u32*)(events + 1),
           (const u32*)events, out, st);
 }
