@@ -442,6 +442,8 @@ struct LdsWin {
     return __builtin_bswap64(*reinterpret_cast<const u64_una*>(stage + mis + x));
   }
   __device__ __forceinline__ u32 at(u32 x) const { return stage[mis + x]; }
+  // bytes [x, x+4) little-endian (the stage is padded past the block)
+  __device__ __forceinline__ u32 le32(u32 x) const { return *reinterpret_cast<const u32_una*>(stage + mis + x); }
 };
 struct GlobalWin {
   static constexpr bool kLds = false;
@@ -453,6 +455,7 @@ struct GlobalWin {
     return v;
   }
   __device__ __forceinline__ u32 at(u32 x) const { return x < n ? p[x] : 0u; }
+  __device__ __forceinline__ u32 le32(u32 x) const { return at(x) | at(x + 1) << 8 | at(x + 2) << 16 | at(x + 3) << 24; }
 };
 template <typename Src>
 __device__ __forceinline__ u32 win_ld32(const Src& s, u32 x) {
@@ -495,10 +498,12 @@ struct KeyBytes {
 
 constexpr u32 kBadOff = 0xffff;  // eoff of an entry whose group failed its checks
 
-template <u32 NWMAX, typename Src>
+// `mid` runs once between the restart-group walk and the entry loop (when
+// nothing but the next block's loads is in flight).
+template <u32 NWMAX, typename Src, typename Mid>
 __device__ __forceinline__ void decode_entries(const Src& src, u32 n, u64 gp, u64 ebase, u32* gpre, u32 max_nr,
                                                u16* eoff, u32 ecap, const Planes& K, u64* vptr, u32* vlen,
-                                               DevStatus* st, u32& maxu) {
+                                               DevStatus* st, u32& maxu, Mid&& mid) {
   const u32 lane = threadIdx.x & 63;
   const u32 NW = K.W + 1;  // words of the internal key: user key words + the tag
   const u32 nr = n >= 4 ? win_ld32(src, n - 4) : 0;
@@ -537,10 +542,18 @@ __device__ __forceinline__ void decode_entries(const Src& src, u32 n, u64 gp, u6
         u32 off = start, i = 0;
         for (; ok && i < cnt; i++) {
           eoff[e0 + i] = (u16)off;
-          u32 sh = 0, ns = 0, vl = 0;
-          ok = win_entry_header(src, off, lim, sh, ns, vl);
-          if (ok && i == 0 && sh != 0) ok = false;
-          if (ok) off += ns + vl;
+          // three one-byte varints: the next entry at off + 3 + ns + vl
+          const u32 x = src.le32(off);
+          if ((x & 0x808080u) == 0) {
+            const u32 nx = off + 3 + ((x >> 8) & 0xff) + ((x >> 16) & 0xff);
+            ok = nx <= lim && (i != 0 || (x & 0xff) == 0);
+            off = nx;
+          } else {
+            u32 sh = 0, ns = 0, vl = 0;
+            ok = win_entry_header(src, off, lim, sh, ns, vl);
+            if (ok && i == 0 && sh != 0) ok = false;
+            if (ok) off += ns + vl;
+          }
         }
         if (ok && off != lim) ok = false;
         if (!ok)
@@ -549,6 +562,7 @@ __device__ __forceinline__ void decode_entries(const Src& src, u32 n, u64 gp, u6
     }
     __syncthreads();
   }
+  mid();
   u64 cm[NWMAX], cv[NWMAX];  // words of the entry before the chunk
 #pragma unroll
   for (u32 w = 0; w < NWMAX; w++) cm[w] = cv[w] = 0;
@@ -692,74 +706,154 @@ __device__ __forceinline__ void decode_entries(const Src& src, u32 n, u64 gp, u6
 
 // One wavefront per data block (see decode_entries).  The block is staged in
 // LDS with 16-byte loads when it fits `stage` bytes, else read in place; the
-// restart groups' entry prefix follows the stage.
+// restart groups' entry prefix follows two stages.  The next block's first
+// 8 KiB are loaded into registers while the current block's restart groups
+// are walked, and put into the other stage before its entries are decoded,
+// so a wave rarely waits for its bytes.  The descriptors of the wave's next
+// 64 blocks are loaded together into LDS.
+constexpr u32 kDescAhead = 64;
 template <u32 NWMAX>
 __global__ __launch_bounds__(64) void k_decode_entries(const BlockDesc* blks, u32 nb, const u32* blk_ebase, Planes K,
                                                        u64* vptr, u32* vlen, DevStatus* st, u32 stage, u32 max_nr,
                                                        u32 ecap) {
   extern __shared__ uint4 smem[];
-  u32* gpre = reinterpret_cast<u32*>(smem + stage / 16);
+  BlockDesc* dtab = reinterpret_cast<BlockDesc*>(smem + 2 * (stage / 16));
+  u64* etab = reinterpret_cast<u64*>(dtab + kDescAhead);
+  u32* gpre = reinterpret_cast<u32*>(etab + kDescAhead);
   u16* eoff = reinterpret_cast<u16*>(gpre + max_nr + 1);
   const u32 lane = threadIdx.x;
+  constexpr u32 kBatch = 8;  // 16-byte pieces per lane in flight: a wave covers 8 KiB
   u32 maxu = 0;
-  u32 b = blockIdx.x;
-  // the next block's descriptor is loaded one block ahead
-  BlockDesc nd = {0, 0, 0};
-  u64 ne = 0;
+  struct Span {
+    u64 a0;     // the block's bytes lie in [a0, a0 + 16 n16) (16-byte pieces)
+    u32 n16;    // 0 when not staged
+    bool staged;
+  };
+  auto span_of = [&](const BlockDesc& d) {
+    const u64 a0 = d.addr & ~15ull;
+    const u64 a1 = (d.addr + d.size + 15) & ~15ull;
+    const bool staged = a1 - a0 + 16 <= stage;
+    return Span{a0, staged ? (u32)((a1 - a0) / 16) : 0u, staged};
+  };
+  auto stage_of = [&](u32 buf) { return reinterpret_cast<u64*>(smem + buf * (stage / 16)); };
+  u64 lo[kBatch], hi[kBatch];
+  // pieces [x0, x0 + 64 kBatch) of a staged block (idle lanes load piece 0)
+  auto load_batch = [&](const Span& sp, u32 x0) {
+    const GMEM u32x4* g16 = (const GMEM u32x4*)sp.a0;
+#pragma unroll
+    for (u32 t = 0; t < kBatch; t++) {
+      const u32 x = x0 + 64 * t + lane;
+      const u32x4 v = g16[x < sp.n16 ? x : 0];
+      lo[t] = (u64)v[0] | ((u64)v[1] << 32);
+      hi[t] = (u64)v[2] | ((u64)v[3] << 32);
+    }
+  };
+  auto store_batch = [&](u64* s8, const Span& sp, u32 x0) {
+#pragma unroll
+    for (u32 t = 0; t < kBatch; t++) {
+      const u32 x = x0 + 64 * t + lane;
+      if (x < sp.n16) {
+        s8[2 * x] = lo[t];
+        s8[2 * x + 1] = hi[t];
+      }
+    }
+  };
+  const u64 G = gridDim.x;
+  u64 b = blockIdx.x;
+  // dtab[j], etab[j]: block b0 + j G (j < kDescAhead); the loop reloads them
+  // when it reaches j = kDescAhead - 1, so the next block's is always there
+  // (field by field: a struct copied between address spaces would go
+  // through scratch)
+  auto load_descs = [&](u64 b0) {
+    const u64 bb = b0 + (u64)lane * G;
+    u64 addr = 0, e = 0;
+    u32 size = 0, file = 0;
+    if (lane < kDescAhead && bb < nb) {
+      addr = blks[bb].addr;
+      size = blks[bb].size;
+      file = blks[bb].file;
+      e = blk_ebase[bb];
+    }
+    if (lane < kDescAhead) {
+      dtab[lane].addr = addr;
+      dtab[lane].size = size;
+      dtab[lane].file = file;
+      etab[lane] = e;
+    }
+    __syncthreads();
+  };
+  auto desc_at = [&](u32 jj) {
+    BlockDesc d;
+    d.addr = dtab[jj].addr;
+    d.size = dtab[jj].size;
+    d.file = dtab[jj].file;
+    return d;
+  };
+  BlockDesc cur = {0, 0, 0};
+  u64 ecur = 0;
+  Span sc = {0, 0, false};
+  bool held = false;  // lo/hi hold the first batch of `cur`, not yet in a stage
+  u32 cb = 0;         // the stage of `cur`
+  u32 j = 0;          // cur = dtab[j]
   if (b < nb) {
-    nd = blks[b];
-    ne = blk_ebase[b];
+    load_descs(b);
+    cur = desc_at(0);
+    ecur = etab[0];
+    sc = span_of(cur);
+    if (sc.n16) {
+      load_batch(sc, 0);
+      held = true;
+    }
   }
-  for (; b < nb; b += gridDim.x) {
-    const BlockDesc bd = nd;
-    const u64 ebase = ne;
+  for (; b < nb; b += G) {
+    if (j == kDescAhead - 1) {
+      load_descs(b);
+      j = 0;
+    }
+    // the rest of the stage: a held first batch, the batches past 8 KiB
+    u64* s8 = stage_of(cb);
+    if (held) {
+      store_batch(s8, sc, 0);
+      held = false;
+    }
+    for (u32 x0 = 64 * kBatch; x0 < sc.n16; x0 += 64 * kBatch) {
+      load_batch(sc, x0);
+      store_batch(s8, sc, x0);
+    }
+    __syncthreads();
+    const BlockDesc bd = cur;
+    const u64 ebase = ecur;
+    const bool staged = sc.staged;
+    sc = {0, 0, false};
+    j++;
+    if (b + G < nb) {
+      cur = desc_at(j);
+      ecur = etab[j];
+      sc = span_of(cur);
+      if (sc.n16) {
+        load_batch(sc, 0);
+        held = true;
+      }
+    }
+    u64* s8n = stage_of(cb ^ 1);
+    auto mid = [&]() {
+      if (held) {
+        store_batch(s8n, sc, 0);
+        held = false;
+      }
+    };
     const u8* gp = reinterpret_cast<const u8*>(bd.addr);
     const u32 n = bd.size;
-    const u64 a0 = (u64)gp & ~15ull;
-    const u64 a1 = ((u64)gp + n + 15) & ~15ull;
-    const bool staged = a1 - a0 + 16 <= stage;
-    const u32 n16 = staged ? (u32)((a1 - a0) / 16) : 0;
-    const GMEM u32x4* g16 = (const GMEM u32x4*)a0;
-    u64* s8 = reinterpret_cast<u64*>(smem);
-    // every staging load of a batch (16 bytes per lane, a wave covering 1 KiB
-    // contiguously) in flight before the first LDS store
-    constexpr u32 kBatch = 8;
-    for (u32 x0 = 0; x0 < n16; x0 += 64 * kBatch) {
-      u64 lo[kBatch], hi[kBatch];
-#pragma unroll
-      for (u32 t = 0; t < kBatch; t++) {
-        const u32 x = x0 + 64 * t + lane;
-        const u32 xc = x < n16 ? x : 0;  // in-bounds index for idle lanes
-        const u32x4 v = g16[xc];
-        lo[t] = (u64)v[0] | ((u64)v[1] << 32);
-        hi[t] = (u64)v[2] | ((u64)v[3] << 32);
-      }
-      if (x0 == 0 && b + gridDim.x < nb) {
-        nd = blks[b + gridDim.x];
-        ne = blk_ebase[b + gridDim.x];
-      }
-#pragma unroll
-      for (u32 t = 0; t < kBatch; t++) {
-        const u32 x = x0 + 64 * t + lane;
-        if (x < n16) {
-          s8[2 * x] = lo[t];
-          s8[2 * x + 1] = hi[t];
-        }
-      }
-    }
-    if (n16 == 0 && b + gridDim.x < nb) {
-      nd = blks[b + gridDim.x];
-      ne = blk_ebase[b + gridDim.x];
-    }
-    __syncthreads();
     if (staged) {
-      const LdsWin src{reinterpret_cast<const u8*>(smem), (u32)((u64)gp - a0)};
-      decode_entries<NWMAX>(src, n, (u64)gp, ebase, gpre, max_nr, eoff, ecap, K, vptr, vlen, st, maxu);
+      const LdsWin src{reinterpret_cast<const u8*>(s8), (u32)((u64)gp & 15)};
+      decode_entries<NWMAX>(src, n, (u64)gp, ebase, gpre, max_nr, eoff, ecap, K, vptr, vlen, st, maxu, mid);
     } else {
       decode_entries<NWMAX>(GlobalWin{gbytes((u64)gp), n}, n, (u64)gp, ebase, gpre, max_nr, eoff, 0, K, vptr, vlen, st,
-                            maxu);
+                            maxu, mid);
     }
+    mid();  // when decode_entries returned early
     __syncthreads();
+    cb ^= 1;
   }
   if (maxu) atomicMax(&st->max_ulen, maxu);
 }
@@ -867,12 +961,12 @@ void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 
                    u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr, u32 max_ent) {
   if (!nb) return;
   if (K.W <= 8) {
-    // stage: the largest block, its misalignment, and the 16 bytes an 8-byte
-    // window may read past it; blocks over 16 KiB are read in place.  Then
-    // the restart groups' entry prefix and, as far as 64 KiB allow, the
-    // entry offsets.
+    // two stages of the largest block, its misalignment, and the 16 bytes an
+    // 8-byte window may read past it; blocks over 16 KiB are read in place.
+    // Then the descriptors of the next blocks, the restart groups' entry
+    // prefix and, as far as 64 KiB allow, the entry offsets.
     const u32 stage = (u32)((std::min<u64>(max_blk, 16384) + 15 + 16 + 15) & ~15ull);
-    size_t lds = stage + 4ull * (max_nr + 1);
+    size_t lds = 2ull * stage + kDescAhead * (sizeof(BlockDesc) + 8) + 4ull * (max_nr + 1);
     if (lds > 65536) launch_refused("decode (restart arrays too long for LDS)", hipErrorInvalidValue);
     const u32 ecap = (u32)std::min<u64>((u64)max_ent, (65536 - lds) / 2);
     lds += ((2ull * ecap + 3) & ~3ull);
