@@ -167,16 +167,18 @@ __global__ void k_entry_sizes(Kept k, const u32* m_dev, EncBufs b, DevStatus* st
 // starting at q, the estimate is
 //   S(q, y) = sum of sznr over [q, y) + sum of (szr - sznr) over its restart
 //             points q, q + R, ... + 4 * (restart points) + 4,
-// which only grows with y.  A workgroup takes 1024 starts (4 consecutive ones
-// per thread) and stages kBnSpan entries past them in LDS as a prefix sum of
-// sznr (with the count of forced entries packed above it) and the restart
-// surcharges.  Per start, the prefix alone bounds the cut from above
+// which only grows with y.  A workgroup takes 2048 starts (8 consecutive ones
+// per thread: the whole grid of a 256 MiB job then fits the device at once,
+// five workgroups per CU) and stages kBnSpan entries past them in LDS as a
+// prefix sum of sznr (with the count of forced entries packed above it) and
+// the restart surcharges.  Per start, the prefix alone bounds the cut from above
 // (S >= prefix difference + 8); a backward walk then finds the exact first y
 // with S >= block_size.  The bound only moves forward from one start to the
 // next.  A block that runs past the staged span is finished by walking the
 // entries in global memory.
-constexpr u32 kBnStarts = 1024;
-constexpr u32 kBnSpan = 768;
+constexpr u32 kBnPerThread = 8;                 // starts per thread
+constexpr u32 kBnStarts = 256 * kBnPerThread;
+constexpr u32 kBnSpan = 512;
 constexpr u32 kBnTile = kBnStarts + kBnSpan;     // staged entries
 constexpr u32 kBnPer = (kBnTile + 255) / 256;    // staged entries per thread
 constexpr u64 kCfOne = 1ull << 48;               // one forced entry in the packed prefix
@@ -244,8 +246,8 @@ __global__ __launch_bounds__(256) void k_bnext(Kept k, const u32* m_dev, EncPara
     // entries per block at the tile's mean entry size: the first search's guess
     const u64 tbytes = sP[lim] & kByteMask;
     const u32 est = tbytes ? (u32)min<u64>((u64)lim, (bs * lim) / tbytes) : 1;
-    for (u32 s = 0; s < 4; s++) {
-      const u32 x = 4 * tid + s;
+    for (u32 s = 0; s < kBnPerThread; s++) {
+      const u32 x = kBnPerThread * tid + s;
       const u64 q = q0 + x;
       if (x >= kBnStarts || q >= m) break;
       const u64 px = sP[x];
