@@ -196,6 +196,99 @@ def run_mix(args):
     eng.close()
 
 
+def release_tree(lsm):
+    for level in lsm.levels:
+        for f in level:
+            f.owner.drop()
+        level.clear()
+
+
+def run_serve(args):
+    """The served form on a fillrandom write stream: memtable flushes and the
+    leveled compactions they cause, tables resident in HBM from flush to
+    deletion.  A calibration run pushes the whole stream through the plain
+    path (one job at a time, every flush available at once); its time per
+    flush is the plain path's sustainable rate.  Then, for each rate multiple,
+    flushes arrive at that multiple of the sustainable rate and the same
+    stream runs through the plain path and through the served form
+    (luda_submit_build / luda_submit on the worker slots).  Reported per run:
+    aggregate input GB/s (flush inputs are memtable bytes, compaction inputs
+    table bytes) and p50/p99 job latency (a flush from its arrival, a
+    compaction from the moment the tree first needed it)."""
+    from luda_amd import served
+    t = time.time()
+    mems = list(served.write_stream(args.serve, args.mix_seed))
+    log(f"{args.serve} puts in {len(mems)} memtables generated in {time.time() - t:.1f}s")
+    opts = ffi.default_options()
+    peng = jobs.Engine(0)
+    seng = jobs.Engine(0)
+    seng.set_slots(args.slots)
+    pbe = served.EngineBackend(peng, opts)
+    sbe = served.EngineBackend(seng, opts)
+    # warm-up: the whole stream through both paths, so that every working
+    # buffer and output buffer is allocated before anything is timed
+    for be, run in ((pbe, served.run_plain), (sbe, served.run_served)):
+        lsm, _ = run(be, mems, 0.0)
+        release_tree(lsm)
+    pbe.acct = dict.fromkeys(pbe.acct, 0)
+    lsm, st = served.run_plain(pbe, mems, 0.0)
+    cal = st.summary()
+    cal["engine_calls"] = {k: round(v, 3) for k, v in pbe.acct.items()}
+    release_tree(lsm)
+    per_flush_ms = (st.end_ms - st.t0_ms) / len(mems)
+    log(f"plain path, whole stream at once: {cal}")
+    out = {"metric": "served compaction: aggregate input GB/s and p50/p99 job latency, served form vs the plain path",
+           "puts": args.serve, "memtables": len(mems), "slots": args.slots,
+           "workload": "fillrandom: 16-byte keys uniform over 1e9, 100-byte random values, 4 MiB memtables "
+                       "(MemTable::Add bytes), LevelDB picking/scoring rules, tables resident in HBM",
+           "calibration": cal, "sustainable_flush_interval_ms": round(per_flush_ms, 4), "rates": []}
+    checked = [0, 0]
+    for k in [float(x) for x in args.rates.split(",")]:
+        interval = per_flush_ms / k
+        row = {"rate": k, "flush_interval_ms": round(interval, 4)}
+        pbe.acct = dict.fromkeys(pbe.acct, 0)
+        lsm, st = served.run_plain(pbe, mems, interval)
+        row["plain"] = st.summary()
+        row["plain"]["engine_calls"] = {k: round(v, 3) for k, v in pbe.acct.items()}
+        release_tree(lsm)
+        samples = []
+
+        def on_job(spec, files):
+            # every 8th compaction is kept (inputs and outputs held) and
+            # checked after the timed run
+            if len(samples) < 16 and len(samples_seen) % 8 == 0:
+                for f in spec.inputs[0] + spec.inputs[1] + files:
+                    f.owner.refs += 1
+                samples.append((spec, files))
+            samples_seen.append(1)
+        samples_seen = []
+        sbe.acct = dict.fromkeys(sbe.acct, 0)
+        lsm, st = served.run_served(sbe, mems, interval, on_job=on_job)
+        row["served"] = st.summary()
+        row["served"]["engine_calls"] = {k: round(v, 3) for k, v in sbe.acct.items()}
+        lsm.check()
+        release_tree(lsm)
+        scratch = served.Lsm()
+        for spec, kept in samples:
+            want = [sha(sbe.table_bytes(f)) for f in kept]
+            files, _ = pbe.compact(scratch, spec)
+            checked[0] += 1
+            checked[1] += [sha(pbe.table_bytes(f)) for f in files] == want
+            if files:
+                files[0].owner.refs = 1
+                files[0].owner.drop()
+            for f in spec.inputs[0] + spec.inputs[1] + kept:
+                f.owner.drop()
+        row["served_vs_plain_gbps"] = round(row["served"]["gbps"] / max(1e-9, row["plain"]["gbps"]), 3)
+        log(f"rate {k}x: {row}")
+        out["rates"].append(row)
+    out["jobs_checked_alone"] = checked[0]
+    out["same_alone"] = checked[0] == checked[1]
+    print(json.dumps(out), flush=True)
+    seng.close()
+    peng.close()
+
+
 def plain_hash_free(eng, results, summary):
     for res in results:
         yield res
@@ -216,6 +309,10 @@ def main():
                          "plain path, one JSON line of aggregate GB/s and job latency")
     ap.add_argument("--mix-seed", type=int, default=3)
     ap.add_argument("--slots", type=int, default=3)
+    ap.add_argument("--serve", type=int, default=0, metavar="PUTS",
+                    help="instead: a fillrandom stream of PUTS puts through the served form against the plain "
+                         "path at --rates multiples of the plain path's sustainable rate, one JSON line")
+    ap.add_argument("--rates", default="1,2,4")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (the known-size kernel that rocprofv3's "
                          "FETCH_SIZE/WRITE_SIZE are scaled against) and exit")
@@ -225,6 +322,9 @@ def main():
         return
     if args.mix:
         run_mix(args)
+        return
+    if args.serve:
+        run_serve(args)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: a compaction job runs whole on one device and this build has no "

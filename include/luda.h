@@ -130,6 +130,7 @@ typedef struct luda_stats {
   uint32_t key_width;       /* user-key bytes held per entry in the key planes */
   uint64_t data_block_bytes;  /* output data blocks with their trailers */
   uint64_t top_kernel_bytes;  /* algorithmic HBM bytes of top_kernel's launches in this call */
+  double end_clock_ms;        /* CLOCK_MONOTONIC, in milliseconds, when the result was complete */
 } luda_stats;
 
 typedef struct luda_result {
@@ -187,6 +188,14 @@ typedef void (*luda_done_fn)(void* user, int status, luda_result* result, const 
 /* Worker slots (default 3); only before the first luda_submit. */
 int luda_set_slots(luda_ctx* ctx, uint32_t slots);
 int luda_submit(luda_ctx* ctx, const luda_job* job, luda_done_fn done, void* user);
+/* Queues a memtable flush: the arguments of luda_build_tables, which must
+ * stay valid until `done` runs.  Flushes share the worker slots and the
+ * budget with compactions, so a flush that arrives while compactions run is
+ * admitted beside them.  Replaces DBImpl::CompactMemTable's BuildTable call
+ * (db/db_impl.cc:645-653 with the GPU gate :94-97). */
+int luda_submit_build(luda_ctx* ctx, const luda_options* opts, uint64_t n, const uint8_t* keys,
+                      const uint64_t* key_off, const uint8_t* vals, const uint64_t* val_off, uint32_t flags,
+                      luda_done_fn done, void* user);
 /* Runs the callbacks of finished jobs; returns how many (never blocks). */
 int luda_poll(luda_ctx* ctx);
 /* Waits for every submitted job and runs its callback; returns how many. */

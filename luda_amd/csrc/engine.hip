@@ -182,6 +182,7 @@ struct luda_ctx {
   u32 kw_hint = 0;   // key words that fit the keys seen so far
   KTimer T;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev_pin2 = nullptr;  // recorded after the last copy out of pin2
   std::multimap<u64, void*> out_pool;  // freed output buffers by size
 
   void* take_out(u64 bytes) {
@@ -249,12 +250,15 @@ void fill_status(luda_ctx* c, const DevStatus& init) {
 }
 
 // Copies host bytes to device memory through the pinned staging buffer.
+// Waits only for the previous copy out of the staging buffer (usually long
+// done), not for the whole stream.
 void upload(luda_ctx* c, void* dst, const void* src, u64 bytes) {
   if (!bytes) return;
+  HIPCHK(hipEventSynchronize(c->ev_pin2));
   HIPCHK(c->pin2.reserve(bytes));
-  HIPCHK(hipStreamSynchronize(c->s));
   memcpy(c->pin2.p, src, bytes);
   HIPCHK(hipMemcpyAsync(dst, c->pin2.p, bytes, hipMemcpyHostToDevice, c->s));
+  HIPCHK(hipEventRecord(c->ev_pin2, c->s));
 }
 
 struct EncodeArrays {
@@ -617,6 +621,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     r->keys = (uint8_t*)malloc(1);
     r->stats = stats;
     r->stats.attempts = 1;
+    r->stats.end_clock_ms = now_ms();
     return r;
   }
 
@@ -833,10 +838,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     void* out = c->take_out(std::max<u64>(out_cap, 1));
     if (!out) throw Fail{LUDA_IO_ERROR, "output allocation failed"};
     {
-      HIPCHK(c->pin2.reserve(blob.size() + 16));
-      HIPCHK(hipStreamSynchronize(c->s));  // the staging buffer may still feed an earlier copy
-      memcpy(c->pin2.p, blob.data(), blob.size());
-      HIPCHK(hipMemcpyAsync(d_tab, c->pin2.p, blob.size(), hipMemcpyHostToDevice, c->s));
+      upload(c, d_tab, blob.data(), blob.size());
 
       dp.S = job->smallest_snapshot;
       dp.limit = 10 * job->options.max_file_size;  // MaxGrandParentOverlapBytes, db/version_set.cc:31-33
@@ -948,6 +950,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     luda_result* r = finish_result(c, es, EA, out, stB, job->flags, stats,
                                    reinterpret_cast<const FileMeta*>(c->pin.p + o_meta), c->pin.p + o_keys);
     r->stats.total_ms = now_ms() - t_call;
+    r->stats.end_clock_ms = now_ms();
     return r;
   }
 }
@@ -980,6 +983,7 @@ luda_result* build_tables(luda_ctx* c, const luda_options* opts, u64 n, const u8
     r->keys = (uint8_t*)malloc(1);
     r->stats = stats;
     r->stats.attempts = 1;
+    r->stats.end_clock_ms = now_ms();
     return r;
   }
   // inputs: keys, offsets and values uploaded once
@@ -1075,6 +1079,7 @@ luda_result* build_tables(luda_ctx* c, const luda_options* opts, u64 n, const u8
     luda_result* r = finish_result(c, es, EA, out, stB, flags, stats,
                                    reinterpret_cast<const FileMeta*>(c->pin.p + o_meta), c->pin.p + o_keys);
     r->stats.total_ms = now_ms() - t_call;
+    r->stats.end_clock_ms = now_ms();
     return r;
   }
 }
@@ -1096,8 +1101,19 @@ int guarded(luda_ctx* c, char* err, size_t errlen, F fn) {
 // (stream, working memory, output buffers) driven by one host thread; the
 // queue admits a job when a slot is idle and the estimated device memory of
 // the running jobs plus this one fits the budget (or nothing runs).
+// A flush queued by luda_submit_build: the arguments of luda_build_tables.
+struct BuildReq {
+  luda_options opts;
+  u64 n;
+  const u8* keys;
+  const u64* key_off;
+  const u8* vals;
+  const u64* val_off;
+  u32 flags;
+};
 struct QueuedJob {
-  const luda_job* job;
+  const luda_job* job;  // nullptr for a flush
+  BuildReq build;
   luda_done_fn done;
   void* user;
   u64 est;  // estimated device bytes while it runs
@@ -1115,6 +1131,11 @@ u64 job_footprint(const luda_job* j) {
   u64 in = 0;
   for (int w = 0; w < 2; w++)
     for (u32 i = 0; i < j->num_inputs[w]; i++) in += j->inputs[w][i].size;
+  return in * 9 / 2 + (64ull << 20);
+}
+
+u64 build_footprint(const BuildReq& b) {
+  const u64 in = b.n ? b.key_off[b.n] + b.val_off[b.n] : 0;
   return in * 9 / 2 + (64ull << 20);
 }
 
@@ -1138,6 +1159,8 @@ namespace {
 int ctx_open(int device, u64 budget, luda_ctx** out, char* err, size_t errlen);
 void ctx_close(luda_ctx* c);
 int ctx_compact(luda_ctx* c, const luda_job* job, luda_result** out, char* err, size_t errlen);
+int ctx_build(luda_ctx* c, const luda_options* opts, u64 n, const u8* keys, const u64* key_off, const u8* vals,
+              const u64* val_off, u32 flags, luda_result** out, char* err, size_t errlen);
 
 void pool_worker(JobPool* P, luda_ctx* w) {
   hipSetDevice(w->device);
@@ -1154,7 +1177,10 @@ void pool_worker(JobPool* P, luda_ctx* w) {
     lk.unlock();
     luda_result* r = nullptr;
     char err[256] = {0};
-    const int rc = ctx_compact(w, q.job, &r, err, sizeof(err));
+    const BuildReq& b = q.build;
+    const int rc = q.job ? ctx_compact(w, q.job, &r, err, sizeof(err))
+                         : ctx_build(w, &b.opts, b.n, b.keys, b.key_off, b.vals, b.val_off, b.flags, &r, err,
+                                     sizeof(err));
     lk.lock();
     P->running--;
     P->inflight -= q.est;
@@ -1254,7 +1280,23 @@ int luda_submit(luda_ctx* c, const luda_job* job, luda_done_fn done, void* user)
   if (!P) return LUDA_IO_ERROR;
   {
     std::lock_guard<std::mutex> lk(P->mu);
-    P->queue.push_back(QueuedJob{job, done, user, job_footprint(job)});
+    P->queue.push_back(QueuedJob{job, BuildReq{}, done, user, job_footprint(job)});
+    P->submitted++;
+  }
+  P->cv_work.notify_one();
+  return LUDA_OK;
+}
+
+int luda_submit_build(luda_ctx* c, const luda_options* opts, uint64_t n, const uint8_t* keys,
+                      const uint64_t* key_off, const uint8_t* vals, const uint64_t* val_off, uint32_t flags,
+                      luda_done_fn done, void* user) {
+  if (!c || !opts || (n && (!keys || !key_off || !vals || !val_off))) return LUDA_INVALID_ARGUMENT;
+  JobPool* P = pool_of(c);
+  if (!P) return LUDA_IO_ERROR;
+  BuildReq b{*opts, n, keys, key_off, vals, val_off, flags};
+  {
+    std::lock_guard<std::mutex> lk(P->mu);
+    P->queue.push_back(QueuedJob{nullptr, b, done, user, build_footprint(b)});
     P->submitted++;
   }
   P->cv_work.notify_one();
@@ -1292,6 +1334,7 @@ int ctx_open(int device, uint64_t hbm_budget_bytes, luda_ctx** out, char* err, s
     HIPCHK(hipMalloc(&c->st, sizeof(DevStatus)));
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));
+    HIPCHK(hipEventCreateWithFlags(&c->ev_pin2, hipEventDisableTiming));
     HIPCHK(c->pin.reserve(1 << 16));
     c->T.s = c->s;
   });
@@ -1317,6 +1360,7 @@ void ctx_close(luda_ctx* c) {
   if (c->st) hipFree(c->st);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->ev_pin2) hipEventDestroy(c->ev_pin2);
   if (c->s) hipStreamDestroy(c->s);
   delete c;
 }
@@ -1327,6 +1371,15 @@ int ctx_compact(luda_ctx* c, const luda_job* job, luda_result** out, char* err, 
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);
   return guarded(c, err, errlen, [&] { *out = compact(c, job); });
+}
+
+int ctx_build(luda_ctx* c, const luda_options* opts, u64 n, const u8* keys, const u64* key_off, const u8* vals,
+              const u64* val_off, u32 flags, luda_result** out, char* err, size_t errlen) {
+  if (!c || !opts || !out || (n && (!keys || !key_off || !vals || !val_off))) return LUDA_INVALID_ARGUMENT;
+  *out = nullptr;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  return guarded(c, err, errlen, [&] { *out = build_tables(c, opts, n, keys, key_off, vals, val_off, flags); });
 }
 
 }  // namespace
@@ -1340,11 +1393,7 @@ int luda_compact(luda_ctx* c, const luda_job* job, luda_result** out, char* err,
 int luda_build_tables(luda_ctx* c, const luda_options* opts, uint64_t n, const uint8_t* keys, const uint64_t* key_off,
                       const uint8_t* vals, const uint64_t* val_off, uint32_t flags, luda_result** out, char* err,
                       size_t errlen) {
-  if (!c || !opts || !out || (n && (!keys || !key_off || !vals || !val_off))) return LUDA_INVALID_ARGUMENT;
-  *out = nullptr;
-  std::lock_guard<std::mutex> lk(c->mu);
-  hipSetDevice(c->device);
-  return guarded(c, err, errlen, [&] { *out = build_tables(c, opts, n, keys, key_off, vals, val_off, flags); });
+  return ctx_build(c, opts, n, keys, key_off, vals, val_off, flags, out, err, errlen);
 }
 
 void luda_free_result(luda_ctx* c, luda_result* r) {

@@ -114,6 +114,7 @@ class luda_stats(C.Structure):
         ("key_width", C.c_uint32),
         ("data_block_bytes", C.c_uint64),
         ("top_kernel_bytes", C.c_uint64),
+        ("end_clock_ms", C.c_double),
     ]
 
 
@@ -178,6 +179,8 @@ ENGINE_SIGNATURES = {
     "luda_version": (C.c_char_p, []),
     "luda_set_slots": (C.c_int, [C.c_void_p, C.c_uint32]),
     "luda_submit": (C.c_int, [C.c_void_p, C.POINTER(luda_job), C.c_void_p, C.c_void_p]),
+    "luda_submit_build": (C.c_int, [C.c_void_p, C.POINTER(luda_options), C.c_uint64, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "luda_poll": (C.c_int, [C.c_void_p]),
     "luda_drain": (C.c_int, [C.c_void_p]),
 }

@@ -213,6 +213,19 @@ class Engine:
         message) runs from poll()/drain().  The job must stay alive until then."""
         if keep_on_device:
             job.flags |= ffi.LUDA_JOB_KEEP_ON_DEVICE
+        return self._queue(lambda uid: self.lib.luda_submit(self.ctx, C.byref(job), C.cast(self._done_fn, C.c_void_p),
+                                                            C.c_void_p(uid)), on_done, job, keep_on_device)
+
+    def submit_build(self, opts, n, keys, key_off, vals, val_off, on_done, keep_on_device=False):
+        """Queues a flush (luda_submit_build); the arrays must stay alive until
+        on_done runs."""
+        flags = ffi.LUDA_JOB_KEEP_ON_DEVICE if keep_on_device else 0
+        hold = (opts, keys, key_off, vals, val_off)
+        return self._queue(lambda uid: self.lib.luda_submit_build(
+            self.ctx, C.byref(opts), n, keys, key_off, vals, val_off, flags, C.cast(self._done_fn, C.c_void_p),
+            C.c_void_p(uid)), on_done, hold, keep_on_device)
+
+    def _queue(self, call, on_done, hold, keep_on_device):
         if not hasattr(self, "_pending"):
             self._pending = {}
             self._next_id = 1
@@ -232,8 +245,8 @@ class Engine:
             self._done_fn = ffi.DONE_FN(trampoline)
         uid = self._next_id
         self._next_id += 1
-        self._pending[uid] = (on_done, job, keep_on_device)
-        rc = self.lib.luda_submit(self.ctx, C.byref(job), C.cast(self._done_fn, C.c_void_p), C.c_void_p(uid))
+        self._pending[uid] = (on_done, hold, keep_on_device)
+        rc = call(uid)
         if rc != ffi.LUDA_OK:
             self._pending.pop(uid)
             raise RuntimeError(f"luda_submit failed ({rc})")

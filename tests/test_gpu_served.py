@@ -1,0 +1,73 @@
+"""The served form on the GPU: flushes and compactions of a write stream
+queued on the engine's worker slots (luda_submit_build / luda_submit), with
+tables resident in HBM from flush to deletion.  Every sampled job's output is
+the same as the same job run alone through luda_compact and as the oracle's
+on host copies of its inputs, and the finished tree holds exactly the newest
+put of every key."""
+import ctypes as C
+import hashlib
+
+import pytest
+
+import oracle
+from luda_amd import ffi, jobs, served
+from served_backends import expected_contents, small_options, tree_contents
+
+pytestmark = pytest.mark.gpu
+
+
+def _sha(b):
+    return hashlib.sha256(b).hexdigest()
+
+
+@pytest.mark.parametrize("slots", [3, 1])
+def test_served_stream_matches_alone(slots):
+    print(f"served stream, {slots} slots", flush=True)
+    mems = list(served.write_stream(60000, 11, key_space=40000, entries_per_flush=2000))
+    opts = small_options(32 << 10)
+    beng = jobs.Engine(0)
+    beng.set_slots(slots)
+    peng = jobs.Engine(0)
+    try:
+        be = served.EngineBackend(beng, opts)
+        lsm = served.Lsm(max_file_size=opts.max_file_size, level1_bytes=96 << 10)
+        samples = []
+
+        def on_job(spec, files):
+            if len(samples) < 24:
+                for which in (0, 1):
+                    for f in spec.inputs[which]:
+                        f.owner.refs += 1  # held until the job is re-run alone
+                samples.append((spec, [_sha(be.table_bytes(f)) for f in files]))
+        lsm, st = served.run_served(be, mems, 0.0, lsm=lsm, on_job=on_job)
+        lsm.check()
+        summ = st.summary()
+        assert summ["flushes"] == len(mems) and summ["compactions"] >= 10
+        assert samples
+        pbe = served.EngineBackend(peng, opts)
+        scratch = served.Lsm()
+        for spec, want in samples:
+            files, _ = pbe.compact(scratch, spec)
+            got = [_sha(pbe.table_bytes(f)) for f in files]
+            assert got == want, f"level-{spec.level} job differs when run alone"
+            # the oracle on host copies of the same inputs
+            hjob, keep = served._job_struct(spec, opts)
+            hjob.flags = 0
+            host = []
+            for which in (0, 1):
+                for i, f in enumerate(spec.inputs[which]):
+                    b = C.create_string_buffer(be.table_bytes(f), f.size)
+                    host.append(b)
+                    hjob.inputs[which][i].data = C.cast(b, C.c_void_p)
+            rc, tables = oracle.compact(hjob)
+            assert rc == 0 and [_sha(t.data) for t in tables] == want, "differs from the oracle"
+            if files:
+                files[0].owner.refs = 1
+                files[0].owner.drop()
+            for which in (0, 1):
+                for f in spec.inputs[which]:
+                    f.owner.drop()
+        assert tree_contents(lsm, be.table_bytes) == expected_contents(mems)
+    finally:
+        peng.close()
+        beng.close()

@@ -1,0 +1,126 @@
+"""The served form's host side (luda_amd/served.py) on the CPU, with the oracle
+doing the table work: the tree it keeps is a valid LevelDB tree and holds
+exactly the newest put of every key, run one job at a time and with queued
+jobs finishing out of order."""
+import pytest
+
+from luda_amd import served
+from served_backends import OracleBackend, expected_contents, small_options, tree_contents
+
+
+def stream(puts=40000, key_space=30000, epf=1500, seed=7):
+    return list(served.write_stream(puts, seed, key_space=key_space, entries_per_flush=epf))
+
+
+def test_write_stream_memtables_sorted():
+    mems = stream(puts=5000, epf=1200)
+    assert [m.n for m in mems] == [1200, 1200, 1200, 1200, 200]
+    seq = 1
+    for m in mems:
+        assert m.first_seq == seq
+        seq = m.last_seq + 1
+        keys = [bytes(k) for k in m.keys]
+        assert all(len(k) == 24 and k[:16].isdigit() for k in keys)
+        assert keys == sorted(keys, key=served.ikey_order)
+        tags = sorted(int.from_bytes(k[16:], "little") for k in keys)
+        assert tags == [(s << 8) | 1 for s in range(m.first_seq, m.last_seq + 1)]
+
+
+def test_max_bytes_for_level():
+    assert served.max_bytes_for_level(1) == 10 * 1048576
+    assert served.max_bytes_for_level(4) == 10000 * 1048576
+
+
+@pytest.mark.parametrize("mode", ["plain", "served"])
+def test_tree_holds_newest_puts(mode):
+    mems = stream()
+    opts = small_options()
+    be = OracleBackend(opts)
+    lsm = served.Lsm(max_file_size=opts.max_file_size, level1_bytes=48 << 10)
+    seen = []
+
+    def on_job(spec, files):
+        # every compaction writes its outputs sorted and disjoint
+        for a, b in zip(files, files[1:]):
+            assert served.ikey_order(a.largest) < served.ikey_order(b.smallest)
+        seen.append((spec.level, len(spec.inputs[0]), len(spec.inputs[1])))
+
+    run = served.run_plain if mode == "plain" else served.run_served
+    lsm, st = run(be, mems, 0.0, lsm=lsm, on_job=on_job)
+    lsm.check()
+    summ = st.summary()
+    assert summ["flushes"] == len(mems)
+    assert summ["compactions"] == len(seen) > 0
+    assert {level for level, _, _ in seen} >= {0, 1}
+    assert lsm.levels[3] and lsm.moves > 0, "the stream never reached level 3"
+    assert not lsm.needs()
+    assert tree_contents(lsm, be.table_bytes) == expected_contents(mems)
+
+
+def test_picker_rules():
+    """Level-0 inputs are widened to every overlapping level-0 file, level+1
+    inputs are the overlapping files, and a lone file with nothing below it
+    moves down."""
+    lsm = served.Lsm(max_file_size=1 << 20, level1_bytes=1 << 20)
+
+    def f(lo, hi, size=1000, seq=1):
+        tag = ((seq << 8) | 1).to_bytes(8, "little")
+        return served.File(lsm.new_number(), size, lo.encode() + tag, hi.encode() + tag, 1, None)
+    a, b, c, d = f("b", "d"), f("c", "f"), f("e", "g"), f("x", "z")
+    for x in (a, b, c, d):
+        lsm.add_file(0, x)
+    l1 = [f("a", "c"), f("d", "e"), f("h", "k")]
+    for x in l1:
+        lsm.add_file(1, x)
+    spec = lsm.pick()
+    assert spec.level == 0
+    assert set(spec.inputs[0]) == {a, b, c}
+    assert set(spec.inputs[1]) == {l1[0], l1[1]}
+    assert lsm.pick() is None  # levels 0 and 1 are in use
+    lsm.install(spec, [f("a", "g")])
+    assert [x for x in lsm.levels[0]] == [d]
+    lsm.check()
+    # a single level-1 file over the limit with no level-2 overlap moves down
+    lsm2 = served.Lsm(max_file_size=1 << 20, level1_bytes=1000)
+    lsm = lsm2
+    g = f("m", "n", size=5000)
+    lsm2.add_file(1, g)
+    assert lsm2.pick() is g
+    assert lsm2.levels[2] == [g] and lsm2.moves == 1
+
+
+def test_cached_deeper_rows_match_the_job():
+    """The deeper-level rows an EngineBackend takes from its per-level cache
+    are the rows of the job's own deeper list, and the cache is not used for a
+    job picked from another tree or before a deeper level changed."""
+    import ctypes as C
+    import numpy as np
+    lsm = served.Lsm(max_file_size=1 << 20, level1_bytes=1000)
+
+    def f(lo, hi, size=600, seq=1):
+        tag = ((seq << 8) | 1).to_bytes(8, "little")
+        return served.File(lsm.new_number(), size, lo.encode() + tag, hi.encode() + tag, 1, 0)
+    lsm.add_file(1, f("a", "c"))
+    lsm.add_file(1, f("d", "f"))
+    lsm.add_file(2, f("b", "e"))
+    for i, k in enumerate("ghijk"):
+        lsm.add_file(3 + i % 3, f(k + "0", k + "9"))
+    spec = lsm.pick()
+    assert spec.level == 1 and len(spec.deeper) == 5
+    cache = served._DeeperCache()
+    opts = small_options()
+
+    def rows(job):
+        n = job.num_deeper
+        return bytes(C.string_at(C.cast(job.deeper, C.c_void_p), n * C.sizeof(served.ffi.luda_level_file)))
+    plain, _k1 = served._job_struct(spec, opts)
+    cached, _k2 = served._job_struct(spec, opts, lsm, cache)
+    assert rows(plain) == rows(cached)
+    assert 3 in cache.rows  # the cache was used
+    other = served.Lsm()
+    again, _k3 = served._job_struct(spec, opts, other, cache)
+    assert rows(again) == rows(plain)
+    lsm.add_file(5, f("z0", "z9"))  # level 5 changed after the pick
+    stale, _k4 = served._job_struct(spec, opts, lsm, cache)
+    assert stale.num_deeper == 5 and rows(stale) == rows(plain)
+    assert np.frombuffer(rows(plain), dtype=served._LEVEL_FILE)["level"].tolist() == [3, 3, 4, 4, 5]
