@@ -224,6 +224,18 @@ void fill_u8(KTimer& T, u8* p, u8 v, u64 n);
 void iota_u32(KTimer& T, u32* p, u64 n);
 
 // k_decode.hip
+// k_stage.hip: src/dst on the host side are device addresses of pinned host
+// memory (hipHostGetDevicePointer).  stage_in copies `bytes` from src to dst
+// and, with init, the status; stage_out copies each segment (device src) to
+// dst + off.
+struct StageSeg {
+  const void* src;
+  u64 off;
+  u64 bytes;
+};
+void launch_stage_in(KTimer& T, const u8* src, u8* dst, u64 bytes, const DevStatus* init, DevStatus* st);
+void launch_stage_out(KTimer& T, u8* dst, StageSeg s0, StageSeg s1, StageSeg s2);
+
 void launch_files(KTimer& T, const FileDesc* files, u32 nfiles, FileInfo* info, u32* nblk, DevStatus* st);
 void launch_index(KTimer& T, const FileDesc* files, const FileInfo* info, const u32* blk_base, u32 nfiles,
                   u32 nb_cap, BlockDesc* blks, u32* blk_nent, DevStatus* st);

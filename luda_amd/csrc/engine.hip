@@ -77,22 +77,35 @@ struct DevBuf {
   }
 };
 
+// Pinned host memory; d is the address kernels use for it.
 struct PinnedBuf {
   u8* p = nullptr;
+  u8* d = nullptr;
   u64 cap = 0;
   hipError_t reserve(u64 need) {
     if (need <= cap) return hipSuccess;
-    if (p) hipHostFree(p);
-    p = nullptr;
-    cap = 0;
+    release();
     u64 c = std::max<u64>(need, 1 << 16);
     hipError_t e = hipHostMalloc(&p, c, hipHostMallocDefault);
-    if (e == hipSuccess) cap = c;
-    return e;
+    if (e != hipSuccess) {
+      p = nullptr;
+      return e;
+    }
+    void* dp = nullptr;
+    e = hipHostGetDevicePointer(&dp, p, 0);
+    if (e != hipSuccess || !dp) {
+      hipHostFree(p);
+      p = nullptr;
+      return e != hipSuccess ? e : hipErrorInvalidValue;
+    }
+    d = static_cast<u8*>(dp);
+    cap = c;
+    return hipSuccess;
   }
   void release() {
     if (p) hipHostFree(p);
     p = nullptr;
+    d = nullptr;
     cap = 0;
   }
 };
@@ -244,21 +257,27 @@ int status_code_of(const DevStatus& st) {
   return LUDA_OK;
 }
 
-void fill_status(luda_ctx* c, const DevStatus& init) {
-  memcpy(c->pin.p, &init, sizeof(DevStatus));
-  HIPCHK(hipMemcpyAsync(c->st, c->pin.p, sizeof(DevStatus), hipMemcpyHostToDevice, c->s));
+// Host bytes to device memory, and with init the status word too, in one
+// kernel that reads the pinned staging buffer pin2.  Waits only for the
+// previous kernel that read pin2 (usually long done), not for the stream.
+void stage_in(luda_ctx* c, void* dst, const void* src, u64 bytes, const DevStatus* init) {
+  HIPCHK(hipEventSynchronize(c->ev_pin2));
+  const u64 o_st = align_up(bytes, 64);
+  HIPCHK(c->pin2.reserve(o_st + sizeof(DevStatus)));
+  if (bytes) memcpy(c->pin2.p, src, bytes);
+  if (init) memcpy(c->pin2.p + o_st, init, sizeof(DevStatus));
+  launch_stage_in(c->T, c->pin2.d, static_cast<u8*>(dst), bytes,
+                  init ? reinterpret_cast<const DevStatus*>(c->pin2.d + o_st) : nullptr, c->st);
+  HIPCHK(hipEventRecord(c->ev_pin2, c->s));
 }
 
-// Copies host bytes to device memory through the pinned staging buffer.
-// Waits only for the previous copy out of the staging buffer (usually long
-// done), not for the whole stream.
-void upload(luda_ctx* c, void* dst, const void* src, u64 bytes) {
-  if (!bytes) return;
-  HIPCHK(hipEventSynchronize(c->ev_pin2));
-  HIPCHK(c->pin2.reserve(bytes));
-  memcpy(c->pin2.p, src, bytes);
-  HIPCHK(hipMemcpyAsync(dst, c->pin2.p, bytes, hipMemcpyHostToDevice, c->s));
-  HIPCHK(hipEventRecord(c->ev_pin2, c->s));
+// Device bytes (up to three segments) into the pinned buffer pin, at the
+// segments' offsets, in one kernel; the caller synchronises the stream before
+// reading pin.  pin must already hold every segment.
+void stage_out(luda_ctx* c, StageSeg a, StageSeg b = {}, StageSeg d = {}) {
+  for (const StageSeg& g : {a, b, d})
+    if (g.bytes && g.off + g.bytes > c->pin.cap) throw Fail{LUDA_IO_ERROR, "read-back staging too small"};
+  launch_stage_out(c->T, c->pin.d, a, b, d);
 }
 
 struct EncodeArrays {
@@ -678,8 +697,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
   }
   DevStatus st0;
   memset(&st0, 0, sizeof(st0));
-  fill_status(c, st0);
-  upload(c, d_files, fdesc.data(), nfiles * sizeof(FileDesc));
+  stage_in(c, d_files, fdesc.data(), nfiles * sizeof(FileDesc), &st0);
   KTimer& T = c->T;
   launch_files(T, d_files, nfiles, d_info, d_nblk, c->st);
   HIPCHK(scan_reserve(c->scans, std::max<u64>(nb_cap, nfiles + 1), c->s));
@@ -689,8 +707,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
   DevStatus stA;
   std::vector<u32> febase(nfiles + 1);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(c->pin.p, c->st, sizeof(DevStatus), hipMemcpyDeviceToHost, c->s));
-  HIPCHK(hipMemcpyAsync(c->pin.p + 1024, d_febase, (nfiles + 1) * 4, hipMemcpyDeviceToHost, c->s));
+  stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)}, StageSeg{d_febase, 1024, (nfiles + 1) * 4ull});
   HIPCHK(hipStreamSynchronize(c->s));
   memcpy(&stA, c->pin.p, sizeof(DevStatus));
   memcpy(febase.data(), c->pin.p + 1024, (nfiles + 1) * 4);
@@ -838,7 +855,6 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     void* out = c->take_out(std::max<u64>(out_cap, 1));
     if (!out) throw Fail{LUDA_IO_ERROR, "output allocation failed"};
     {
-      upload(c, d_tab, blob.data(), blob.size());
 
       dp.S = job->smallest_snapshot;
       dp.limit = 10 * job->options.max_file_size;  // MaxGrandParentOverlapBytes, db/version_set.cc:31-33
@@ -857,7 +873,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
       sb.err = 0;
       sb.site = 0;
       sb.max_ulen = 0;
-      fill_status(c, sb);
+      stage_in(c, d_tab, blob.data(), blob.size(), &sb);
       // decode, check, merge, drop, gather, stops
       HIPCHK(scan_reserve(c->scans,
                           std::max<u64>(std::max<u64>(n, es.m_cap), std::max<u64>(es.nbo_cap, std::max(es.nf_cap, es.nw_cap))) + 1,
@@ -894,9 +910,8 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     const u64 o_keys = align_up(o_meta + meta_bytes, 64);
     const u64 key_bytes = (u64)es.nf_cap * 2 * es.ep.kw;
     HIPCHK(c->pin.reserve(o_keys + key_bytes + 64));
-    HIPCHK(hipMemcpyAsync(c->pin.p, c->st, sizeof(DevStatus), hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipMemcpyAsync(c->pin.p + o_meta, EA.eb.meta, meta_bytes, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipMemcpyAsync(c->pin.p + o_keys, EA.eb.keys, key_bytes, hipMemcpyDeviceToHost, c->s));
+    stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)}, StageSeg{EA.eb.meta, o_meta, meta_bytes},
+              StageSeg{EA.eb.keys, o_keys, key_bytes});
     HIPCHK(hipStreamSynchronize(c->s));
     DevStatus stB;
     memcpy(&stB, c->pin.p, sizeof(DevStatus));
@@ -1036,7 +1051,7 @@ luda_result* build_tables(luda_ctx* c, const luda_options* opts, u64 n, const u8
     memset(&s0, 0, sizeof(s0));
     s0.n_entries = (u32)n;
     s0.n_kept = (u32)n;
-    fill_status(c, s0);
+    stage_in(c, nullptr, nullptr, 0, &s0);
     launch_ingest(c->T, n, d_keys, d_koff, d_vals, d_voff, EA.kept, c->st);
     run_encode(c, es, EA, (u8*)out);
     HIPCHK(hipGetLastError());
@@ -1047,9 +1062,8 @@ luda_result* build_tables(luda_ctx* c, const luda_options* opts, u64 n, const u8
     const u64 o_keys = align_up(o_meta + meta_bytes, 64);
     const u64 key_bytes = (u64)es.nf_cap * 2 * es.ep.kw;
     HIPCHK(c->pin.reserve(o_keys + key_bytes + 64));
-    HIPCHK(hipMemcpyAsync(c->pin.p, c->st, sizeof(DevStatus), hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipMemcpyAsync(c->pin.p + o_meta, EA.eb.meta, meta_bytes, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipMemcpyAsync(c->pin.p + o_keys, EA.eb.keys, key_bytes, hipMemcpyDeviceToHost, c->s));
+    stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)}, StageSeg{EA.eb.meta, o_meta, meta_bytes},
+              StageSeg{EA.eb.keys, o_keys, key_bytes});
     HIPCHK(hipStreamSynchronize(c->s));
     DevStatus stB;
     memcpy(&stB, c->pin.p, sizeof(DevStatus));
