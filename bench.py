@@ -308,7 +308,7 @@ def main():
                     help="instead: JOBS jobs of 4-256 MiB through the batched form (luda_submit) against the "
                          "plain path, one JSON line of aggregate GB/s and job latency")
     ap.add_argument("--mix-seed", type=int, default=3)
-    ap.add_argument("--slots", type=int, default=3)
+    ap.add_argument("--slots", type=int, default=6)
     ap.add_argument("--serve", type=int, default=0, metavar="PUTS",
                     help="instead: a fillrandom stream of PUTS puts through the served form against the plain "
                          "path at --rates multiples of the plain path's sustainable rate, one JSON line")

@@ -185,7 +185,7 @@ void luda_free_result(luda_ctx* ctx, luda_result* res);
  * inputs and `user` must stay valid until then.  Results are the same as
  * luda_compact's for the same job. */
 typedef void (*luda_done_fn)(void* user, int status, luda_result* result, const char* err);
-/* Worker slots (default 3); only before the first luda_submit. */
+/* Worker slots (default 6); only before the first luda_submit. */
 int luda_set_slots(luda_ctx* ctx, uint32_t slots);
 int luda_submit(luda_ctx* ctx, const luda_job* job, luda_done_fn done, void* user);
 /* Queues a memtable flush: the arguments of luda_build_tables, which must

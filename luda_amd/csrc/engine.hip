@@ -181,7 +181,7 @@ struct JobPool;
 struct luda_ctx {
   int device = 0;
   JobPool* pool = nullptr;  // worker slots of luda_submit, created on first use
-  u32 slots = 3;
+  u32 slots = 6;
   hipStream_t s = nullptr;
   u64 budget = 0;
   std::mutex mu;

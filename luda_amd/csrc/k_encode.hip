@@ -183,14 +183,9 @@ constexpr u32 kBnTile = kBnStarts + kBnSpan;     // staged entries
 constexpr u32 kBnPer = (kBnTile + 255) / 256;    // staged entries per thread
 constexpr u64 kCfOne = 1ull << 48;               // one forced entry in the packed prefix
 constexpr u64 kByteMask = kCfOne - 1;
-// LDS index of staged entry x in k_bnext: one spare slot after every 8, so
-// that the 64 lanes of a wave, whose starts lie 8 entries apart, read from
-// different banks.
-__device__ __forceinline__ constexpr u32 bn_pad(u32 x) { return x + (x >> 3); }
-
 __global__ __launch_bounds__(256) void k_bnext(Kept k, const u32* m_dev, EncParams ep, EncBufs b, DevStatus* st) {
-  __shared__ u64 sP[bn_pad(kBnTile) + 1];  // sP[bn_pad(x)]: sznr of staged entries [0, x), plus kCfOne per forced one
-  __shared__ u32 sD[bn_pad(kBnTile)];      // sD[bn_pad(x)]: szr - sznr, what a restart point adds (>= 0)
+  __shared__ u64 sP[kBnTile + 1];  // sP[x]: sznr of staged entries [0, x), plus kCfOne per forced one
+  __shared__ u32 sD[kBnTile];      // szr - sznr: what a restart point adds (>= 0)
   __shared__ u64 s_wsum[4];
   const u64 m = *m_dev;
   const u32 R = ep.interval;
@@ -219,8 +214,8 @@ __global__ __launch_bounds__(256) void k_bnext(Kept k, const u32* m_dev, EncPara
         const u32 x = 256 * i + tid;
         if (x < lim) {
           bad |= r[i] >= (1u << 31) || nr[i] >= (1u << 31) || r[i] < nr[i];
-          sD[bn_pad(x)] = r[i] - nr[i];
-          sP[bn_pad(x + 1)] = (u64)nr[i] + (fo[i] ? kCfOne : 0);
+          sD[x] = r[i] - nr[i];
+          sP[x + 1] = (u64)nr[i] + (fo[i] ? kCfOne : 0);
         }
       }
       if (bad) set_err(st, ERR_UNSUPPORTED, SITE_BLOCK_ENTRIES);
@@ -231,7 +226,7 @@ __global__ __launch_bounds__(256) void k_bnext(Kept k, const u32* m_dev, EncPara
 #pragma unroll
     for (u32 i = 0; i < kBnPer; i++) {
       const u32 x = kBnPer * tid + i;
-      v[i] = x < lim ? sP[bn_pad(x + 1)] : 0;
+      v[i] = x < lim ? sP[x + 1] : 0;
       tot += v[i];
     }
     const u64 incl = wave_incl_sum(tot);
@@ -239,29 +234,29 @@ __global__ __launch_bounds__(256) void k_bnext(Kept k, const u32* m_dev, EncPara
     __syncthreads();  // also: every thread has read its sizes before they are overwritten
     u64 run = incl - tot;
     for (u32 w = 0; w < wave; w++) run += s_wsum[w];
-    if (tid == 0) sP[bn_pad(0)] = 0;
+    if (tid == 0) sP[0] = 0;
 #pragma unroll
     for (u32 i = 0; i < kBnPer; i++) {
       const u32 x = kBnPer * tid + i;
       run += v[i];
-      if (x < lim) sP[bn_pad(x + 1)] = run;
+      if (x < lim) sP[x + 1] = run;
     }
     __syncthreads();
     u32 ub = 0;  // the upper bound of the previous start
     // entries per block at the tile's mean entry size: the first search's guess
-    const u64 tbytes = sP[bn_pad(lim)] & kByteMask;
+    const u64 tbytes = sP[lim] & kByteMask;
     const u32 est = tbytes ? (u32)min<u64>((u64)lim, (bs * lim) / tbytes) : 1;
     for (u32 s = 0; s < kBnPerThread; s++) {
       const u32 x = kBnPerThread * tid + s;
       const u64 q = q0 + x;
       if (x >= kBnStarts || q >= m) break;
-      const u64 px = sP[bn_pad(x)];
+      const u64 px = sP[x];
       const u64 bx = px & kByteMask;
       // smallest y in (x, lim] with bytes(x, y) + 8 >= bs, else lim + 1:
       // for the thread's first start from the tile's mean entry size, for
       // the others from the previous start's bound, stepping at most 16
       // entries before a binary search takes over
-      auto reaches = [&](u32 yy) { return (sP[bn_pad(yy)] & kByteMask) - bx + 8 >= bs; };
+      auto reaches = [&](u32 yy) { return (sP[yy] & kByteMask) - bx + 8 >= bs; };
       u32 y = s == 0 ? min(lim, x + 1 + est) : max(ub, x + 1);
       u32 lo = x + 1, hi = lim + 1;  // the answer lies in [lo, hi]
       bool done = false;
@@ -292,8 +287,8 @@ __global__ __launch_bounds__(256) void k_bnext(Kept k, const u32* m_dev, EncPara
       const u32 y0 = min(y, lim);
       const u32 g = (y0 - x + R - 1) / R;
       u64 sur = 0;
-      for (u32 t = 0; t < g; t++) sur += sD[bn_pad(x + t * R)];
-      u64 S = (sP[bn_pad(y0)] & kByteMask) - bx + sur + 4ull * g + 4;
+      for (u32 t = 0; t < g; t++) sur += sD[x + t * R];
+      u64 S = (sP[y0] & kByteMask) - bx + sur + 4ull * g + 4;
       u32 end;       // the cut, when it lies within the staged entries
       bool found = S >= bs;
       if (found) {
@@ -302,7 +297,7 @@ __global__ __launch_bounds__(256) void k_bnext(Kept k, const u32* m_dev, EncPara
         while (end > x + 1) {
           const u32 e = end - 1;  // the entry that would be dropped
           const bool rs = (e - x) % R == 0;
-          const u64 sz = (sP[bn_pad(e + 1)] & kByteMask) - (sP[bn_pad(e)] & kByteMask) + (rs ? (u64)sD[bn_pad(e)] + 4 : 0);
+          const u64 sz = (sP[e + 1] & kByteMask) - (sP[e] & kByteMask) + (rs ? (u64)sD[e] + 4 : 0);
           if (S - sz < bs) break;
           S -= sz;
           end = e;
@@ -311,19 +306,19 @@ __global__ __launch_bounds__(256) void k_bnext(Kept k, const u32* m_dev, EncPara
         end = lim;  // not within the staged entries
       }
       // a forced entry in (x, end) cuts first
-      const u64 cf0 = sP[bn_pad(x + 1)] >> 48;
-      if (end > x + 1 && (sP[bn_pad(end)] >> 48) > cf0) {
-        u32 lo = x + 1, hi = end - 1;  // smallest e in [x + 1, end) with forced[e]: sP[bn_pad(e + 1)] counts it
+      const u64 cf0 = sP[x + 1] >> 48;
+      if (end > x + 1 && (sP[end] >> 48) > cf0) {
+        u32 lo = x + 1, hi = end - 1;  // smallest e in [x + 1, end) with forced[e]: sP[e + 1] counts it
         while (lo < hi) {
           const u32 mid = (lo + hi) >> 1;
-          if ((sP[bn_pad(mid + 1)] >> 48) > cf0) hi = mid;
+          if ((sP[mid + 1] >> 48) > cf0) hi = mid;
           else lo = mid + 1;
         }
         end = lo;
         const u32 ge = (end - x + R - 1) / R;
         u64 se = 0;
-        for (u32 t = 0; t < ge; t++) se += sD[bn_pad(x + t * R)];
-        S = (sP[bn_pad(end)] & kByteMask) - bx + se + 4ull * ge + 4;
+        for (u32 t = 0; t < ge; t++) se += sD[x + t * R];
+        S = (sP[end] & kByteMask) - bx + se + 4ull * ge + 4;
         found = true;
       }
       u64 j = q0 + end;
