@@ -195,7 +195,7 @@ struct luda_ctx {
   u32 kw_hint = 0;   // key words that fit the keys seen so far
   KTimer T;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipEvent_t ev_pin2 = nullptr;  // recorded after the last copy out of pin2
+  bool pin2_busy = false;  // a staging kernel that reads pin2 may not have run yet
   std::multimap<u64, void*> out_pool;  // freed output buffers by size
 
   void* take_out(u64 bytes) {
@@ -257,18 +257,32 @@ int status_code_of(const DevStatus& st) {
   return LUDA_OK;
 }
 
+// Waits for the stream.  The waits of a job are short (its phase A, then its
+// phase B), and a blocking wait adds the thread's wake-up to each, so the
+// stream is polled for up to 2 ms before the wait blocks.
+void stream_sync(luda_ctx* c) {
+  const double t0 = now_ms();
+  hipError_t e;
+  while ((e = hipStreamQuery(c->s)) == hipErrorNotReady && now_ms() - t0 < 2.0) {
+  }
+  if (e == hipErrorNotReady) e = hipStreamSynchronize(c->s);
+  HIPCHK(e);
+  c->pin2_busy = false;
+}
+
 // Host bytes to device memory, and with init the status word too, in one
-// kernel that reads the pinned staging buffer pin2.  Waits only for the
-// previous kernel that read pin2 (usually long done), not for the stream.
+// kernel that reads the pinned staging buffer pin2.  Every job synchronises
+// its stream between two staging calls, so pin2 is free again by then; the
+// flag only guards against a caller that does not.
 void stage_in(luda_ctx* c, void* dst, const void* src, u64 bytes, const DevStatus* init) {
-  HIPCHK(hipEventSynchronize(c->ev_pin2));
+  if (c->pin2_busy) stream_sync(c);
   const u64 o_st = align_up(bytes, 64);
   HIPCHK(c->pin2.reserve(o_st + sizeof(DevStatus)));
   if (bytes) memcpy(c->pin2.p, src, bytes);
   if (init) memcpy(c->pin2.p + o_st, init, sizeof(DevStatus));
   launch_stage_in(c->T, c->pin2.d, static_cast<u8*>(dst), bytes,
                   init ? reinterpret_cast<const DevStatus*>(c->pin2.d + o_st) : nullptr, c->st);
-  HIPCHK(hipEventRecord(c->ev_pin2, c->s));
+  c->pin2_busy = true;
 }
 
 // Device bytes (up to three segments) into the pinned buffer pin, at the
@@ -568,7 +582,7 @@ luda_result* finish_result(luda_ctx* c, const EncodeSetup& s, const EncodeArrays
       throw Fail{LUDA_IO_ERROR, "pinned host allocation for the output failed"};
     }
     if (st.out_bytes) HIPCHK(hipMemcpyAsync(h, out, st.out_bytes, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
+    stream_sync(c);
     r->data = (uint8_t*)h;
     stats.d2h_ms = now_ms() - t;
   }
@@ -661,7 +675,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
       fdesc[i] = {c->in_buf.p + off, files[i]->size};
       off += align_up(files[i]->size + 16, 256);
     }
-    HIPCHK(hipStreamSynchronize(c->s));
+    stream_sync(c);
     stats.h2d_ms = now_ms() - t;
   }
   HIPCHK(hipEventRecord(c->ev0, c->s));
@@ -708,7 +722,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
   std::vector<u32> febase(nfiles + 1);
   HIPCHK(hipGetLastError());
   stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)}, StageSeg{d_febase, 1024, (nfiles + 1) * 4ull});
-  HIPCHK(hipStreamSynchronize(c->s));
+  stream_sync(c);
   memcpy(&stA, c->pin.p, sizeof(DevStatus));
   memcpy(febase.data(), c->pin.p + 1024, (nfiles + 1) * 4);
   if (status_code_of(stA) != LUDA_OK)
@@ -912,7 +926,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     HIPCHK(c->pin.reserve(o_keys + key_bytes + 64));
     stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)}, StageSeg{EA.eb.meta, o_meta, meta_bytes},
               StageSeg{EA.eb.keys, o_keys, key_bytes});
-    HIPCHK(hipStreamSynchronize(c->s));
+    stream_sync(c);
     DevStatus stB;
     memcpy(&stB, c->pin.p, sizeof(DevStatus));
     float dms = 0;
@@ -1025,7 +1039,7 @@ luda_result* build_tables(luda_ctx* c, const luda_options* opts, u64 n, const u8
   HIPCHK(hipMemcpyAsync(d_vals, vals, vbytes, hipMemcpyHostToDevice, c->s));
   HIPCHK(hipMemcpyAsync(d_koff, key_off, (n + 1) * 8, hipMemcpyHostToDevice, c->s));
   HIPCHK(hipMemcpyAsync(d_voff, val_off, (n + 1) * 8, hipMemcpyHostToDevice, c->s));
-  HIPCHK(hipStreamSynchronize(c->s));
+  stream_sync(c);
   stats.h2d_ms = now_ms() - t;
   stats.bytes_read = kbytes + vbytes;
   HIPCHK(hipEventRecord(c->ev0, c->s));
@@ -1064,7 +1078,7 @@ luda_result* build_tables(luda_ctx* c, const luda_options* opts, u64 n, const u8
     HIPCHK(c->pin.reserve(o_keys + key_bytes + 64));
     stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)}, StageSeg{EA.eb.meta, o_meta, meta_bytes},
               StageSeg{EA.eb.keys, o_keys, key_bytes});
-    HIPCHK(hipStreamSynchronize(c->s));
+    stream_sync(c);
     DevStatus stB;
     memcpy(&stB, c->pin.p, sizeof(DevStatus));
     float dms = 0;
@@ -1106,7 +1120,10 @@ int guarded(luda_ctx* c, char* err, size_t errlen, F fn) {
   } catch (const Fail& f) {
     if (err && errlen) snprintf(err, errlen, "%s", f.msg.c_str());
     // a failed call may leave work queued; drain it so the next call starts clean
-    if (c && c->s) hipStreamSynchronize(c->s);
+    if (c && c->s) {
+      hipStreamSynchronize(c->s);
+      c->pin2_busy = false;
+    }
     return f.code;
   }
 }
@@ -1348,7 +1365,6 @@ int ctx_open(int device, uint64_t hbm_budget_bytes, luda_ctx** out, char* err, s
     HIPCHK(hipMalloc(&c->st, sizeof(DevStatus)));
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));
-    HIPCHK(hipEventCreateWithFlags(&c->ev_pin2, hipEventDisableTiming));
     HIPCHK(c->pin.reserve(1 << 16));
     c->T.s = c->s;
   });
@@ -1374,7 +1390,6 @@ void ctx_close(luda_ctx* c) {
   if (c->st) hipFree(c->st);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
-  if (c->ev_pin2) hipEventDestroy(c->ev_pin2);
   if (c->s) hipStreamDestroy(c->s);
   delete c;
 }
