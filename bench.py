@@ -177,6 +177,9 @@ def run_mix(args):
         batch_lat.append(ms)
         same &= rc == ffi.LUDA_OK and [sha(x) for x in summary(o[2])] == plain_hash[i]
         beng.free(o[2])
+    cpu = None
+    if not args.no_cpu:
+        cpu = mix_cpu_baseline(eng, opts, mix, plain_hash)
     line = {
         "metric": "batched compaction input GB/s vs the plain path; p50/p99 job latency",
         "jobs": len(mix), "input_bytes": total, "slots": args.slots,
@@ -185,6 +188,7 @@ def run_mix(args):
         "plain_p50_ms": round(percentile(plain_lat, 50), 3), "plain_p99_ms": round(percentile(plain_lat, 99), 3),
         "batched_p50_ms": round(percentile(batch_lat, 50), 3), "batched_p99_ms": round(percentile(batch_lat, 99), 3),
         "same_tables": same,
+        "cpu_baseline": cpu,
         "workload": "log-uniform 4-256 MiB jobs: Ln->Ln+1 (16-byte keys, 100-byte values), bottommost "
                     "(Zipf 8-64-byte keys, 16-1024-byte values) and L0->L1 (up to 40 MiB), inputs in HBM",
     }
@@ -194,6 +198,54 @@ def run_mix(args):
         for res in keep:
             eng.free(res)
     eng.close()
+
+
+def mix_cpu_baseline(eng, opts, mix, gpu_hashes):
+    """The CPU compaction (the reference's own code when oracle/_ref was
+    built, else the C restatement) on the mix's jobs, from host copies of
+    their inputs: one thread over the jobs in order until 1 GiB of input
+    (their output hashes checked against the GPU's), then every job at once,
+    one job per thread over the host's cores."""
+    import oracle  # CPU baseline leg only
+    from concurrent.futures import ThreadPoolExecutor
+    kind = "reference" if oracle.ref_available() else "port"
+    which = "ref" if kind == "reference" else "oracle"
+    hjobs = []
+    for shape, b, job, keep, wl in mix:
+        host = ([], [])
+        for w in (0, 1):
+            for i in range(job.job.num_inputs[w]):
+                f = job.job.inputs[w][i]
+                host[w].append(eng.d2h(f.data, f.size))
+        hjobs.append((b, jobs.make_job(wl, opts, host)))
+
+    def run(k):
+        rc, tabs = oracle.compact(hjobs[k][1].job, which)
+        if rc != 0:
+            raise RuntimeError(f"CPU compaction of job {k} failed ({rc})")
+        return [sha(t.data) for t in tabs]
+    seq_bytes, seq_same, k = 0, True, 0
+    t0 = time.perf_counter()
+    while k < len(hjobs) and seq_bytes < (1 << 30):
+        seq_same &= run(k) == gpu_hashes[k]
+        seq_bytes += hjobs[k][0]
+        k += 1
+    t_seq = time.perf_counter() - t0
+    threads = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(run, range(len(hjobs))))
+    t_par = time.perf_counter() - t0
+    total = sum(b for b, _ in hjobs)
+    log(f"CPU ({kind}): {k} jobs one after another {seq_bytes / t_seq / 1e9:.3f} GB/s, "
+        f"all {len(hjobs)} on {threads} threads {total / t_par / 1e9:.3f} GB/s")
+    return {"kind": kind, "unit": "GB/s",
+            "sequential": {"value": round(seq_bytes / t_seq / 1e9, 4), "cores": 1, "jobs": k,
+                           "input_bytes": seq_bytes, "same_tables_as_gpu": seq_same},
+            "parallel": {"value": round(total / t_par / 1e9, 4), "cores": threads, "jobs": len(hjobs),
+                         "input_bytes": total},
+            "sample": "sequential: the first jobs of the mix up to 1 GiB of input, one thread; parallel: every "
+                      "job, one job per thread; inputs and outputs in RAM"}
 
 
 def release_tree(lsm):
