@@ -280,8 +280,9 @@ def run_serve(args):
     # warm-up: the whole stream through both paths, so that every working
     # buffer and output buffer is allocated before anything is timed
     for be, run in ((pbe, served.run_plain), (sbe, served.run_served)):
-        lsm, _ = run(be, mems, 0.0)
+        lsm, wst = run(be, mems, 0.0)
         release_tree(lsm)
+        log(f"warm-up {run.__name__}: {wst.summary()['jobs']} jobs in {wst.summary()['seconds']} s")
     pbe.acct = dict.fromkeys(pbe.acct, 0)
     lsm, st = served.run_plain(pbe, mems, 0.0)
     cal = st.summary()

@@ -255,11 +255,12 @@ u32 merge_items_per_tile(u32 W);  // outputs per merge tile for key planes of W 
 void launch_check_sorted(KTimer& T, Planes K, u64 n, const u32* child_start, u32 nchild, u32* iota, DevStatus* st);
 
 // k_select.hip
-// events[0]: the grandparent-crossing count that the keep scan appends to
+// events[0..ngp]: the grandparent-crossing slots, cleared here and filled by
+// the keep scan (events[g] = the entry where the grandparent count reaches g)
 void launch_drop(KTimer& T, const u32* perm, u32 n, Planes K, const DropParams& dp, u32* keep, u32* gcnt, u32* events,
                  DevStatus* st);
-// kpos = exclusive scan of keep, n_kept; with grandparents also the crossings
-// (events[1..]) that launch_stops walks
+// kpos = exclusive scan of keep, n_kept; with grandparents also the crossing
+// slots (events[1..ngp]) that launch_stops walks
 void scan_keep(KTimer& T, ScanState& S, const u32* keep, u32* kpos, u32 n, const DropParams& dp, const u32* gcnt,
                u32* events, u32 ev_cap, DevStatus* st);
 void launch_gather(KTimer& T, const u32* perm, u32 n, const u32* keep, const u32* kpos, Planes K, const u64* vptr,

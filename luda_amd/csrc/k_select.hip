@@ -42,8 +42,11 @@ __device__ bool is_base_level_for_key(const Planes& K, u64 e, const DropParams& 
   return true;
 }
 
-__global__ void k_drop(const u32* perm, u32 n, Planes K, DropParams dp, u32* keep, u32* gcnt, u32* nev) {
-  if (dp.ngp && blockIdx.x == 0 && threadIdx.x == 0) *nev = 0;  // the crossing count of the keep scan
+// slot[g] of the grandparent crossings (g = 1..ngp): no crossing yet
+constexpr u32 kNoEvent = 0xffffffffu;
+
+__global__ void k_drop(const u32* perm, u32 n, Planes K, DropParams dp, u32* keep, u32* gcnt, u32* slot) {
+  if (dp.ngp) GRID_STRIDE(g, (u64)dp.ngp + 1) slot[g] = kNoEvent;  // filled by the keep scan
   GRID_STRIDE(p, n) {
     const u32 e = perm[p];
     const u64 tag = K.tag[e];
@@ -87,90 +90,73 @@ __global__ void k_gather(const u32* perm, u32 n, const u32* keep, const u32* kpo
   }
 }
 
-// The keep scan's epilogue: positions p >= 1 where the grandparent count
-// grows (at most ngp of them), in any order.
+// The keep scan's epilogue: the positions p >= 1 where the grandparent count
+// grows.  gcnt only grows along the merged order, so each such p has a count
+// of its own and goes to slot[gcnt[p]]: the slots list the crossings in order.
 struct StopEventsEpi {
   const u32* gcnt;
-  u32* events;
-  u32* nev;
-  u32 cap;
+  u32* slot;
   struct Pre {
     bool cross;
   };
-  __device__ Pre pre(u64 p, u32) const { return {events && p > 0 && gcnt[p] > gcnt[p - 1]}; }
+  __device__ Pre pre(u64 p, u32) const { return {slot && p > 0 && gcnt[p] > gcnt[p - 1]}; }
   __device__ void item(u64 p, u32, u32, const Pre& c) const {
-    if (!c.cross) return;
-    const u32 i = atomicAdd(nev, 1u);
-    if (i < cap) events[i] = (u32)p;
+    if (c.cross) slot[gcnt[p]] = (u32)p;
   }
   __device__ void done(u32, u64) const {}
 };
 
 constexpr u32 kStopThreads = 1024;
-constexpr u32 kMaxEvents = 2048;
+constexpr u32 kStopChunk = 2048;  // crossing slots per pass through LDS
+constexpr u32 kNoCut = 0xffffffffu, kEmpty = 0xfffffffeu;
 
-// One workgroup: sort the crossings, gather what each adds to the overlap
-// budget, walk them in order as ShouldStopBefore would, and mark the kept
-// entry each cut lands on.
+// One workgroup walks the crossing slots in order, kStopChunk at a time: all
+// threads gather what each crossing adds to the overlap budget and where it
+// would cut, thread 0 walks them as ShouldStopBefore would, and the cuts
+// that fire mark the kept entry they land on.
 __global__ __launch_bounds__(kStopThreads) void k_stop_apply(const u32* gcnt, const u32* kpos, u32 n,
-                                                              DropParams dp, const u32* events, const u32* nev_dev,
-                                                              Kept out, DevStatus* st) {
-  __shared__ u32 ev[kMaxEvents];
-  __shared__ u64 add[kMaxEvents];
-  __shared__ u32 cut[kMaxEvents];
-  if (*nev_dev > kMaxEvents) {
-    if (threadIdx.x == 0) set_err(st, ERR_UNSUPPORTED, SITE_CAPACITY);
-    return;
-  }
-  const u32 nev = *nev_dev;
+                                                              DropParams dp, const u32* slot, Kept out,
+                                                              DevStatus* st) {
+  __shared__ u64 add[kStopChunk];
+  __shared__ u32 cut[kStopChunk];
   const u32 nkept = st->n_kept;
-  u32 np2 = 1;
-  while (np2 < nev) np2 <<= 1;
-  for (u32 i = threadIdx.x; i < np2; i += blockDim.x) ev[i] = i < nev ? events[i] : 0xffffffffu;
-  __syncthreads();
-  // bitonic sort of the event positions
-  for (u32 k = 2; k <= np2; k <<= 1)
-    for (u32 j = k >> 1; j > 0; j >>= 1) {
-      for (u32 i = threadIdx.x; i < np2; i += blockDim.x) {
-        const u32 ixj = i ^ j;
-        if (ixj > i) {
-          const u32 a = ev[i], b = ev[ixj];
-          const bool up = (i & k) == 0;
-          if ((a > b) == up) {
-            ev[i] = b;
-            ev[ixj] = a;
-          }
+  // Compaction::ShouldStopBefore: overlapped_bytes_ grows by every
+  // grandparent passed after the first key and resets when a cut fires.
+  u64 over = 0;
+  u32 nstops = 0;
+  for (u32 g0 = 1; g0 <= dp.ngp; g0 += kStopChunk) {
+    const u32 cnt = min(kStopChunk, dp.ngp + 1 - g0);
+    for (u32 i = threadIdx.x; i < cnt; i += blockDim.x) {
+      const u32 p = slot[g0 + i];
+      if (p == kNoEvent || p == 0 || p >= n) {
+        add[i] = 0;
+        cut[i] = kEmpty;
+      } else {
+        add[i] = dp.gp_prefix[gcnt[p]] - dp.gp_prefix[gcnt[p - 1]];
+        cut[i] = kpos[p];
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (u32 i = 0; i < cnt; i++) {
+        if (cut[i] == kEmpty) continue;
+        over += add[i];
+        if (over > dp.limit) {
+          over = 0;
+          nstops++;
+        } else {
+          cut[i] = kNoCut;
         }
       }
-      __syncthreads();
     }
-  for (u32 i = threadIdx.x; i < nev; i += blockDim.x) {
-    const u32 p = ev[i];
-    add[i] = dp.gp_prefix[gcnt[p]] - dp.gp_prefix[gcnt[p - 1]];
-    cut[i] = kpos[p];
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    // Compaction::ShouldStopBefore: overlapped_bytes_ grows by every
-    // grandparent passed after the first key and resets when a cut fires.
-    u64 over = 0;
-    u32 nstops = 0;
-    for (u32 i = 0; i < nev; i++) {
-      over += add[i];
-      if (over > dp.limit) {
-        over = 0;
-        nstops++;
-      } else {
-        cut[i] = 0xffffffffu;
-      }
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < cnt; i += blockDim.x) {
+      const u32 q = cut[i];
+      if (q != kNoCut && q != kEmpty && q > 0 && q < nkept) out.forced[q] = 1;
     }
-    st->n_stops = nstops;
+    __syncthreads();
   }
-  __syncthreads();
-  for (u32 i = threadIdx.x; i < nev; i += blockDim.x) {
-    const u32 q = cut[i];
-    if (q != 0xffffffffu && q > 0 && q < nkept) out.forced[q] = 1;
-  }
+  if (threadIdx.x == 0) st->n_stops = nstops;
 }
 
 u32 grid_for(u64 n, u32 per_block) {
@@ -191,9 +177,10 @@ void launch_drop(KTimer& T, const u32* perm, u32 n, Planes K, const DropParams& 
 
 void scan_keep(KTimer& T, ScanState& S, const u32* keep, u32* kpos, u32 n, const DropParams& dp, const u32* gcnt,
                u32* events, u32 ev_cap, DevStatus* st) {
+  (void)ev_cap;
   const bool stops = n && dp.ngp;
   lbscan::run<u32, u32>(T, "scan_keep", S, lbscan::ArrLoad<u32>{keep}, kpos, nullptr, n, n, &st->n_kept, st, nullptr,
-                        StopEventsEpi{gcnt, stops ? events + 1 : nullptr, events, ev_cap});
+                        StopEventsEpi{gcnt, stops ? events : nullptr});
 }
 
 void launch_gather(KTimer& T, const u32* perm, u32 n, const u32* keep, const u32* kpos, Planes K, const u64* vptr,
@@ -205,8 +192,8 @@ void launch_gather(KTimer& T, const u32* perm, u32 n, const u32* keep, const u32
 void launch_stops(KTimer& T, const u32* gcnt, const u32* kpos, const u32* keep, u32 n, const DropParams& dp,
                   u32* events, u32 ev_cap, Kept out, DevStatus* st) {
   (void)keep;
+  (void)ev_cap;
   if (!n || !dp.ngp) return;
-  // events[0] is the counter (zeroed by the drop kernel), events + 1 the list (filled by the keep scan).
-  KLAUNCH(T, "stop_apply", k_stop_apply, 1, kStopThreads, 0, gcnt, kpos, n, dp, (const u32*)(events + 1),
-          (const u32*)events, out, st);
+  // events[g], g = 0..ngp: the crossing slots (cleared by the drop kernel, filled by the keep scan).
+  KLAUNCH(T, "stop_apply", k_stop_apply, 1, kStopThreads, 0, gcnt, kpos, n, dp, (const u32*)events, out, st);
 }

@@ -231,17 +231,24 @@ class Engine:
             self._next_id = 1
 
             def trampoline(user, status, res, err):
-                cb, jb, kod = self._pending.pop(user)
-                if status != ffi.LUDA_OK:
-                    cb(status, None, (err or b"").decode())
-                    return
+                # an exception cannot cross the C frame of luda_poll/luda_drain:
+                # keep the first one and raise it when that call returns
                 try:
-                    stats = res.contents.stats
-                    tables = [] if kod else result_tables(res)
-                    cb(status, (tables, _stats_dict(stats), res), "")
-                finally:
-                    if not kod:
-                        self.lib.luda_free_result(self.ctx, res)
+                    cb, jb, kod = self._pending.pop(user)
+                    if status != ffi.LUDA_OK:
+                        cb(status, None, (err or b"").decode())
+                        return
+                    try:
+                        stats = res.contents.stats
+                        tables = [] if kod else result_tables(res)
+                        cb(status, (tables, _stats_dict(stats), res), "")
+                    finally:
+                        if not kod:
+                            self.lib.luda_free_result(self.ctx, res)
+                except BaseException as e:  # noqa: BLE001
+                    if self._cb_error is None:
+                        self._cb_error = e
+            self._cb_error = None
             self._done_fn = ffi.DONE_FN(trampoline)
         uid = self._next_id
         self._next_id += 1
@@ -252,11 +259,21 @@ class Engine:
             raise RuntimeError(f"luda_submit failed ({rc})")
         return uid
 
+    def _reraise(self):
+        e = getattr(self, "_cb_error", None)
+        if e is not None:
+            self._cb_error = None
+            raise e
+
     def poll(self):
-        return self.lib.luda_poll(self.ctx)
+        n = self.lib.luda_poll(self.ctx)
+        self._reraise()
+        return n
 
     def drain(self):
-        return self.lib.luda_drain(self.ctx)
+        n = self.lib.luda_drain(self.ctx)
+        self._reraise()
+        return n
 
     def builder(self):
         """A table builder for build_group_tables backed by the engine."""

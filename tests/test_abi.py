@@ -46,3 +46,17 @@ def test_init_fails_cleanly_without_device():
     err = C.create_string_buffer(256)
     rc = lib.luda_init(99, 0, C.byref(ctx), err, 256)
     assert rc != ffi.LUDA_OK and not ctx.value
+
+
+def test_queued_entry_points_reject_a_missing_context():
+    lib = ffi.engine()
+    opts = ffi.default_options()
+    assert lib.luda_submit(None, None, None, None) == ffi.LUDA_INVALID_ARGUMENT
+    assert lib.luda_submit_build(None, C.byref(opts), 0, None, None, None, None, 0, None,
+                                 None) == ffi.LUDA_INVALID_ARGUMENT
+    assert lib.luda_poll(None) == 0 and lib.luda_drain(None) == 0
+
+
+def test_stats_layout():
+    # luda_stats ends with end_clock_ms (include/luda.h)
+    assert ffi.luda_stats.end_clock_ms.offset == C.sizeof(ffi.luda_stats) - 8

@@ -11,7 +11,7 @@ import pytest
 
 import oracle
 from luda_amd import ffi, jobs
-from tests.tiny_jobs import TinyJob, ikey
+from tests.tiny_jobs import TinyJob, ikey, many_grandparents_job
 
 pytestmark = pytest.mark.gpu
 
@@ -171,3 +171,15 @@ def test_golden_jobs_on_engine(eng, rec):
     got = [{"sha256": sha(t.data), "file_size": len(t.data), "num_entries": t.num_entries,
             "smallest": t.smallest.hex(), "largest": t.largest.hex()} for t in out[0]]
     assert got == rec["outputs"]
+
+
+@pytest.mark.parametrize("ngp", [5, 2047, 2049, 5000])
+def test_many_grandparents_match_oracle(eng, ngp):
+    print("grandparents", ngp, flush=True)
+    tj = many_grandparents_job(ngp)
+    job, keep = tj.make_job(lambda e, o: [t.data for t in oracle.build_tables(e, o, "oracle")])
+    rc1, want = oracle.compact(job, "oracle")
+    rc2, out, msg = eng.compact(job)
+    assert rc1 == 0 and rc2 == 0, msg
+    assert len(want) > 10
+    assert summary(out[0]) == summary(want)

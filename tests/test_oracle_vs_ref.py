@@ -4,7 +4,7 @@ compaction loop.  Skipped where the reference could not be built."""
 import pytest
 
 import oracle
-from tests.tiny_jobs import TinyJob
+from tests.tiny_jobs import TinyJob, many_grandparents_job
 
 pytestmark = pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built (no reference tree)")
 
@@ -30,3 +30,16 @@ def test_tiny_jobs_oracle_equals_reference(block):
         assert rc1 == rc2 == 0, seed
         assert [(t.data, t.smallest, t.largest, t.num_entries) for t in o1] == \
                [(t.data, t.smallest, t.largest, t.num_entries) for t in o2], seed
+
+
+@pytest.mark.parametrize("ngp", [5, 2049, 5000])
+def test_many_grandparents_oracle_equals_reference(ngp):
+    """ShouldStopBefore over thousands of small grandparents (cuts every few)."""
+    tj = many_grandparents_job(ngp)
+    job, keep = tj.make_job(build_with("ref"))
+    rc1, o1 = oracle.compact(job, "oracle")
+    rc2, o2 = oracle.compact(job, "ref")
+    assert rc1 == rc2 == 0
+    assert len(o1) > 10
+    assert [(t.data, t.smallest, t.largest, t.num_entries) for t in o1] == \
+           [(t.data, t.smallest, t.largest, t.num_entries) for t in o2]

@@ -139,3 +139,25 @@ class TinyJob:
         job.deeper = C.cast(dp, C.POINTER(ffi.luda_level_file))
         job.num_deeper = len(self.deeper)
         return job, keep
+
+
+def many_grandparents_job(ngp, seed=5):
+    """A level-1 job over 6000 user keys whose grandparents (ngp of them, small
+    files) interleave the keys, so ShouldStopBefore fires every few of them
+    and the crossings outnumber one LDS pass of the cut walk."""
+    rng = random.Random(seed)
+    users = [b"k%06d" % i for i in range(6000)]
+    seq = 0
+    a, b = [], []
+    for u in users:
+        seq += 1
+        (a if rng.random() < 0.3 else b).append((ikey(u, seq, 1), bytes(rng.randrange(256) for _ in range(20))))
+    tj = TinyJob(0)
+    tj.level = 1
+    tj.opts = ffi.default_options(max_file_size=4096)
+    tj.files = ([a], [b])
+    tj.snapshot = seq
+    picks = sorted(rng.sample(range(6000), ngp))
+    tj.grandparents = [(ikey(users[i] + b"~", 7, 1), rng.choice([900, 2500, 6000])) for i in picks]
+    tj.deeper = []
+    return tj
