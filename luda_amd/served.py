@@ -134,6 +134,40 @@ def write_stream(puts, seed, key_space=10 ** 9, entries_per_flush=None):
 # ---------------------------------------------------------------------------
 # the tree
 
+_LEVEL_FILE = np.dtype([("sd", "<u8"), ("ss", "<u8"), ("ld", "<u8"), ("ls", "<u8"), ("level", "<i4"),
+                        ("reserved", "<u4")])
+assert _LEVEL_FILE.itemsize == C.sizeof(ffi.luda_level_file)
+
+
+def _cbufs(f):
+    """C copies of a file's smallest/largest user keys and largest internal
+    key, made once per file."""
+    cb = getattr(f, "_cb", None)
+    if cb is None:
+        cb = (C.create_string_buffer(ukey(f.smallest), len(f.smallest) - 8),
+              C.create_string_buffer(ukey(f.largest), len(f.largest) - 8),
+              C.create_string_buffer(f.largest, len(f.largest)))
+        f._cb = cb
+    return cb
+
+
+def _level_row(f, level):
+    """f's luda_level_file as a record (f keeps the key buffers alive)."""
+    b = _cbufs(f)
+    return np.array([(C.addressof(b[0]), len(f.smallest) - 8, C.addressof(b[1]), len(f.largest) - 8, level, 0)],
+                    dtype=_LEVEL_FILE)
+
+
+def _row_bytes(f, level):
+    """_level_row's bytes, kept with the file while it stays on one level."""
+    rb = getattr(f, "_rb", None)
+    if rb is None or rb[0] != level:
+        rb = (level, _level_row(f, level).tobytes())
+        f._rb = rb
+    return rb[1]
+
+
+
 @dataclass(eq=False)
 class File:
     """FileMetaData (db/version_edit.h:18-27) plus where its bytes are: a
@@ -154,12 +188,16 @@ class JobSpec:
     level: int
     inputs: tuple            # ([File] of level, [File] of level + 1)
     grandparents: list       # [File] of level + 2 overlapping the inputs
-    deeper: list             # [(level, File)] for levels >= level + 2
+    deeper_files: list       # [(level, [File])]: files of levels >= level + 2 that overlap the job
     smallest_snapshot: int
     arrival: float = 0.0
     bytes_in: int = 0
-    src: object = None        # the Lsm it was picked from
-    versions: tuple = ()      # that Lsm's level versions at the time
+    deeper_rows: object = None  # the same files as luda_level_file rows (a numpy record array)
+
+    @property
+    def deeper(self):
+        """[(level, File)] in level order."""
+        return [(d, f) for d, fs in self.deeper_files for f in fs]
 
 
 class Owner:
@@ -189,23 +227,30 @@ class Lsm:
         self.last_seq = 0
         self.max_file_size = max_file_size
         self.moves = 0
-        self.version = [0] * NUM_LEVELS   # bumped on every change of a level
         # per level, kept in step with levels[]: smallest and largest user
         # keys in level order, and the level's bytes
         self._sk = [[] for _ in range(NUM_LEVELS)]
         self._lk = [[] for _ in range(NUM_LEVELS)]
         self._bytes = [0] * NUM_LEVELS
+        # and the levels' luda_level_file rows for the deeper-level ranges of
+        # jobs, rebuilt when a changed level is next asked for
+        self._rows = [np.zeros(0, dtype=_LEVEL_FILE) for _ in range(NUM_LEVELS)]
+        self._dirty = set()
 
-    def _level_index(self, level):
-        return None, self._sk[level], self._lk[level], self._bytes[level]
+    def level_rows(self, level):
+        if level in self._dirty:
+            self._rows[level] = np.frombuffer(b"".join(_row_bytes(f, level) for f in self.levels[level]),
+                                              dtype=_LEVEL_FILE)
+            self._dirty.discard(level)
+        return self._rows[level]
 
     def _remove(self, level, f):
         i = self.levels[level].index(f)
         del self.levels[level][i]
         del self._sk[level][i]
         del self._lk[level][i]
+        self._dirty.add(level)
         self._bytes[level] -= f.size
-        self.version[level] += 1
 
     def new_number(self):
         self.next_number += 1
@@ -215,12 +260,8 @@ class Lsm:
     def overlapping(self, level, begin, end):
         files = self.levels[level]
         if level > 0:
-            # sorted, and disjoint but for a shared boundary user key: the
-            # overlapping files are one run, found by binary search
-            _, sk, lk, _ = self._level_index(level)
-            i = 0 if begin is None else bisect.bisect_left(lk, begin)
-            j = len(files) if end is None else bisect.bisect_right(sk, end)
-            return files[i:max(i, j)]
+            i, j = self.overlap_range(level, begin, end)
+            return files[i:j]
         out = []
         i = 0
         while i < len(files):
@@ -238,6 +279,14 @@ class Lsm:
                     end, out, i = fl, [], 0
         return out
 
+    def overlap_range(self, level, begin, end):
+        """Level >= 1 is sorted, and disjoint but for a shared boundary user
+        key: the files overlapping [begin, end] are one run [i, j)."""
+        sk, lk = self._sk[level], self._lk[level]
+        i = 0 if begin is None else bisect.bisect_left(lk, begin)
+        j = len(sk) if end is None else bisect.bisect_right(sk, end)
+        return i, max(i, j)
+
     @staticmethod
     def key_range(files):
         lo = min((f.smallest for f in files), key=ikey_order)
@@ -245,7 +294,7 @@ class Lsm:
         return lo, hi
 
     def level_bytes(self, level):
-        return self._level_index(level)[3]
+        return self._bytes[level]
 
     def score(self, level):
         if level == 0:
@@ -265,7 +314,7 @@ class Lsm:
         if level > 0:
             # the smallest file past `largest` that starts on its user key is
             # the next file in level order
-            _, sk, lk, _ = self._level_index(level)
+            sk = self._sk[level]
             j = bisect.bisect_left(sk, ukey(largest))
             while j < len(files):
                 f = files[j]
@@ -332,10 +381,19 @@ class Lsm:
             self.add_file(L + 1, f)
             self.moves += 1
             return f
-        deeper = [(d, f) for d in range(L + 2, NUM_LEVELS) for f in self.levels[d]]
+        # IsBaseLevelForKey (db/version_set.cc:1535-1554) is only asked about
+        # the job's own keys, so deeper files outside the job's key range can
+        # never answer it: they are left out
+        dfiles, drows = [], []
+        for d in range(L + 2, NUM_LEVELS):
+            i, j = self.overlap_range(d, ukey(alo), ukey(ahi))
+            if j > i:
+                dfiles.append((d, self.levels[d][i:j]))
+                drows.append(self.level_rows(d)[i:j])
         order0 = sorted(inputs0, key=lambda f: ikey_order(f.smallest))
         order1 = sorted(inputs1, key=lambda f: ikey_order(f.smallest))
-        spec = JobSpec(L, (order0, order1), gps, deeper, self.last_seq, src=self, versions=tuple(self.version))
+        spec = JobSpec(L, (order0, order1), gps, dfiles, self.last_seq,
+                       deeper_rows=np.concatenate(drows) if drows else np.zeros(0, dtype=_LEVEL_FILE))
         spec.bytes_in = sum(f.size for f in order0 + order1)
         self.busy.update((L, L + 1))
         return spec
@@ -359,7 +417,7 @@ class Lsm:
     def add_file(self, level, f):
         files = self.levels[level]
         if level > 0:
-            _, sk, _, _ = self._level_index(level)
+            sk = self._sk[level]
             i = bisect.bisect_left(sk, ukey(f.smallest))
             while i < len(files) and ikey_order(files[i].smallest) <= ikey_order(f.smallest):
                 i += 1
@@ -369,8 +427,8 @@ class Lsm:
         files.insert(i, f)
         self._sk[level].insert(i, ukey(f.smallest))
         self._lk[level].insert(i, ukey(f.largest))
+        self._dirty.add(level)
         self._bytes[level] += f.size
-        self.version[level] += 1
 
     def install(self, spec, outputs):
         """InstallCompactionResults (db/db_impl.cc:1040-1056)."""
@@ -399,50 +457,8 @@ class Lsm:
 # ---------------------------------------------------------------------------
 # the engine as the backend: tables in HBM
 
-_LEVEL_FILE = np.dtype([("sd", "<u8"), ("ss", "<u8"), ("ld", "<u8"), ("ls", "<u8"), ("level", "<i4"),
-                        ("reserved", "<u4")])
-assert _LEVEL_FILE.itemsize == C.sizeof(ffi.luda_level_file)
-
-
-def _cbufs(f):
-    """C copies of a file's smallest/largest user keys and largest internal
-    key, made once per file."""
-    cb = getattr(f, "_cb", None)
-    if cb is None:
-        cb = (C.create_string_buffer(ukey(f.smallest), len(f.smallest) - 8),
-              C.create_string_buffer(ukey(f.largest), len(f.largest) - 8),
-              C.create_string_buffer(f.largest, len(f.largest)))
-        f._cb = cb
-    return cb
-
-
-class _DeeperCache:
-    """luda_level_file rows of each level, rebuilt only when the level changed."""
-
-    def __init__(self):
-        self.rows = {}
-
-    def level_rows(self, lsm, level):
-        hit = self.rows.get(level)
-        if hit is not None and hit[0] == lsm.version[level] and hit[2] is lsm:
-            return hit[1]
-        files = lsm.levels[level]
-        a = np.zeros(len(files), dtype=_LEVEL_FILE)
-        if files:
-            bufs = [_cbufs(f) for f in files]
-            a["sd"] = [C.addressof(b[0]) for b in bufs]
-            a["ss"] = [len(f.smallest) - 8 for f in files]
-            a["ld"] = [C.addressof(b[1]) for b in bufs]
-            a["ls"] = [len(f.largest) - 8 for f in files]
-            a["level"] = level
-        self.rows[level] = (lsm.version[level], a, lsm)
-        return a
-
-
-def _job_struct(spec, opts, lsm=None, cache=None):
-    """luda_job for a JobSpec whose file handles are device pointers.  With an
-    Lsm and a _DeeperCache the deeper-level rows come from the cache (the
-    JobSpec's deeper list is the same files)."""
+def _job_struct(spec, opts):
+    """luda_job for a JobSpec whose file handles are device pointers."""
     keep = []
     job = ffi.luda_job()
     job.level = spec.level
@@ -469,18 +485,13 @@ def _job_struct(spec, opts, lsm=None, cache=None):
     keep.append(gp)
     job.grandparents = C.cast(gp, C.POINTER(ffi.luda_grandparent))
     job.num_grandparents = len(spec.grandparents)
-    if (lsm is not None and cache is not None and spec.src is lsm
-            and all(lsm.version[L] == spec.versions[L] for L in range(spec.level + 2, NUM_LEVELS))):
-        parts = [cache.level_rows(lsm, L) for L in range(spec.level + 2, NUM_LEVELS)]
-        rows = np.concatenate(parts) if parts else np.zeros(0, dtype=_LEVEL_FILE)
-        assert len(rows) == len(spec.deeper)
-        keep.append([f for _, f in spec.deeper])
+    if spec.deeper_rows is not None:
+        rows = spec.deeper_rows
     else:
         rows = np.zeros(len(spec.deeper), dtype=_LEVEL_FILE)
         for i, (lvl, f) in enumerate(spec.deeper):
-            b = _cbufs(f)
-            rows[i] = (C.addressof(b[0]), len(f.smallest) - 8, C.addressof(b[1]), len(f.largest) - 8, lvl, 0)
-        keep.append([f for _, f in spec.deeper])
+            rows[i] = _level_row(f, lvl)
+    keep.append(spec.deeper_files)  # the key buffers the rows point into
     rows = np.ascontiguousarray(rows) if len(rows) else np.zeros(1, dtype=_LEVEL_FILE)
     keep.append(rows)
     job.deeper = C.cast(C.c_void_p(rows.ctypes.data), C.POINTER(ffi.luda_level_file))
@@ -497,7 +508,6 @@ class EngineBackend:
         self.flush_opts = ffi.luda_options()
         C.pointer(self.flush_opts)[0] = opts
         self.flush_opts.max_file_size = (1 << 64) - 1  # BuildTable writes one table
-        self.deeper = _DeeperCache()
         self.acct = {"jobs": 0, "device_ms": 0.0, "call_ms": 0.0}  # summed over the engine's calls
 
     def _account(self, stats):
@@ -536,7 +546,7 @@ class EngineBackend:
         return self._files(lsm, out[2]), out[1]["end_clock_ms"]
 
     def compact(self, lsm, spec):
-        job, keep = _job_struct(spec, self.opts, lsm, self.deeper)
+        job, keep = _job_struct(spec, self.opts)
         rc, out, msg = self.eng.compact(job, keep_on_device=True)
         if rc != ffi.LUDA_OK:
             raise RuntimeError(f"luda_compact failed ({rc}): {msg}")
@@ -557,7 +567,7 @@ class EngineBackend:
                               cb, keep_on_device=True)
 
     def submit_compact(self, lsm, spec, done):
-        job, keep = _job_struct(spec, self.opts, lsm, self.deeper)
+        job, keep = _job_struct(spec, self.opts)
         spec._job = (job, keep)
 
         def cb(rc, out, msg):

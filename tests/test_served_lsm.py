@@ -89,10 +89,11 @@ def test_picker_rules():
     assert lsm2.levels[2] == [g] and lsm2.moves == 1
 
 
-def test_cached_deeper_rows_match_the_job():
-    """The deeper-level rows an EngineBackend takes from its per-level cache
-    are the rows of the job's own deeper list, and the cache is not used for a
-    job picked from another tree or before a deeper level changed."""
+def test_deeper_rows_match_the_job():
+    """A job carries, as luda_level_file rows, exactly the files of levels >=
+    level + 2 that overlap its key range (IsBaseLevelForKey is only asked
+    about the job's keys), the same rows _job_struct builds from the files,
+    and keeps them when a deeper level changes after the pick."""
     import ctypes as C
     import numpy as np
     lsm = served.Lsm(max_file_size=1 << 20, level1_bytes=1000)
@@ -100,27 +101,27 @@ def test_cached_deeper_rows_match_the_job():
     def f(lo, hi, size=600, seq=1):
         tag = ((seq << 8) | 1).to_bytes(8, "little")
         return served.File(lsm.new_number(), size, lo.encode() + tag, hi.encode() + tag, 1, 0)
-    lsm.add_file(1, f("a", "c"))
-    lsm.add_file(1, f("d", "f"))
-    lsm.add_file(2, f("b", "e"))
-    for i, k in enumerate("ghijk"):
-        lsm.add_file(3 + i % 3, f(k + "0", k + "9"))
+    lsm.add_file(1, f("c", "e"))
+    lsm.add_file(1, f("f", "h"))
+    lsm.add_file(2, f("d", "g"))
+    for lvl, lo, hi in [(3, "a0", "b9"), (3, "c5", "d5"), (3, "g0", "g9"), (4, "e0", "e9"), (4, "x0", "x9"),
+                        (5, "a", "z"), (6, "h5", "h9")]:
+        lsm.add_file(lvl, f(lo, hi))
     spec = lsm.pick()
-    assert spec.level == 1 and len(spec.deeper) == 5
-    cache = served._DeeperCache()
+    assert spec.level == 1 and [x.smallest[:-8] for x in spec.inputs[0]] == [b"c"]
+    # the job spans user keys c .. g (its level-1 file and the level-2 file);
+    # "g0" sorts after "g", so the level-3 file g0 .. g9 is outside
+    got = [(d, x.smallest[:-8].decode()) for d, x in spec.deeper]
+    assert got == [(3, "c5"), (4, "e0"), (5, "a")]
     opts = small_options()
 
     def rows(job):
-        n = job.num_deeper
-        return bytes(C.string_at(C.cast(job.deeper, C.c_void_p), n * C.sizeof(served.ffi.luda_level_file)))
-    plain, _k1 = served._job_struct(spec, opts)
-    cached, _k2 = served._job_struct(spec, opts, lsm, cache)
-    assert rows(plain) == rows(cached)
-    assert 3 in cache.rows  # the cache was used
-    other = served.Lsm()
-    again, _k3 = served._job_struct(spec, opts, other, cache)
-    assert rows(again) == rows(plain)
-    lsm.add_file(5, f("z0", "z9"))  # level 5 changed after the pick
-    stale, _k4 = served._job_struct(spec, opts, lsm, cache)
-    assert stale.num_deeper == 5 and rows(stale) == rows(plain)
-    assert np.frombuffer(rows(plain), dtype=served._LEVEL_FILE)["level"].tolist() == [3, 3, 4, 4, 5]
+        return bytes(C.string_at(C.cast(job.deeper, C.c_void_p), job.num_deeper * C.sizeof(served.ffi.luda_level_file)))
+    with_rows, _k1 = served._job_struct(spec, opts)
+    rebuilt = served.JobSpec(spec.level, spec.inputs, spec.grandparents, spec.deeper_files, spec.smallest_snapshot)
+    from_files, _k2 = served._job_struct(rebuilt, opts)
+    assert rows(with_rows) == rows(from_files) and with_rows.num_deeper == 3
+    lsm.add_file(4, f("ea", "eb"))  # a deeper level changes after the pick
+    again, _k3 = served._job_struct(spec, opts)
+    assert rows(again) == rows(with_rows)
+    assert np.frombuffer(rows(with_rows), dtype=served._LEVEL_FILE)["level"].tolist() == [3, 4, 5]
