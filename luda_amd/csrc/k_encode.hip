@@ -1565,8 +1565,11 @@ constexpr u32 kTailThreads = 1024;
 constexpr u32 kTailWaves = kTailThreads / 64;
 constexpr u32 kTailZeros = 14;  // zero-append maps kept in LDS: up to 32 KiB of zeros
 
-// One workgroup per table: filter offsets, filter/metaindex/index blocks with
-// trailers, footer, and the table's metadata.
+// Two workgroups per table when it has a filter, one otherwise: the first
+// writes the filter block's offsets, trailer and CRC; the other the
+// metaindex and index blocks with their trailers, the footer and the table's
+// metadata.  The two blocks' CRCs (the long pieces of the tail) thus run on
+// different CUs; everything either side needs is known from the layout.
 template <u32 NWMAX>
 __global__ __launch_bounds__(kTailThreads) void k_write_tail(Kept k, const u32* nf_dev, EncParams ep, EncBufs b, u8* out,
                                                           u64 out_cap, DevStatus* st) {
@@ -1582,23 +1585,35 @@ __global__ __launch_bounds__(kTailThreads) void k_write_tail(Kept k, const u32* 
   const u64 nf = *nf_dev;
   const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const Planes& K = k.K;
-  for (u64 f = blockIdx.x; f < nf; f += gridDim.x) {
+  const u64 items = ep.bloom_bits ? 2 * nf : nf;
+  for (u64 t = blockIdx.x; t < items; t += gridDim.x) {
+    const bool filt = ep.bloom_bits && t < nf;  // this workgroup's part: the filter block
+    const u64 f = ep.bloom_bits && !filt ? t - nf : t;
     const u32 b0 = b.f_first[f], b1 = b.f_first[f + 1];
     u8* base = out + b.f_off[f];
     const u64 dend = b.f_dend[f];
     u64 fbsize = 0;
-    // filter block: offsets array, array offset, base lg (FilterBlockBuilder::Finish)
     if (ep.bloom_bits) {
       const u32 w0 = b.f_win[f], w1 = b.f_win[f + 1];
       const u64 fbytes = b.f_fbytes[f];
-      u8* fs = base + dend;
-      for (u32 i = threadIdx.x; i < w1 - w0; i += blockDim.x)
-        st32(fs + fbytes + 4ull * i, (u32)(b.w_fsz[w0 + i] - b.w_fsz[w0]));
-      if (threadIdx.x == 0) {
-        st32(fs + fbytes + 4ull * (w1 - w0), (u32)fbytes);
-        fs[fbytes + 4ull * (w1 - w0) + 4] = 11;
-      }
       fbsize = fbytes + 4ull * (w1 - w0) + 5;
+      if (filt) {
+        // filter block: offsets array, array offset, base lg
+        // (FilterBlockBuilder::Finish), type byte, CRC
+        u8* fs = base + dend;
+        for (u32 i = threadIdx.x; i < w1 - w0; i += blockDim.x)
+          st32(fs + fbytes + 4ull * i, (u32)(b.w_fsz[w0 + i] - b.w_fsz[w0]));
+        if (threadIdx.x == 0) {
+          st32(fs + fbytes + 4ull * (w1 - w0), (u32)fbytes);
+          fs[fbytes + 4ull * (w1 - w0) + 4] = 11;
+          fs[fbsize] = 0;
+        }
+        __syncthreads();
+        const u32 crc = block_crc_split<kTailWaves>(fbsize + 1, (const GMEM u8*)fs, sl, zl, kTailZeros, s_part);
+        if (threadIdx.x == 0) st32(fs + fbsize + 1, crc_mask(crc));
+        __syncthreads();
+        continue;
+      }
     }
     const u64 moff = dend + (ep.bloom_bits ? fbsize + 5 : 0);
     const u64 msize = meta_block_size(ep, dend, fbsize);
@@ -1664,20 +1679,13 @@ __global__ __launch_bounds__(kTailThreads) void k_write_tail(Kept k, const u32* 
       }
       if (threadIdx.x == 0) st32(ib + isum + 4ull * (b1 - b0), b1 - b0);
     }
-    // type bytes, then the filter and index blocks' CRCs with the rows of
-    // each split over the waves (the long ones: ~47 KB of filters per
-    // 2 MiB table), the metaindex's by wave 0
+    // type bytes, then the index block's CRC with its rows split over the
+    // waves, the metaindex's by wave 0
     if (threadIdx.x == 0) {
-      if (ep.bloom_bits) base[dend + fbsize] = 0;
       base[ioff + isize] = 0;
       base[s_moff + s_msize] = 0;
     }
     __syncthreads();
-    if (ep.bloom_bits) {
-      u8* fs = base + dend;
-      const u32 crc = block_crc_split<kTailWaves>(fbsize + 1, (const GMEM u8*)fs, sl, zl, kTailZeros, s_part);
-      if (threadIdx.x == 0) st32(fs + fbsize + 1, crc_mask(crc));
-    }
     {
       u8* is = base + ioff;
       const u32 crc = block_crc_split<kTailWaves>(isize + 1, (const GMEM u8*)is, sl, zl, kTailZeros, s_part);
@@ -1803,7 +1811,7 @@ void launch_write_filters(KTimer& T, Kept k, const u32* nw_dev, u32 nw_cap, cons
 }
 void launch_write_tail(KTimer& T, Kept k, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b, u8* out,
                        u64 out_cap, DevStatus* st) {
-  const u32 g = grid_for(nf_cap, 1);
+  const u32 g = grid_for(ep.bloom_bits ? 2ull * nf_cap : nf_cap, 1);
   if (k.K.W <= 2)
     KLAUNCH(T, "write_tail", k_write_tail<3>, g, kTailThreads, 0, k, nf_dev, ep, b, out, out_cap, st);
   else if (k.K.W <= 4)
