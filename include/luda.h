@@ -168,6 +168,19 @@ int luda_build_tables(luda_ctx* ctx, const luda_options* opts, uint64_t n, const
                       uint32_t flags, luda_result** out, char* err, size_t errlen);
 
 void luda_free_result(luda_ctx* ctx, luda_result* res);
+/* Verifies tables the way Table::Open plus a full read with
+ * ReadOptions::verify_checksums does (table/table.cc:38-134, ReadBlock
+ * table/format.cc:83-96, which the fork leaves unchecked at :88): the footer,
+ * the index block, the metaindex block and the blocks it names (the filter),
+ * and every data block's masked CRC-32C over the block and its type byte.
+ * LUDA_OK when every table passes; LUDA_CORRUPTION otherwise, with
+ * *first_bad the lowest failing table when a checksum or the footer/metaindex
+ * check failed (UINT32_MAX when the fault was found while the index blocks
+ * were read).  `flags` takes LUDA_JOB_INPUTS_ON_DEVICE.  A caller can run it
+ * on a compaction's outputs where DBImpl::FinishCompactionOutputFile
+ * re-opens them (db/db_impl.cc:1023-1033). */
+int luda_verify_tables(luda_ctx* ctx, const luda_file* tables, uint32_t n, uint32_t flags, uint32_t* first_bad,
+                       char* err, size_t errlen);
 
 /* Batched and served compaction: jobs queued on the context run on worker
  * slots, each with its own HIP stream and working memory, so that the

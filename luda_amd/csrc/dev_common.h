@@ -41,6 +41,7 @@ enum ErrSite : u32 {
   SITE_BLOCK_ENTRIES = 11,
   SITE_FILE_BLOCKS = 12,
   SITE_SCAN = 13,
+  SITE_CHECKSUM = 14,
 };
 
 // Device-resident counters and flags of one job.  The host reads this struct
@@ -69,6 +70,8 @@ struct DevStatus {
   u64 out_bytes;      // sum of output table sizes
   u64 data_bytes;     // output data blocks with trailers
   u64 keys_needed;    // bytes of smallest/largest keys
+  u32 bad_file;       // luda_verify_tables: lowest table with a failed check
+  u32 pad;
 };
 
 // ---------------------------------------------------------------------------

@@ -237,6 +237,11 @@ void launch_stage_in(KTimer& T, const u8* src, u8* dst, u64 bytes, const DevStat
 void launch_stage_out(KTimer& T, u8* dst, StageSeg s0, StageSeg s1, StageSeg s2);
 
 void launch_files(KTimer& T, const FileDesc* files, u32 nfiles, FileInfo* info, u32* nblk, DevStatus* st);
+// k_verify.hip: every block's masked CRC (data blocks from launch_index's
+// descriptors, then per table the index, metaindex and named meta blocks);
+// a failure sets ERR_CORRUPT / SITE_CHECKSUM and the lowest bad table in
+// st->bad_file.
+void launch_verify(KTimer& T, const FileDesc* files, u32 nfiles, const BlockDesc* blks, u32 nb_cap, DevStatus* st);
 void launch_index(KTimer& T, const FileDesc* files, const FileInfo* info, const u32* blk_base, u32 nfiles,
                   u32 nb_cap, BlockDesc* blks, u32* blk_nent, DevStatus* st);
 // blk_ebase = exclusive scan of blk_nent over st->n_blocks_in blocks,

@@ -125,6 +125,7 @@ const char* site_name(u32 s) {
     case SITE_BLOCK_ENTRIES: return "block entries";
     case SITE_FILE_BLOCKS: return "table blocks";
     case SITE_SCAN: return "prefix sum";
+    case SITE_CHECKSUM: return "block checksum";
     default: return "unknown";
   }
 }
@@ -985,6 +986,72 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
 }
 
 // ---------------------------------------------------------------------------
+// luda_verify_tables
+
+void verify_tables(luda_ctx* c, const luda_file* tables, u32 n, u32 flags, u32* first_bad) {
+  *first_bad = 0xffffffffu;
+  if (n == 0) return;
+  std::vector<FileDesc> fdesc(n);
+  u64 nb_cap = 0;
+  if (flags & LUDA_JOB_INPUTS_ON_DEVICE) {
+    for (u32 i = 0; i < n; i++) fdesc[i] = {tables[i].data, tables[i].size};
+  } else {
+    u64 need = 0;
+    for (u32 i = 0; i < n; i++) need += align_up(tables[i].size + 16, 256);
+    HIPCHK(c->in_buf.reserve(need));
+    u64 off = 0;
+    for (u32 i = 0; i < n; i++) {
+      HIPCHK(hipMemcpyAsync(c->in_buf.p + off, tables[i].data, tables[i].size, hipMemcpyHostToDevice, c->s));
+      fdesc[i] = {c->in_buf.p + off, tables[i].size};
+      off += align_up(tables[i].size + 16, 256);
+    }
+  }
+  for (u32 i = 0; i < n; i++) nb_cap += tables[i].size / 41 + 1;
+  if (nb_cap > 0xffffffffull / 2) throw Fail{LUDA_NOT_SUPPORTED, "tables too large"};
+  FileDesc* d_files;
+  FileInfo* d_info;
+  u32 *d_nblk, *d_bbase, *d_nent;
+  BlockDesc* d_blks;
+  auto carve = [&](Arena& a) {
+    d_files = a.take<FileDesc>(n);
+    d_info = a.take<FileInfo>(n);
+    d_nblk = a.take<u32>(n + 1);
+    d_bbase = a.take<u32>(n + 1);
+    d_blks = a.take<BlockDesc>(nb_cap);
+    d_nent = a.take<u32>(nb_cap + 1);
+  };
+  {
+    Arena dry;
+    dry.dry = true;
+    carve(dry);
+    HIPCHK(c->a_buf.reserve(dry.used));
+    Arena a;
+    a.base = c->a_buf.p;
+    a.cap = c->a_buf.cap;
+    carve(a);
+  }
+  init_ctx_timer(c);
+  KTimer& T = c->T;
+  DevStatus st0;
+  memset(&st0, 0, sizeof(st0));
+  st0.bad_file = 0xffffffffu;
+  stage_in(c, d_files, fdesc.data(), n * sizeof(FileDesc), &st0);
+  launch_files(T, d_files, n, d_info, d_nblk, c->st);
+  HIPCHK(scan_reserve(c->scans, std::max<u64>(nb_cap, n + 1), c->s));
+  scan_u32(T, c->scans, d_nblk, d_bbase, nullptr, n + 1, n + 1, nullptr, c->st);
+  launch_index(T, d_files, d_info, d_bbase, n, (u32)nb_cap, d_blks, d_nent, c->st);
+  launch_verify(T, d_files, n, d_blks, (u32)nb_cap, c->st);
+  HIPCHK(hipGetLastError());
+  stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)});
+  stream_sync(c);
+  DevStatus st;
+  memcpy(&st, c->pin.p, sizeof(DevStatus));
+  *first_bad = st.bad_file;
+  if (status_code_of(st) != LUDA_OK)
+    throw Fail{status_code_of(st), std::string("table rejected: ") + site_name(st.site)};
+}
+
+// ---------------------------------------------------------------------------
 // luda_build_tables
 
 luda_result* build_tables(luda_ctx* c, const luda_options* opts, u64 n, const u8* keys, const u64* key_off,
@@ -1417,6 +1484,15 @@ extern "C" {
 
 int luda_compact(luda_ctx* c, const luda_job* job, luda_result** out, char* err, size_t errlen) {
   return ctx_compact(c, job, out, err, errlen);
+}
+
+int luda_verify_tables(luda_ctx* c, const luda_file* tables, uint32_t n, uint32_t flags, uint32_t* first_bad,
+                       char* err, size_t errlen) {
+  if (!c || !first_bad || (n && !tables)) return LUDA_INVALID_ARGUMENT;
+  *first_bad = 0xffffffffu;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  return guarded(c, err, errlen, [&] { verify_tables(c, tables, n, flags, first_bad); });
 }
 
 int luda_build_tables(luda_ctx* c, const luda_options* opts, uint64_t n, const uint8_t* keys, const uint64_t* key_off,

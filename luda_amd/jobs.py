@@ -202,6 +202,25 @@ class Engine:
     def free(self, res):
         self.lib.luda_free_result(self.ctx, res)
 
+    def verify(self, tables, on_device=False):
+        """luda_verify_tables over table images (bytes, or (device pointer,
+        size) pairs with on_device): (status, first bad table or None, message)."""
+        arr = (ffi.luda_file * max(1, len(tables)))()
+        keep = []
+        for i, t in enumerate(tables):
+            if on_device:
+                arr[i].data, arr[i].size = t[0], t[1]
+            else:
+                b = C.create_string_buffer(bytes(t), len(t))
+                keep.append(b)
+                arr[i].data, arr[i].size = C.cast(b, C.c_void_p), len(t)
+            arr[i].number = i + 1
+        bad = C.c_uint32()
+        err = C.create_string_buffer(512)
+        rc = self.lib.luda_verify_tables(self.ctx, arr, len(tables), ffi.LUDA_JOB_INPUTS_ON_DEVICE if on_device else 0,
+                                         C.byref(bad), err, 512)
+        return rc, (None if bad.value == 0xffffffff else bad.value), err.value.decode()
+
     # -- batched / served form (luda_submit, luda_poll, luda_drain) --------
     def set_slots(self, slots):
         rc = self.lib.luda_set_slots(self.ctx, slots)
