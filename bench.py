@@ -441,12 +441,22 @@ def main():
     log(f"{args.steps} steps in {elapsed:.3f}s: {value:.2f} GB/s input, device {dev_mean:.2f} ms/job, "
         f"{top} {per_launch_ms:.3f} ms/launch")
 
-    # Output tables of the last timed step, read back for the parity check.
+    # Output tables of the last timed step: verified in HBM (luda_verify_tables,
+    # every block's CRC), then read back for the parity check.
     r = last.contents
-    gpu_tables = []
-    for i in range(r.num_outputs):
-        o = r.outputs[i]
-        gpu_tables.append(eng.d2h(r.device_data + o.offset, o.file_size))
+    outs = [(r.device_data + r.outputs[i].offset, r.outputs[i].file_size) for i in range(r.num_outputs)]
+    vt = []
+    for _ in range(3):
+        v0 = time.perf_counter()
+        vrc, vbad, vmsg = eng.verify(outs, on_device=True)
+        vt.append(time.perf_counter() - v0)
+        if vrc != ffi.LUDA_OK:
+            raise RuntimeError(f"luda_verify_tables rejected output table {vbad}: {vmsg}")
+    out_bytes = sum(sz for _, sz in outs)
+    verify = {"tables": len(outs), "ms": round(min(vt) * 1e3, 3), "gbps": round(out_bytes / min(vt) / 1e9, 3),
+              "passed": True}
+    log(f"outputs verified on the GPU: {len(outs)} tables in {verify['ms']} ms ({verify['gbps']} GB/s)")
+    gpu_tables = [eng.d2h(p, sz) for p, sz in outs]
     eng.free(last)
 
     cpu = None
@@ -503,6 +513,7 @@ def main():
                      "bytes_per_launch": per_launch_bytes, "ms_per_launch": round(per_launch_ms, 4),
                      "traffic": measured_traffic(top, args.config, args.scale)},
         "cpu_baseline": cpu,
+        "verify_outputs": verify,
         "parity": parity,
     }
     print(json.dumps(line), flush=True)
