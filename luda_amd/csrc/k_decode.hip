@@ -11,8 +11,6 @@
 // lane decode one group with its output position known up front.
 #include <algorithm>
 
-#include <atomic>
-
 #include "engine.h"
 #include "scan.h"
 
@@ -959,27 +957,6 @@ void scan_entries(KTimer& T, ScanState& S, const BlockDesc* blks, const u32* blk
   lbscan::run<u32, u32>(T, "scan_entries", S, lbscan::ArrLoad<u32>{blk_nent}, blk_ebase, &st->n_blocks_in, 0, nb_cap,
                         &st->n_entries, st, nullptr, FileEbaseEpi{blks, file_ebase, nfiles});
 }
-// Workgroups of one wavefront and `lds` bytes of LDS that the device holds
-// at once (per CU: as many as its LDS fits, at most 32 wavefronts), from the
-// first device asked (every device here is the same part).
-static u32 resident_workgroups(u32 lds) {
-  static std::atomic<int> n_cu{0}, lds_cu{0};
-  if (n_cu.load() == 0) {
-    int dev = 0, a = 0, b = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&a, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-        hipDeviceGetAttribute(&b, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess && a > 0 &&
-        b > 0) {
-      lds_cu.store(b);
-      n_cu.store(a);
-    } else {
-      return 4096;  // unknown: the plain cap
-    }
-  }
-  const u32 per_cu = std::min<u32>(32, std::max<u32>(1, (u32)lds_cu.load() / std::max<u32>(lds, 1)));
-  return (u32)n_cu.load() * per_cu;
-}
-
 void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 nb, const u32* blk_ebase, Planes K,
                    u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr, u32 max_ent) {
   if (!nb) return;
@@ -993,12 +970,8 @@ void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 
     if (lds > 65536) launch_refused("decode (restart arrays too long for LDS)", hipErrorInvalidValue);
     const u32 ecap = (u32)std::min<u64>((u64)max_ent, (65536 - lds) / 2);
     lds += ((2ull * ecap + 3) & ~3ull);
-    // a few blocks per wavefront, so that the next descriptor's load overlaps;
-    // and no more wavefronts than the CUs' LDS holds at once: the blocks are
-    // dealt out by stride, so a wavefront that only starts when another ends
-    // would run its whole share after everyone else
-    const u32 resident = resident_workgroups((u32)lds);
-    const u32 grid = std::max<u32>(1, std::min<u32>(std::min<u32>((nb + 3) / 4, 4096), resident));
+    // a few blocks per wavefront, so that the next descriptor's load overlaps
+    const u32 grid = std::max<u32>(1, std::min<u32>((nb + 3) / 4, 4096));
     if (K.W <= 2)
       KLAUNCH(T, "decode", k_decode_entries<3>, grid, 64, lds, blks, nb, blk_ebase, K, vptr, vlen, st, stage, max_nr,
               ecap);
