@@ -125,3 +125,20 @@ def test_deeper_rows_match_the_job():
     again, _k3 = served._job_struct(spec, opts)
     assert rows(again) == rows(with_rows)
     assert np.frombuffer(rows(with_rows), dtype=served._LEVEL_FILE)["level"].tolist() == [3, 4, 5]
+
+
+@pytest.mark.parametrize("mode", ["plain", "served"])
+def test_a_failed_job_stops_the_run(mode):
+    """A compaction the backend rejects ends the run with its error instead
+    of leaving the tree waiting on a job that never completes."""
+    mems = stream(puts=12000, key_space=9000)
+    opts = small_options()
+
+    class Failing(OracleBackend):
+        def compact(self, lsm, spec):
+            raise RuntimeError("compaction rejected")
+    be = Failing(opts)
+    lsm = served.Lsm(max_file_size=opts.max_file_size, level1_bytes=48 << 10)
+    run = served.run_plain if mode == "plain" else served.run_served
+    with pytest.raises(RuntimeError, match="compaction rejected"):
+        run(be, mems, 0.0, lsm=lsm)
