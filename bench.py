@@ -77,22 +77,7 @@ def percentile(xs, p):
 
 
 def build_device_inputs(eng, wl, opts):
-    """Input tables built on the GPU and left in HBM: per which, (ptr, size)."""
-    dev = ([], [])
-    keep = []
-    for g in wl.groups:
-        o = jobs.group_opts(opts, g)
-        rc, out, msg = eng.build_tables(C.byref(o), g.n, C.cast(g.keys, C.c_void_p), C.cast(g.key_off, C.c_void_p),
-                                        C.cast(g.vals, C.c_void_p), C.cast(g.val_off, C.c_void_p),
-                                        keep_on_device=True)
-        if rc != ffi.LUDA_OK:
-            raise RuntimeError(f"luda_build_tables failed ({rc}): {msg}")
-        res = out[2]
-        keep.append(res)
-        r = res.contents
-        for i in range(r.num_outputs):
-            dev[g.which].append((r.device_data + r.outputs[i].offset, r.outputs[i].file_size))
-    return dev, keep
+    return jobs.build_device_inputs(eng, wl, opts)
 
 
 def copy_probe(nbytes):
@@ -108,20 +93,14 @@ def copy_probe(nbytes):
 
 
 def mix_jobs(eng, opts, n, seed):
-    """n compaction jobs of log-uniform input sizes in [4, 256] MiB, inputs
-    built on the GPU and resident in HBM: Ln->Ln+1 jobs with the 256 MiB
-    job's key/value distribution, bottommost jobs with the 512 MiB job's,
-    and L0->L1 jobs (4 overlapping L0 tables + L1 tables, up to 40 MiB)."""
-    import random
-    rng = random.Random(seed)
+    """The mixed batch (luda_amd.jobs.mix_specs): n compaction jobs of
+    log-uniform input sizes in [4, 256] MiB, half L0->L1 with 4-12 level-0
+    tables, half Ln->Ln+1, half with the 256 MiB job's key/value distribution
+    and half with the skewed 512 MiB job's; inputs built on the GPU and
+    resident in HBM."""
     out = []
-    for k in range(n):
-        mib = 4.0 * (64.0 ** rng.random())  # log-uniform 4 .. 256
-        shape = [1, 2, 0][k % 3]
-        if shape == 0 and mib > 40:
-            shape = 1
-        scale = mib / {0: 20.0, 1: 256.0, 2: 512.0}[shape]
-        wl = jobs.Workload(shape, seed=0x4C554441 + 1000 + k, scale=scale)
+    for shape, wseed, scale, _mib in jobs.mix_specs(n, seed):
+        wl = jobs.Workload(shape, seed=wseed, scale=scale)
         dev, keep = build_device_inputs(eng, wl, opts)
         job = jobs.make_job(wl, opts, None, flags=ffi.LUDA_JOB_KEEP_ON_DEVICE, device_inputs=dev)
         out.append((shape, sum(s for which in dev for _, s in which), job, keep, wl))
@@ -140,9 +119,7 @@ def run_mix(args):
     log(f"{len(mix)} jobs, {total / 2**30:.2f} GiB of inputs built in {time.time() - t:.1f}s")
 
     def summary(res):
-        r = res.contents
-        return [eng.d2h(r.device_data + r.outputs[i].offset, r.outputs[i].file_size)
-                for i in range(r.num_outputs)]
+        return jobs.device_result_tables(eng, res)
     # plain: one luda_compact after another
     for shape, b, job, keep, wl in mix[:2]:  # warm-up
         rc, o, msg = eng.compact(job.job, keep_on_device=True)
@@ -156,7 +133,8 @@ def run_mix(args):
         plain_lat.append((time.perf_counter() - t0) * 1e3)
         plain_hash.append(o[2])
     t_plain = time.perf_counter() - t0
-    plain_hash = [[sha(x) for x in summary(res)] for res in plain_hash_free(eng, plain_hash, summary)]
+    plain_hash = [jobs.output_digest(summary(res)) for res in plain_hash_free(eng, plain_hash, summary)]
+    golden = mix_golden(args.mix, args.mix_seed)
     # batched: everything queued at once on the worker slots
     beng = jobs.Engine(0)
     beng.set_slots(args.slots)
@@ -175,7 +153,7 @@ def run_mix(args):
     for i in range(len(mix)):
         rc, o, ms = got[i]
         batch_lat.append(ms)
-        same &= rc == ffi.LUDA_OK and [sha(x) for x in summary(o[2])] == plain_hash[i]
+        same &= rc == ffi.LUDA_OK and jobs.output_digest(summary(o[2])) == plain_hash[i]
         beng.free(o[2])
     cpu = None
     if not args.no_cpu:
@@ -188,9 +166,11 @@ def run_mix(args):
         "plain_p50_ms": round(percentile(plain_lat, 50), 3), "plain_p99_ms": round(percentile(plain_lat, 99), 3),
         "batched_p50_ms": round(percentile(batch_lat, 50), 3), "batched_p99_ms": round(percentile(batch_lat, 99), 3),
         "same_tables": same,
+        "same_as_reference": (plain_hash == golden) if golden else None,
         "cpu_baseline": cpu,
-        "workload": "log-uniform 4-256 MiB jobs: Ln->Ln+1 (16-byte keys, 100-byte values), bottommost "
-                    "(Zipf 8-64-byte keys, 16-1024-byte values) and L0->L1 (up to 40 MiB), inputs in HBM",
+        "workload": "log-uniform 4-256 MiB jobs: half L0->L1 with 4-12 level-0 tables, half Ln->Ln+1; half "
+                    "16-byte decimal keys with 100-byte values, half Zipf 8-64-byte keys with 16-1024-byte "
+                    "values; inputs in HBM",
     }
     print(json.dumps(line), flush=True)
     beng.close()
@@ -223,7 +203,7 @@ def mix_cpu_baseline(eng, opts, mix, gpu_hashes):
         rc, tabs = oracle.compact(hjobs[k][1].job, which)
         if rc != 0:
             raise RuntimeError(f"CPU compaction of job {k} failed ({rc})")
-        return [sha(t.data) for t in tabs]
+        return jobs.output_digest(tabs)
     seq_bytes, seq_same, k = 0, True, 0
     t0 = time.perf_counter()
     while k < len(hjobs) and seq_bytes < (1 << 30):
@@ -340,6 +320,19 @@ def run_serve(args):
     print(json.dumps(out), flush=True)
     seng.close()
     peng.close()
+
+
+def mix_golden(n, seed):
+    """Per-job output digests the reference's own code produced for this mix
+    (tests/golden/mix_jobs.json), or None when it holds no record for it."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "mix_jobs.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    if rec.get("jobs") != n or rec.get("seed") != seed:
+        return None
+    return [j["digest"] for j in rec["records"]]
 
 
 def plain_hash_free(eng, results, summary):

@@ -48,7 +48,14 @@ extern "C" {
 typedef struct luda_options {
   uint64_t block_size;             /* Options::block_size, options.h:100 (4096) */
   uint64_t max_file_size;          /* Options::max_file_size, options.h:115 (2 MiB) */
-  uint32_t block_restart_interval; /* Options::block_restart_interval, options.h:105 (16) */
+  uint32_t block_restart_interval; /* Options::block_restart_interval, options.h:105 (16).  The fork
+                                      stores each restart as (offset << 8) | count
+                                      (table/block_builder.cc:62-68,89-94), so a restart group of 256 or
+                                      more entries spills its count into the offset bits.  The writers
+                                      reproduce those bytes for any interval, as the fork's release build
+                                      does; tables with such a group cannot be read back, by the fork's
+                                      Block::Iter or by luda_compact/luda_verify_tables (CORRUPTION).
+                                      Intervals up to 255 always round-trip. */
   uint32_t bloom_bits_per_key;     /* NewBloomFilterPolicy(bits), util/bloom.cc:17; 0 = no filter_policy */
   uint32_t compression;            /* Options::compression; only kNoCompression (0) is supported */
   uint32_t reserved;

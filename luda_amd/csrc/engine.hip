@@ -19,6 +19,8 @@
 #include <condition_variable>
 #include <deque>
 #include <mutex>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1179,20 +1181,52 @@ luda_result* build_tables(luda_ctx* c, const luda_options* opts, u64 n, const u8
   }
 }
 
+// LUDA_FAULT=<site>:<kind> makes the call at <site> ("init", "compact",
+// "build", "verify") throw before it does any work: kind "bad_alloc" throws
+// std::bad_alloc, "runtime" a std::runtime_error, anything else an int.  It
+// lets a test check that no exception crosses the C-ABI.
+void fault_point(const char* site) {
+  const char* e = getenv("LUDA_FAULT");
+  if (!e || !*e) return;
+  const char* colon = strchr(e, ':');
+  const size_t n = colon ? (size_t)(colon - e) : strlen(e);
+  if (strlen(site) != n || strncmp(e, site, n) != 0) return;
+  const char* kind = colon ? colon + 1 : "";
+  if (!strcmp(kind, "bad_alloc")) throw std::bad_alloc();
+  if (!strcmp(kind, "runtime")) throw std::runtime_error("injected fault");
+  throw 42;
+}
+
+// Every entry point runs its body through here, so that no C++ exception
+// crosses the C-ABI (a caller's background thread would end in
+// std::terminate): the engine's own failures keep their status; running out
+// of host memory, or any other exception, is an IO_ERROR, which sends the
+// caller to its CPU path like every other non-OK status.
 template <typename F>
 int guarded(luda_ctx* c, char* err, size_t errlen, F fn) {
+  int code = LUDA_OK;
   try {
     fn();
     return LUDA_OK;
   } catch (const Fail& f) {
     if (err && errlen) snprintf(err, errlen, "%s", f.msg.c_str());
-    // a failed call may leave work queued; drain it so the next call starts clean
-    if (c && c->s) {
-      hipStreamSynchronize(c->s);
-      c->pin2_busy = false;
-    }
-    return f.code;
+    code = f.code;
+  } catch (const std::bad_alloc&) {
+    if (err && errlen) snprintf(err, errlen, "host memory exhausted");
+    code = LUDA_IO_ERROR;
+  } catch (const std::exception& x) {
+    if (err && errlen) snprintf(err, errlen, "internal error: %s", x.what());
+    code = LUDA_IO_ERROR;
+  } catch (...) {
+    if (err && errlen) snprintf(err, errlen, "internal error");
+    code = LUDA_IO_ERROR;
   }
+  // a failed call may leave work queued; drain it so the next call starts clean
+  if (c && c->s) {
+    hipStreamSynchronize(c->s);
+    c->pin2_busy = false;
+  }
+  return code;
 }
 
 // The worker slots of luda_submit.  Each slot is a context of its own
@@ -1374,31 +1408,33 @@ int luda_set_slots(luda_ctx* c, uint32_t slots) {
 
 int luda_submit(luda_ctx* c, const luda_job* job, luda_done_fn done, void* user) {
   if (!c || !job) return LUDA_INVALID_ARGUMENT;
-  JobPool* P = pool_of(c);
-  if (!P) return LUDA_IO_ERROR;
-  {
-    std::lock_guard<std::mutex> lk(P->mu);
-    P->queue.push_back(QueuedJob{job, BuildReq{}, done, user, job_footprint(job)});
-    P->submitted++;
-  }
-  P->cv_work.notify_one();
-  return LUDA_OK;
+  return guarded(nullptr, nullptr, 0, [&] {
+    JobPool* P = pool_of(c);
+    if (!P) throw Fail{LUDA_IO_ERROR, "no worker slot"};
+    {
+      std::lock_guard<std::mutex> lk(P->mu);
+      P->queue.push_back(QueuedJob{job, BuildReq{}, done, user, job_footprint(job)});
+      P->submitted++;
+    }
+    P->cv_work.notify_one();
+  });
 }
 
 int luda_submit_build(luda_ctx* c, const luda_options* opts, uint64_t n, const uint8_t* keys,
                       const uint64_t* key_off, const uint8_t* vals, const uint64_t* val_off, uint32_t flags,
                       luda_done_fn done, void* user) {
   if (!c || !opts || (n && (!keys || !key_off || !vals || !val_off))) return LUDA_INVALID_ARGUMENT;
-  JobPool* P = pool_of(c);
-  if (!P) return LUDA_IO_ERROR;
-  BuildReq b{*opts, n, keys, key_off, vals, val_off, flags};
-  {
-    std::lock_guard<std::mutex> lk(P->mu);
-    P->queue.push_back(QueuedJob{nullptr, b, done, user, build_footprint(b)});
-    P->submitted++;
-  }
-  P->cv_work.notify_one();
-  return LUDA_OK;
+  return guarded(nullptr, nullptr, 0, [&] {
+    JobPool* P = pool_of(c);
+    if (!P) throw Fail{LUDA_IO_ERROR, "no worker slot"};
+    BuildReq b{*opts, n, keys, key_off, vals, val_off, flags};
+    {
+      std::lock_guard<std::mutex> lk(P->mu);
+      P->queue.push_back(QueuedJob{nullptr, b, done, user, build_footprint(b)});
+      P->submitted++;
+    }
+    P->cv_work.notify_one();
+  });
 }
 
 int luda_poll(luda_ctx* c) {
@@ -1418,15 +1454,18 @@ namespace {
 int ctx_open(int device, uint64_t hbm_budget_bytes, luda_ctx** out, char* err, size_t errlen) {
   if (!out) return LUDA_INVALID_ARGUMENT;
   *out = nullptr;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
-    if (err && errlen) snprintf(err, errlen, "no HIP device %d", device);
-    return LUDA_NOT_SUPPORTED;
+  luda_ctx* c = new (std::nothrow) luda_ctx;
+  if (!c) {
+    if (err && errlen) snprintf(err, errlen, "host memory exhausted");
+    return LUDA_IO_ERROR;
   }
-  luda_ctx* c = new luda_ctx;
   c->device = device;
   c->budget = hbm_budget_bytes ? hbm_budget_bytes : (64ull << 30);
   int rc = guarded(c, err, errlen, [&] {
+    fault_point("init");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)
+      throw Fail{LUDA_NOT_SUPPORTED, "no HIP device " + std::to_string(device)};
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
     HIPCHK(hipMalloc(&c->st, sizeof(DevStatus)));
@@ -1466,7 +1505,10 @@ int ctx_compact(luda_ctx* c, const luda_job* job, luda_result** out, char* err, 
   *out = nullptr;
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);
-  return guarded(c, err, errlen, [&] { *out = compact(c, job); });
+  return guarded(c, err, errlen, [&] {
+    fault_point("compact");
+    *out = compact(c, job);
+  });
 }
 
 int ctx_build(luda_ctx* c, const luda_options* opts, u64 n, const u8* keys, const u64* key_off, const u8* vals,
@@ -1475,7 +1517,10 @@ int ctx_build(luda_ctx* c, const luda_options* opts, u64 n, const u8* keys, cons
   *out = nullptr;
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);
-  return guarded(c, err, errlen, [&] { *out = build_tables(c, opts, n, keys, key_off, vals, val_off, flags); });
+  return guarded(c, err, errlen, [&] {
+    fault_point("build");
+    *out = build_tables(c, opts, n, keys, key_off, vals, val_off, flags);
+  });
 }
 
 }  // namespace
@@ -1492,7 +1537,10 @@ int luda_verify_tables(luda_ctx* c, const luda_file* tables, uint32_t n, uint32_
   *first_bad = 0xffffffffu;
   std::lock_guard<std::mutex> lk(c->mu);
   hipSetDevice(c->device);
-  return guarded(c, err, errlen, [&] { verify_tables(c, tables, n, flags, first_bad); });
+  return guarded(c, err, errlen, [&] {
+    fault_point("verify");
+    verify_tables(c, tables, n, flags, first_bad);
+  });
 }
 
 int luda_build_tables(luda_ctx* c, const luda_options* opts, uint64_t n, const uint8_t* keys, const uint64_t* key_off,

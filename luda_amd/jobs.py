@@ -38,6 +38,22 @@ def result_tables(res):
     return out
 
 
+def device_result_tables(eng, res):
+    """Like result_tables for a result kept on the device
+    (LUDA_JOB_KEEP_ON_DEVICE): table bytes are read back from HBM."""
+    r = res.contents
+    out = []
+    keys = C.string_at(r.keys, max(1, sum(r.outputs[i].smallest_size + r.outputs[i].largest_size
+                                          for i in range(r.num_outputs)))) if r.num_outputs else b""
+    for i in range(r.num_outputs):
+        o = r.outputs[i]
+        out.append(Table(data=eng.d2h(r.device_data + o.offset, o.file_size),
+                         smallest=keys[o.smallest_offset:o.smallest_offset + o.smallest_size],
+                         largest=keys[o.largest_offset:o.largest_offset + o.largest_size],
+                         num_entries=o.num_entries))
+    return out
+
+
 class Workload:
     """A seeded job from libluda_workload.so (see luda_amd/workload/workload.h)."""
 
@@ -77,6 +93,35 @@ class Workload:
         return [(kb[ko[i]:ko[i + 1]], vb[vo[i]:vo[i + 1]]) for i in range(g.n)]
 
 
+def mix_specs(n=64, seed=3):
+    """The mixed batch of n compaction jobs: (shape, workload seed, scale,
+    MiB) with input sizes log-uniform in [4, 256] MiB.  Half are L0->L1 jobs
+    with 4-12 level-0 tables (shapes 3 and 4), half Ln->Ln+1 jobs (shapes 1
+    and 2); half use the 256 MiB job's key/value distribution (shapes 1 and 3)
+    and half the skewed 512 MiB job's (shapes 2 and 4)."""
+    import random
+    rng = random.Random(seed)
+    out = []
+    for k in range(n):
+        mib = 4.0 * (64.0 ** rng.random())
+        shape = [3, 1, 4, 2][k % 4]
+        scale = mib / (512.0 if shape == 2 else 256.0)
+        out.append((shape, 0x4C554441 + 1000 + k, scale, mib))
+    return out
+
+
+def output_digest(tables):
+    """SHA-256 over every output table's SHA-256, size, entry count and
+    boundary keys, in output order: one value that pins a whole job."""
+    import hashlib
+    h = hashlib.sha256()
+    for t in tables:
+        h.update(hashlib.sha256(bytes(t.data)).digest())
+        h.update(b"%d:%d:" % (len(t.data), t.num_entries))
+        h.update(bytes(t.smallest) + b"|" + bytes(t.largest) + b";")
+    return h.hexdigest()
+
+
 def group_opts(opts, g):
     o = ffi.luda_options()
     C.pointer(o)[0] = opts
@@ -88,6 +133,27 @@ def build_group_tables(builder, opts, g):
     """builder(opts_ptr, n, keys, key_off, vals, val_off) -> list[Table]."""
     return builder(C.byref(group_opts(opts, g)), g.n, C.cast(g.keys, C.c_void_p), C.cast(g.key_off, C.c_void_p),
                    C.cast(g.vals, C.c_void_p), C.cast(g.val_off, C.c_void_p))
+
+
+def build_device_inputs(eng, wl, opts):
+    """A workload's input tables built on the GPU by luda_build_tables and
+    left in HBM: per which, [(device pointer, size)], plus the results that
+    own them (free each with eng.free)."""
+    dev = ([], [])
+    keep = []
+    for g in wl.groups:
+        o = group_opts(opts, g)
+        rc, out, msg = eng.build_tables(C.byref(o), g.n, C.cast(g.keys, C.c_void_p), C.cast(g.key_off, C.c_void_p),
+                                        C.cast(g.vals, C.c_void_p), C.cast(g.val_off, C.c_void_p),
+                                        keep_on_device=True)
+        if rc != ffi.LUDA_OK:
+            raise RuntimeError(f"luda_build_tables failed ({rc}): {msg}")
+        res = out[2]
+        keep.append(res)
+        r = res.contents
+        for i in range(r.num_outputs):
+            dev[g.which].append((r.device_data + r.outputs[i].offset, r.outputs[i].file_size))
+    return dev, keep
 
 
 @dataclass

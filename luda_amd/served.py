@@ -343,14 +343,17 @@ class Lsm:
         """The highest-scoring level with a score >= 1 whose level pair is not
         in use; returns a JobSpec, a trivially moved file (installed here), or
         None."""
-        best, best_score = None, 1.0
+        # VersionSet::Finalize (db/version_set.cc:1050-1086) keeps the first
+        # level with the strictly highest score; PickCompaction runs it only
+        # when that score is >= 1 (:1276-1277).
+        best, best_score = None, -1.0
         for L in range(NUM_LEVELS - 1):
             if L in self.busy or L + 1 in self.busy:
                 continue
             s = self.score(L)
-            if s >= best_score:
+            if s > best_score:
                 best, best_score = L, s
-        if best is None:
+        if best is None or best_score < 1.0:
             return None
         L = best
         files = self.levels[L]

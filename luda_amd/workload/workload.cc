@@ -338,6 +338,55 @@ lw_workload* MakeSkewed(uint64_t seed, double scale) {
   return b.Finish(1, snap);
 }
 
+// ---- jobs 3 and 4: L0 -> L1 with 4-12 level-0 files --------------------------
+// The mixed batch's level-0 jobs.  The seed picks the number of level-0 files
+// k in [4, 12]; 40% of the entries are the newest, split by sequence into k
+// memtable-sized runs (one table each, as flushes write them), the older 60%
+// form the level-1 input cut at 2 MiB.  Job 3 uses the 256 MiB job's key and
+// value distribution (16-byte decimal keys uniform over 1e8, 100-byte values,
+// 2% deletes, no snapshot); job 4 the skewed one (Zipf(0.99) ids, 8-64 byte
+// keys, 16-1024 byte values, 5% deletes, oldest snapshot at the 25% quantile).
+// scale 1.0 is 256 MiB of input; no deeper level, so tombstones older than
+// the snapshot are dropped.
+lw_workload* MakeL0Many(uint64_t seed, double scale, bool skewed) {
+  Rng rng(seed);
+  Builder b;
+  const uint32_t k = 4 + (uint32_t)rng.Below(9);
+  const uint64_t m = Scaled(skewed ? 475000 : 2250000, scale);
+  std::vector<Entry> all(m);
+  const uint64_t nid = skewed ? std::max<uint64_t>(1000, (uint64_t)(10000000ull * std::min(1.0, scale))) : 0;
+  Zipf* z = skewed ? new Zipf(nid, 0.99) : nullptr;
+  for (uint64_t i = 0; i < m; i++) {
+    Entry& e = all[i];
+    bool del;
+    if (skewed) {
+      const uint64_t id = (z->Next(rng) * 2654435761ull) % nid;
+      e.ukey = SkewKey(id);
+      del = rng.Uniform() < 0.05;
+      e.vlen = del ? 0 : 16 + (uint32_t)rng.Below(1009);
+    } else {
+      e.ukey = Decimal16(rng.Next() % 100000000ull);
+      del = rng.Uniform() < 0.02;
+      e.vlen = del ? 0 : 100;
+    }
+    e.seq = i + 1;
+    e.type = del ? 0 : 1;
+    e.vseed = rng.Next();
+  }
+  delete z;
+  const uint64_t cut = (uint64_t)(0.6 * m);
+  std::vector<Entry> older(all.begin(), all.begin() + cut);
+  // A level holds each user key once per version, sorted; no dedupe is
+  // needed for a compaction input (DoCompactionWork drops the shadowed ones).
+  for (uint32_t f = 0; f < k; f++) {
+    const uint64_t a = cut + (m - cut) * f / k, e = cut + (m - cut) * (f + 1) / k;
+    std::vector<Entry> run(all.begin() + a, all.begin() + e);
+    b.AddGroup(0, UINT64_MAX, run);
+  }
+  b.AddGroup(1, 2 << 20, older);
+  return b.Finish(0, skewed ? std::max<uint64_t>(1, m / 4) : m);
+}
+
 // ---- job 5: grandparent cuts and a patchy deeper level ----------------------
 lw_workload* MakeCuts(uint64_t seed, double scale) {
   Rng rng(seed);
@@ -404,6 +453,10 @@ lw_workload* lw_make(int config, uint64_t seed, double scale) {
       return MakeL1(seed, scale);
     case 2:
       return MakeSkewed(seed, scale);
+    case 3:
+      return MakeL0Many(seed, scale, false);
+    case 4:
+      return MakeL0Many(seed, scale, true);
     case 5:
       return MakeCuts(seed, scale);
     default:

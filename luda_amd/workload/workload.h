@@ -18,6 +18,9 @@
  *   2  bottommost L1->L2, 512 MiB, Zipf(0.99) ids over 1e7, 8-64 byte keys with
  *      an 8-byte prefix per 4096-id range, 16-1024 byte values, 5% deletes,
  *      snapshots at the 25/50/75% sequence quantiles
+ *   3  L0->L1 with 4-12 level-0 tables (the seed picks how many), the
+ *      distribution of job 1 (256 MiB at scale 1)
+ *   4  the same with the distribution of job 2
  *   5  grandparent-cut stress: L1->L2 with many small L3 grandparents so
  *      ShouldStopBefore fires, plus deeper-level ranges that make
  *      IsBaseLevelForKey vary

@@ -734,7 +734,11 @@ static int block_iterate(span_t blk, buf_t* key, int (*fn)(void*, const buf_t*, 
   if (nr > (blk.n - 4) / 4) return LUDA_CORRUPTION;
   uint64_t restart_off = blk.n - (1 + (uint64_t)nr) * 4;
   if (nr == 0) return LUDA_OK; /* NewEmptyIterator */
-  const uint8_t* p = blk.p;
+  /* SeekToFirst (table/block.cc:202-205) starts at GetRestartPoint(0), the
+   * fork's first restart word shifted right by 8 (:98-102): 0 unless the
+   * first group's count spilled into the offset bits (256 or more entries),
+   * in which case decoding starts mid-entry, as the fork's reader does. */
+  const uint8_t* p = blk.p + (dec32(blk.p + restart_off) >> 8);
   const uint8_t* limit = blk.p + restart_off;
   key->n = 0;
   while (p < limit) {

@@ -3,13 +3,18 @@ CPU code (oracle/_ref/libluda_ref.so, built from the reference sources by
 oracle/Makefile).  Run in the build container, where the reference tree
 exists:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            (small jobs and corner tables)
+    python tests/golden/make_golden.py --full     (configs 0-2 at full size)
+    python tests/golden/make_golden.py --mix      (the 64-job mixed batch)
 
 Outputs (data only; no reference source is copied):
   jobs.json       SHA-256 of every input and output table, output metadata,
                   for small seeded instances of each workload shape
   tables/*.ldb    tiny tables covering block-format corner cases, with the
                   (key, value) entries they were built from in tables.json
+  full_jobs.json  configs 0, 1 and 2 at full size: SHA-256 and metadata of
+                  every output table
+  mix_jobs.json   one digest over each mixed-batch job's outputs
 """
 import hashlib
 import json
@@ -118,9 +123,58 @@ def build_ref_tables(entries, opts):
                                  C.cast(vb, C.c_void_p), C.cast(vo, C.c_void_p))
 
 
+def full_record(cfg):
+    """Config cfg at its full size with its default seed: every output's
+    SHA-256 and metadata, and the digest over them."""
+    opts = ffi.default_options()
+    wl = jobs.Workload(cfg)
+    ins = ([], [])
+    for g in wl.groups:
+        ins[g.which].extend(t.data for t in jobs.build_group_tables(oracle.builder("ref"), opts, g))
+    job = jobs.make_job(wl, opts, ins)
+    rc, outs = oracle.compact(job.job, "ref")
+    assert rc == 0
+    return {
+        "config": cfg, "seed": 0x4C554441 + cfg, "scale": 1.0, "input_bytes": job.bytes_in,
+        "inputs_digest": sha(b"".join(hashlib.sha256(t).digest() for t in ins[0] + ins[1])),
+        "digest": jobs.output_digest(outs),
+        "outputs": [{"sha256": sha(t.data), "file_size": len(t.data), "num_entries": t.num_entries,
+                     "smallest": t.smallest.hex(), "largest": t.largest.hex()} for t in outs],
+    }
+
+
+def mix_records(n=64, seed=3):
+    """Every job of the mixed batch (luda_amd.jobs.mix_specs) compacted by the
+    reference: one output digest per job."""
+    opts = ffi.default_options()
+    recs = []
+    for k, (shape, wseed, scale, mib) in enumerate(jobs.mix_specs(n, seed)):
+        wl = jobs.Workload(shape, seed=wseed, scale=scale)
+        ins = ([], [])
+        for g in wl.groups:
+            ins[g.which].extend(t.data for t in jobs.build_group_tables(oracle.builder("ref"), opts, g))
+        job = jobs.make_job(wl, opts, ins)
+        rc, outs = oracle.compact(job.job, "ref")
+        assert rc == 0
+        recs.append({"shape": shape, "seed": wseed, "scale": scale, "input_bytes": job.bytes_in,
+                     "files": [len(ins[0]), len(ins[1])], "outputs": len(outs), "digest": jobs.output_digest(outs)})
+        print("mix job", k, shape, round(mib, 1), "MiB", len(outs), "outputs", flush=True)
+        wl.close()
+    return {"jobs": n, "seed": seed, "records": recs}
+
+
 def main():
     if not oracle.ref_available():
         sys.exit("oracle/_ref/libluda_ref.so is not built (needs the reference tree)")
+    if "--full" in sys.argv:
+        recs = [full_record(c) for c in (0, 1, 2)]
+        with open(os.path.join(HERE, "full_jobs.json"), "w") as f:
+            json.dump(recs, f, indent=0)
+        return
+    if "--mix" in sys.argv:
+        with open(os.path.join(HERE, "mix_jobs.json"), "w") as f:
+            json.dump(mix_records(), f, indent=0)
+        return
     recs = []
     for cfg, seed, scale in JOB_CASES:
         for optname in OPTION_SETS:

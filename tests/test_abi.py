@@ -60,3 +60,27 @@ def test_queued_entry_points_reject_a_missing_context():
 def test_stats_layout():
     # luda_stats ends with end_clock_ms (include/luda.h)
     assert ffi.luda_stats.end_clock_ms.offset == C.sizeof(ffi.luda_stats) - 8
+
+
+def _init_with_fault(kind):
+    lib = ffi.engine()
+    ctx = C.c_void_p()
+    err = C.create_string_buffer(256)
+    os.environ["LUDA_FAULT"] = "init:" + kind
+    try:
+        rc = lib.luda_init(0, 0, C.byref(ctx), err, 256)
+    finally:
+        del os.environ["LUDA_FAULT"]
+    return rc, ctx.value, err.value.decode()
+
+
+def test_exceptions_never_cross_the_abi():
+    """A C++ exception other than the engine's own (host allocation failure,
+    a standard exception, anything thrown) comes back as LUDA_IO_ERROR with a
+    message, not as std::terminate in the caller's thread."""
+    rc, ctx, msg = _init_with_fault("bad_alloc")
+    assert rc == ffi.LUDA_IO_ERROR and not ctx and "memory" in msg
+    rc, ctx, msg = _init_with_fault("runtime")
+    assert rc == ffi.LUDA_IO_ERROR and not ctx and "injected" in msg
+    rc, ctx, msg = _init_with_fault("other")
+    assert rc == ffi.LUDA_IO_ERROR and not ctx and msg == "internal error"

@@ -1,6 +1,7 @@
 """Parity of the HIP engine with the oracle (and, through the golden
 fixtures, with the reference's own code): byte-identical tables and identical
 table metadata.  Every case calls the engine through the C-ABI."""
+import ctypes as C
 import hashlib
 import json
 import os
@@ -11,7 +12,8 @@ import pytest
 
 import oracle
 from luda_amd import ffi, jobs
-from tests.tiny_jobs import TinyJob, ikey, many_grandparents_job
+from tests.tiny_jobs import (TinyJob, ikey, interval_job, malformed_job, many_children_job, many_grandparents_job,
+                             wide_key_job)
 
 pytestmark = pytest.mark.gpu
 
@@ -55,10 +57,6 @@ def test_build_golden_corner_tables(eng):
             assert tables[0].data == f.read(), name
 
 
-@pytest.mark.skip(reason="faulted the GPU twice (aperture violation in k_write_blocks) before the cause was found: "
-                         "a flat load whose address register pointed 3 bytes below the LDS aperture; the writer no "
-                         "longer uses generic LDS pointers (tests/test_isa.py), but this exact case stays off the GPU "
-                         "until the next round")
 def test_build_30_entries_one_block(eng):
     """The first case that ran on the GPU: 30 entries in one 684-byte block,
     assembled by wave 0 at LDS offset 0, CRC pad of 83 bytes."""
@@ -183,3 +181,132 @@ def test_many_grandparents_match_oracle(eng, ngp):
     assert rc1 == 0 and rc2 == 0, msg
     assert len(want) > 10
     assert summary(out[0]) == summary(want)
+
+
+def oracle_build(e, o):
+    return [t.data for t in oracle.build_tables(e, o, "oracle")]
+
+
+def check_against_oracle(eng, tj, label):
+    job, keep = tj.make_job(oracle_build)
+    rc1, want = oracle.compact(job, "oracle")
+    rc2, out, msg = eng.compact(job)
+    assert rc1 == 0 and rc2 == 0, (label, msg)
+    assert summary(out[0]) == summary(want), label
+    return out
+
+
+@pytest.mark.parametrize("nl0", [6, 7, 9, 12, 13])
+def test_many_children_match_oracle(eng, nl0):
+    """Level-0 jobs of 6-13 merge children (3-4 merge rounds)."""
+    for seed in range(4):
+        print("children", nl0, seed, flush=True)
+        out = check_against_oracle(eng, many_children_job(nl0, 100 * nl0 + seed), (nl0, seed))
+        assert out[1]["entries_in"] > 0
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_wide_keys_match_oracle(eng, seed):
+    """User keys of 65-256 bytes: key planes of 9-32 words (the grouped
+    decoder and the untiled merge), six merge children."""
+    print("wide keys", seed, flush=True)
+    out = check_against_oracle(eng, wide_key_job(seed), seed)
+    assert out[1]["key_width"] > 64
+
+
+def test_key_over_256_bytes_is_left_to_the_cpu(eng):
+    tj = wide_key_job(9, lo=200, hi=257)
+    job, keep = tj.make_job(oracle_build)
+    assert max(len(k) - 8 for f in tj.files[0] + tj.files[1] for k, _ in f) == 257
+    rc, out, msg = eng.compact(job)
+    assert rc == ffi.LUDA_NOT_SUPPORTED, msg
+
+
+@pytest.mark.parametrize("block", range(2))
+def test_malformed_tags_match_oracle(eng, block):
+    """Entries whose tag type is above kTypeValue: kept, and the drop rule's
+    state is reset (db/db_impl.cc:1113-1119)."""
+    for seed in range(block * 60, (block + 1) * 60):
+        check_against_oracle(eng, malformed_job(seed), seed)
+
+
+@pytest.mark.parametrize("interval", [17, 31, 100, 127, 128, 200, 255])
+def test_restart_intervals_match_oracle(eng, interval):
+    """Restart groups of up to 255 entries in inputs and outputs."""
+    print("interval", interval, flush=True)
+    check_against_oracle(eng, interval_job(interval, interval), interval)
+
+
+@pytest.mark.parametrize("interval", [256, 1000])
+def test_count_spilling_into_restart_offset_is_corruption(eng, interval):
+    """A restart group of 256+ entries spills its count into the offset bits
+    (table/block_builder.cc:62-68,89-94); the fork's reader then fails with
+    Corruption (oracle/_ref agrees), and so does the engine."""
+    tj = interval_job(interval, interval)
+    job, keep = tj.make_job(oracle_build)
+    rc1, _ = oracle.compact(job, "oracle")
+    rc2, out, msg = eng.compact(job)
+    assert rc1 == ffi.LUDA_CORRUPTION and rc2 == ffi.LUDA_CORRUPTION, msg
+
+
+def small_job(seed=3):
+    tj = many_children_job(4, seed)
+    return tj, tj.make_job(oracle_build)
+
+
+def test_error_statuses(eng):
+    """Each status the caller maps onto leveldb::Status, with the context
+    usable afterwards."""
+    tj, (job, keep) = small_job()
+    # compression other than kNoCompression -> NOT_SUPPORTED
+    job.options.compression = 1
+    rc, _, msg = eng.compact(job)
+    assert rc == ffi.LUDA_NOT_SUPPORTED, msg
+    job.options.compression = 0
+    # block_restart_interval 0 -> INVALID_ARGUMENT; level out of range too
+    job.options.block_restart_interval = 0
+    assert eng.compact(job)[0] == ffi.LUDA_INVALID_ARGUMENT
+    job.options.block_restart_interval = 16
+    job.level = 7
+    assert eng.compact(job)[0] == ffi.LUDA_INVALID_ARGUMENT
+    job.level = 0
+    # still works
+    rc, out, msg = eng.compact(job)
+    assert rc == 0, msg
+    rc1, want = oracle.compact(job, "oracle")
+    assert summary(out[0]) == summary(want)
+
+
+def test_corrupt_inputs_are_corruption(eng):
+    """A broken footer magic, an index handle past the end and a truncated
+    table are CORRUPTION (Table::Open, table/table.cc:38-82; the caller then
+    runs its CPU path)."""
+    tj, (job, keep) = small_job(4)
+    t0 = bytearray(tj.tables[0][0])
+    cases = {
+        "magic": t0[:-1] + bytes([t0[-1] ^ 1]),
+        "truncated": t0[:-20],
+        "index_handle": t0[:-48] + bytes([0xff] * 8) + t0[-40:],
+    }
+    for name, bad in cases.items():
+        job2, keep2 = tj.make_job(oracle_build)
+        b = C.create_string_buffer(bytes(bad), len(bad))
+        keep2.append(b)
+        job2.inputs[0][0].data = C.cast(b, C.c_void_p)
+        job2.inputs[0][0].size = len(bad)
+        rc, _, msg = eng.compact(job2)
+        assert rc == ffi.LUDA_CORRUPTION, (name, rc, msg)
+        rc1, _ = oracle.compact(job2, "oracle")
+        assert rc1 == ffi.LUDA_CORRUPTION, name
+
+
+def test_job_over_budget_is_not_supported():
+    """A job whose working memory exceeds the context's HBM budget is left to
+    the CPU path."""
+    tj, (job, keep) = small_job(5)
+    e = jobs.Engine(0, budget=1 << 16)
+    try:
+        rc, _, msg = e.compact(job)
+        assert rc == ffi.LUDA_NOT_SUPPORTED and "budget" in msg, msg
+    finally:
+        e.close()

@@ -77,3 +77,41 @@ def test_reader_rejects_bad_crc():
     assert rc == ffi.LUDA_CORRUPTION
     rc, _ = oracle.table_entries(bytes(data), verify_crc=False)
     assert rc == 0
+
+
+with open(os.path.join(GOLD, "full_jobs.json")) as f:
+    FULL = json.load(f)
+with open(os.path.join(GOLD, "mix_jobs.json")) as f:
+    MIX = json.load(f)
+
+
+def oracle_outputs(cfg, seed, scale):
+    opts = ffi.default_options()
+    wl = jobs.Workload(cfg, seed=seed, scale=scale)
+    ins = ([], [])
+    for g in wl.groups:
+        ins[g.which].extend(t.data for t in jobs.build_group_tables(oracle.builder("oracle"), opts, g))
+    job = jobs.make_job(wl, opts, ins)
+    rc, outs = oracle.compact(job.job)
+    assert rc == 0
+    return job.bytes_in, outs
+
+
+def test_oracle_full_size_l0_job_matches_reference():
+    """Config 0 at its full size (20 MiB), every output table."""
+    rec = [r for r in FULL if r["config"] == 0][0]
+    nbytes, outs = oracle_outputs(0, rec["seed"], 1.0)
+    assert nbytes == rec["input_bytes"]
+    assert [sha(t.data) for t in outs] == [o["sha256"] for o in rec["outputs"]]
+    assert jobs.output_digest(outs) == rec["digest"]
+
+
+@pytest.mark.parametrize("k", [0, 2, 3, 6])
+def test_oracle_mix_jobs_match_reference(k):
+    """Mixed-batch jobs of every shape, including L0->L1 jobs with 4-12
+    level-0 tables (only jobs small enough for the CPU suite)."""
+    small = [(i, r) for i, r in enumerate(MIX["records"]) if r["input_bytes"] <= (24 << 20)]
+    i, rec = small[k]
+    nbytes, outs = oracle_outputs(rec["shape"], rec["seed"], rec["scale"])
+    assert nbytes == rec["input_bytes"]
+    assert jobs.output_digest(outs) == rec["digest"]

@@ -161,3 +161,143 @@ def many_grandparents_job(ngp, seed=5):
     tj.grandparents = [(ikey(users[i] + b"~", 7, 1), rng.choice([900, 2500, 6000])) for i in picks]
     tj.deeper = []
     return tj
+
+
+def _tj_shell(seed, level, opts):
+    tj = TinyJob(0)
+    tj.seed = seed
+    tj.level = level
+    tj.opts = opts
+    return tj
+
+
+def many_children_job(nl0, seed):
+    """A level-0 job with nl0 overlapping L0 files (each its own merge child,
+    db/version_set.cc:1245-1252) over a shared key set, plus 1-3 L1 files:
+    ceil(log2(nl0 + 1)) merge rounds, versions spread over many children."""
+    rng = random.Random(seed)
+    opts = ffi.default_options(block_size=rng.choice([128, 512, 4096]),
+                               block_restart_interval=rng.choice([1, 4, 16]),
+                               bloom_bits_per_key=rng.choice([0, 10]),
+                               max_file_size=rng.choice([3000, 20000, 1 << 20]))
+    tj = _tj_shell(seed, 0, opts)
+    users = [b"u%05d" % i for i in range(600)]
+    seq = 0
+    l1 = []
+    for u in sorted(rng.sample(users, 200)):
+        seq += 1
+        l1.append((u, seq, 1))
+    l1e = _entries(l1, rng)
+    files0 = []
+    for _ in range(nl0):
+        run = []
+        for u in rng.sample(users, rng.randint(10, 60)):
+            for _ in range(rng.randint(1, 2)):
+                seq += 1
+                run.append((u, seq, 0 if rng.random() < 0.2 else 1))
+        run.sort(key=lambda e: (e[0], -e[1]))
+        files0.append(_entries(run, rng))
+    tj.files = (files0, [p for p in _split(l1e, rng, rng.randint(1, 3)) if p])
+    tj.snapshot = rng.randint(seq // 2, seq + 1)
+    gk = sorted(rng.sample(users, rng.randint(0, 8)))
+    tj.grandparents = [(ikey(u, rng.randint(1, 50), 1), rng.choice([800, 3000, 9000])) for u in gk]
+    tj.deeper = []
+    if rng.random() < 0.5:
+        ks = sorted(rng.sample(users, 6))
+        tj.deeper = [(2, ks[0], ks[1]), (2, ks[2], ks[3]), (4, ks[4], ks[5])]
+    return tj
+
+
+def wide_key_job(seed, lo=65, hi=256, nl0=5):
+    """User keys of lo..hi bytes (key planes wider than 8 words: the grouped
+    decoder and the untiled merge) sharing long prefixes, over nl0 L0 files and
+    two L1 files."""
+    rng = random.Random(seed)
+    opts = ffi.default_options(block_size=rng.choice([512, 4096]),
+                               block_restart_interval=rng.choice([2, 16]),
+                               max_file_size=rng.choice([6000, 1 << 20]))
+    tj = _tj_shell(seed, 0, opts)
+    stems = [bytes(rng.randrange(256) for _ in range(rng.randint(lo - 8, lo))) for _ in range(4)]
+    users = set()
+    while len(users) < 120:
+        s = rng.choice(stems)
+        L = rng.randint(lo, hi)
+        users.add((s + bytes(rng.randrange(256) for _ in range(L)))[:L])
+    users = sorted(users)
+    seq = 0
+    l1 = []
+    for u in users[::2]:
+        seq += 1
+        l1.append((u, seq, 1))
+    files0 = []
+    for _ in range(nl0):
+        run = []
+        for u in rng.sample(users, 30):
+            seq += 1
+            run.append((u, seq, 0 if rng.random() < 0.2 else 1))
+        run.sort(key=lambda e: (e[0], -e[1]))
+        files0.append(_entries(run, rng))
+    l1e = _entries(l1, rng)
+    tj.files = (files0, [l1e[:len(l1e) // 2], l1e[len(l1e) // 2:]])
+    tj.snapshot = seq
+    tj.grandparents = []
+    tj.deeper = []
+    return tj
+
+
+MALFORMED_TYPES = (2, 3, 0x7F, 0x80, 0xFF)
+
+
+def malformed_job(seed):
+    """A TinyJob in which about a fifth of the entries carry a tag type above
+    kTypeValue: ParseInternalKey fails for them, so DoCompactionWork keeps
+    them and forgets the current user key (db/db_impl.cc:1113-1119)."""
+    tj = TinyJob(seed)
+    rng = random.Random(seed ^ 0x5A5A)
+
+    def spoil(ents):
+        out = []
+        for k, v in ents:
+            if rng.random() < 0.2:
+                tag = struct.unpack("<Q", k[-8:])[0]
+                k = k[:-8] + struct.pack("<Q", (tag & ~0xFF) | rng.choice(MALFORMED_TYPES))
+            out.append((k, v))
+        # the tag's type byte takes part in the order: re-sort by user key,
+        # then tag descending (db/dbformat.cc:51-67)
+        out.sort(key=lambda e: (e[0][:-8], -struct.unpack("<Q", e[0][-8:])[0]))
+        return out
+
+    tj.files = ([spoil(e) for e in tj.files[0]], [spoil(e) for e in tj.files[1]])
+    # files of one level must stay disjoint and sorted across the level
+    return tj
+
+
+def interval_job(seed, interval, block_size=8192):
+    """A level-1 job whose tables (inputs and outputs) use restart interval
+    `interval`, with hundreds of 10-20 byte entries per block, so restart
+    groups reach the interval."""
+    rng = random.Random(seed)
+    opts = ffi.default_options(block_size=block_size, block_restart_interval=interval,
+                               max_file_size=rng.choice([30000, 1 << 20]))
+    tj = _tj_shell(seed, 1, opts)
+    users = [b"k%06d" % i for i in range(4000)]
+    seq = 0
+    older, newer = [], []
+    for u in users:
+        if rng.random() < 0.7:
+            seq += 1
+            older.append((u, seq, 1))
+    for u in sorted(rng.sample(users, 1500)):
+        seq += 1
+        newer.append((u, seq, 0 if rng.random() < 0.1 else 1))
+
+    def ents(run):
+        return [(ikey(u, s, t), b"" if t == 0 else bytes(rng.randrange(256) for _ in range(rng.randint(0, 3))))
+                for u, s, t in run]
+
+    oe = ents(older)
+    tj.files = ([ents(newer)], [oe[:len(oe) // 2], oe[len(oe) // 2:]])
+    tj.snapshot = seq
+    tj.grandparents = []
+    tj.deeper = []
+    return tj
