@@ -189,6 +189,19 @@ void luda_free_result(luda_ctx* ctx, luda_result* res);
 int luda_verify_tables(luda_ctx* ctx, const luda_file* tables, uint32_t n, uint32_t flags, uint32_t* first_bad,
                        char* err, size_t errlen);
 
+/* luda_verify_tables for callers that cannot allocate memory (code under
+ * HIP graph capture, serving loops that forbid allocation).  It has no
+ * counterpart in the reference.  The call allocates nothing: its working
+ * memory is a fixed array in the library's code object, one per device (72
+ * MiB, held until the process ends), and the staging buffers luda_init
+ * reserved.  Tables must be device-resident (flags must hold
+ * LUDA_JOB_INPUTS_ON_DEVICE, else LUDA_INVALID_ARGUMENT); at most 4096 tables
+ * and 128 MiB of table bytes per call (LUDA_NOT_SUPPORTED beyond).  Results
+ * equal luda_verify_tables'.  Every call on one device shares that device's
+ * array, so calls that overlap on one device, from any context, race. */
+int luda_verify_tables_noalloc(luda_ctx* ctx, const luda_file* tables, uint32_t n, uint32_t flags,
+                               uint32_t* first_bad, char* err, size_t errlen);
+
 /* Batched and served compaction: jobs queued on the context run on worker
  * slots, each with its own HIP stream and working memory, so that the
  * kernels of several jobs overlap on the device.  A queued job starts when a

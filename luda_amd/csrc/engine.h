@@ -208,6 +208,9 @@ u64 scan_tiles_for(u64 n_cap);
 // The words for scans of up to n_cap items (the stream is idle when it grows).
 hipError_t scan_reserve(ScanState& S, u64 n_cap, hipStream_t s);
 void scan_release(ScanState& S);
+// Look-back words the caller owns (`tiles` u64 at `words`), zeroed on the
+// stream; such a state is never grown or released by the scans.
+hipError_t scan_attach(ScanState& S, u64* words, u64 tiles, hipStream_t s);
 u32 scan_next_epoch(ScanState& S, hipStream_t s);
 [[noreturn]] void throw_scan_capacity(u64 need, u64 have);
 // Exclusive scan of n = (n_dev ? *n_dev : 0) + n_add items (n <= n_cap);

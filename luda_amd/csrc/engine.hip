@@ -1054,6 +1054,109 @@ void verify_tables(luda_ctx* c, const luda_file* tables, u32 n, u32 flags, u32* 
 }
 
 // ---------------------------------------------------------------------------
+// luda_verify_tables_noalloc: luda_verify_tables with its working memory in
+// the code object.  Every device the library runs on holds its own copy of
+// `scratch` until the process ends; host code reaches it only through the
+// device address hipGetSymbolAddress returned for that device (noalloc_setup,
+// called once per device by luda_init).  Layout: the look-back words of the
+// call's one prefix sum, then the table descriptors, per-table index
+// handles and block counts, and the data block descriptors.  Calls that
+// overlap on one device share it and race.
+
+}  // namespace
+
+constexpr u64 kNoAllocScratch = 72ull << 20;
+constexpr u64 kNoAllocMaxBytes = 128ull << 20;  // sum of table sizes per call
+constexpr u32 kNoAllocMaxTables = 4096;
+constexpr u64 kNoAllocScanTiles = 4096;         // look-back words: scans of up to 16 Mi items
+static __device__ __align__(256) u8 scratch[kNoAllocScratch];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+struct NoAllocDev {
+  u8* ptr = nullptr;  // this device's copy of `scratch`
+  ScanState scans;    // look-back words at ptr, epochs advanced under mu
+};
+std::mutex g_noalloc_mu;
+NoAllocDev g_noalloc[kMaxDevices];
+
+// With `device` current: its copy's address, and the look-back words zeroed
+// (on stream s, which the caller synchronises).
+void noalloc_setup(int device, hipStream_t s) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  NoAllocDev& d = g_noalloc[device];
+  if (d.ptr) return;
+  void* p = nullptr;
+  HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) throw Fail{LUDA_IO_ERROR, "no device address for the scratch array"};
+  HIPCHK(scan_attach(d.scans, reinterpret_cast<u64*>(p), kNoAllocScanTiles, s));
+  HIPCHK(hipStreamSynchronize(s));
+  d.ptr = static_cast<u8*>(p);
+}
+
+void verify_tables_noalloc(luda_ctx* c, const luda_file* tables, u32 n, u32 flags, u32* first_bad) {
+  *first_bad = 0xffffffffu;
+  if (!(flags & LUDA_JOB_INPUTS_ON_DEVICE))
+    throw Fail{LUDA_INVALID_ARGUMENT, "luda_verify_tables_noalloc takes device-resident tables only"};
+  if (n == 0) return;
+  if (n > kNoAllocMaxTables) throw Fail{LUDA_NOT_SUPPORTED, "more than 4096 tables in one call"};
+  u64 bytes = 0, nb_cap = 0;
+  for (u32 i = 0; i < n; i++) {
+    bytes += tables[i].size;
+    nb_cap += tables[i].size / 41 + 1;
+  }
+  if (bytes > kNoAllocMaxBytes) throw Fail{LUDA_NOT_SUPPORTED, "more than 128 MiB of tables in one call"};
+  if (c->device < 0 || c->device >= kMaxDevices || !g_noalloc[c->device].ptr)
+    throw Fail{LUDA_IO_ERROR, "scratch array not set up on this device"};
+  NoAllocDev& D = g_noalloc[c->device];
+  FileDesc* d_files;
+  FileInfo* d_info;
+  u32 *d_nblk, *d_bbase, *d_nent;
+  BlockDesc* d_blks;
+  Arena a;
+  a.base = D.ptr + align_up(kNoAllocScanTiles * sizeof(u64), kAlign);
+  a.cap = kNoAllocScratch - align_up(kNoAllocScanTiles * sizeof(u64), kAlign);
+  d_files = a.take<FileDesc>(n);
+  d_info = a.take<FileInfo>(n);
+  d_nblk = a.take<u32>(n + 1);
+  d_bbase = a.take<u32>(n + 1);
+  d_blks = a.take<BlockDesc>(nb_cap);
+  d_nent = a.take<u32>(nb_cap + 1);
+  if (a.used > a.cap) throw Fail{LUDA_NOT_SUPPORTED, "tables need more block slots than the scratch array holds"};
+  // descriptors and the cleared status through the pinned staging buffer
+  // that luda_init reserved, written in place (no host array either)
+  if (c->pin2_busy) stream_sync(c);
+  const u64 o_st = align_up((u64)n * sizeof(FileDesc), 64);
+  if (o_st + sizeof(DevStatus) > c->pin2.cap) throw Fail{LUDA_IO_ERROR, "staging buffer too small"};
+  FileDesc* hd = reinterpret_cast<FileDesc*>(c->pin2.p);
+  for (u32 i = 0; i < n; i++) hd[i] = FileDesc{tables[i].data, tables[i].size};
+  DevStatus* hst = reinterpret_cast<DevStatus*>(c->pin2.p + o_st);
+  memset(hst, 0, sizeof(DevStatus));
+  hst->bad_file = 0xffffffffu;
+  KTimer T;  // no per-kernel timing: it would create events
+  T.s = c->s;
+  launch_stage_in(T, c->pin2.d, reinterpret_cast<u8*>(d_files), (u64)n * sizeof(FileDesc),
+                  reinterpret_cast<const DevStatus*>(c->pin2.d + o_st), c->st);
+  c->pin2_busy = true;
+  launch_files(T, d_files, n, d_info, d_nblk, c->st);
+  {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);  // the words' epoch is per device
+    scan_u32(T, D.scans, d_nblk, d_bbase, nullptr, n + 1, n + 1, nullptr, c->st);
+  }
+  launch_index(T, d_files, d_info, d_bbase, n, (u32)nb_cap, d_blks, d_nent, c->st);
+  launch_verify(T, d_files, n, d_blks, (u32)nb_cap, c->st);
+  HIPCHK(hipGetLastError());
+  launch_stage_out(T, c->pin.d, StageSeg{c->st, 0, sizeof(DevStatus)}, StageSeg{}, StageSeg{});
+  stream_sync(c);
+  DevStatus st;
+  memcpy(&st, c->pin.p, sizeof(DevStatus));
+  *first_bad = st.bad_file;
+  if (status_code_of(st) != LUDA_OK) throw Fail{status_code_of(st), "table rejected"};
+}
+
+// ---------------------------------------------------------------------------
 // luda_build_tables
 
 luda_result* build_tables(luda_ctx* c, const luda_options* opts, u64 n, const u8* keys, const u64* key_off,
@@ -1472,7 +1575,9 @@ int ctx_open(int device, uint64_t hbm_budget_bytes, luda_ctx** out, char* err, s
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));
     HIPCHK(c->pin.reserve(1 << 16));
+    HIPCHK(c->pin2.reserve(128 << 10));
     c->T.s = c->s;
+    noalloc_setup(device, c->s);
   });
   if (rc != LUDA_OK) {
     ctx_close(c);
@@ -1540,6 +1645,18 @@ int luda_verify_tables(luda_ctx* c, const luda_file* tables, uint32_t n, uint32_
   return guarded(c, err, errlen, [&] {
     fault_point("verify");
     verify_tables(c, tables, n, flags, first_bad);
+  });
+}
+
+int luda_verify_tables_noalloc(luda_ctx* c, const luda_file* tables, uint32_t n, uint32_t flags, uint32_t* first_bad,
+                               char* err, size_t errlen) {
+  if (!c || !first_bad || (n && !tables)) return LUDA_INVALID_ARGUMENT;
+  *first_bad = 0xffffffffu;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  return guarded(c, err, errlen, [&] {
+    fault_point("verify");
+    verify_tables_noalloc(c, tables, n, flags, first_bad);
   });
 }
 

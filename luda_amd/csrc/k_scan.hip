@@ -19,6 +19,13 @@ u32 grid_for(u64 n, u32 per_block) {
   return (u32)g;
 }
 
+// Zeroes the look-back words with a kernel, not a runtime memset, so that a
+// stream captured into a graph replays it as an ordinary kernel node.
+hipError_t zero_words(u64* p, u64 n, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill<u64>, dim3(grid_for(n, 256 * 4)), dim3(256), 0, s, p, (u64)0, n);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 u64 scan_tiles_for(u64 n_cap) { return std::max<u64>(1, (n_cap + lbscan::kTile - 1) / lbscan::kTile); }
@@ -37,11 +44,18 @@ hipError_t scan_reserve(ScanState& S, u64 n_cap, hipStream_t s) {
     S.words = nullptr;
     return e;
   }
-  e = hipMemsetAsync(S.words, 0, want * sizeof(u64), s);
+  e = zero_words(S.words, want, s);
   if (e != hipSuccess) return e;
   S.tiles = want;
   S.epoch = 0;
   return hipSuccess;
+}
+
+hipError_t scan_attach(ScanState& S, u64* words, u64 tiles, hipStream_t s) {
+  S.words = words;
+  S.tiles = tiles;
+  S.epoch = 0;
+  return zero_words(words, tiles, s);
 }
 
 void scan_release(ScanState& S) {
@@ -53,7 +67,7 @@ void scan_release(ScanState& S) {
 
 u32 scan_next_epoch(ScanState& S, hipStream_t s) {
   if (++S.epoch >= 0x4000) {
-    const hipError_t e = hipMemsetAsync(S.words, 0, S.tiles * sizeof(u64), s);
+    const hipError_t e = zero_words(S.words, S.tiles, s);
     if (e != hipSuccess) launch_refused("scan words reset", e);
     S.epoch = 1;
   }

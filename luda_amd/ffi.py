@@ -174,6 +174,8 @@ ENGINE_SIGNATURES = {
     "luda_free_result": (None, [C.c_void_p, RESULT_P]),
     "luda_verify_tables": (C.c_int, [C.c_void_p, C.POINTER(luda_file), C.c_uint32, C.c_uint32,
                                      C.POINTER(C.c_uint32), C.c_char_p, C.c_size_t]),
+    "luda_verify_tables_noalloc": (C.c_int, [C.c_void_p, C.POINTER(luda_file), C.c_uint32, C.c_uint32,
+                                             C.POINTER(C.c_uint32), C.c_char_p, C.c_size_t]),
     "luda_device_alloc": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
     "luda_device_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "luda_memcpy_h2d": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64]),
