@@ -92,6 +92,25 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": nbytes}), flush=True)
 
 
+def host_info():
+    """The CPU the baseline ran on: logical CPUs the OS reports, CPUs this
+    process may use, and the model name."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"nproc": os.cpu_count(), "usable_cpus": usable, "cpu_model": model}
+
+
 def mix_jobs(eng, opts, n, seed):
     """The mixed batch (luda_amd.jobs.mix_specs): n compaction jobs of
     log-uniform input sizes in [4, 256] MiB, half L0->L1 with 4-12 level-0
@@ -211,7 +230,10 @@ def mix_cpu_baseline(eng, opts, mix, gpu_hashes):
         seq_bytes += hjobs[k][0]
         k += 1
     t_seq = time.perf_counter() - t0
-    threads = max(1, min(16, os.cpu_count() or 1))
+    # every CPU this process may use, up to 16: the host CPUs one GPU's job
+    # gets on the measurement machines
+    host = host_info()
+    threads = max(1, min(16, host["usable_cpus"] or 1))
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=threads) as ex:
         list(ex.map(run, range(len(hjobs))))
@@ -219,7 +241,7 @@ def mix_cpu_baseline(eng, opts, mix, gpu_hashes):
     total = sum(b for b, _ in hjobs)
     log(f"CPU ({kind}): {k} jobs one after another {seq_bytes / t_seq / 1e9:.3f} GB/s, "
         f"all {len(hjobs)} on {threads} threads {total / t_par / 1e9:.3f} GB/s")
-    return {"kind": kind, "unit": "GB/s",
+    return {"kind": kind, "unit": "GB/s", "host": host,
             "sequential": {"value": round(seq_bytes / t_seq / 1e9, 4), "cores": 1, "jobs": k,
                            "input_bytes": seq_bytes, "same_tables_as_gpu": seq_same},
             "parallel": {"value": round(total / t_par / 1e9, 4), "cores": threads, "jobs": len(hjobs),
@@ -472,7 +494,7 @@ def main():
         cpu = {"value": round(bytes_in / tmed / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": kind,
                "sample": f"the whole job ({bytes_in / 2**20:.0f} MiB of input tables, median of {len(times)} runs, "
                          f"one thread, inputs and outputs in RAM)",
-               "ms_per_job": round(tmed * 1e3, 1)}
+               "ms_per_job": round(tmed * 1e3, 1), "host": host_info()}
         parity = [sha(t.data) for t in cpu_tables] == [sha(t) for t in gpu_tables]
         log(f"CPU ({kind}): {cpu['value']} GB/s; outputs identical: {parity}")
 

@@ -207,6 +207,8 @@ struct ScanGuard {
 u64 scan_tiles_for(u64 n_cap);
 // The words for scans of up to n_cap items (the stream is idle when it grows).
 hipError_t scan_reserve(ScanState& S, u64 n_cap, hipStream_t s);
+// The words for a look-back over `tiles` tiles of any size.
+hipError_t scan_reserve_tiles(ScanState& S, u64 tiles, hipStream_t s);
 void scan_release(ScanState& S);
 // Look-back words the caller owns (`tiles` u64 at `words`), zeroed on the
 // stream; such a state is never grown or released by the scans.
@@ -257,7 +259,18 @@ void launch_ingest(KTimer& T, u64 n, const u8* keys, const u64* key_off, const u
                    DevStatus* st);
 
 // k_merge.hip
-void launch_merge(KTimer& T, const MergePair* pairs, u32 npairs, u32 total_tiles, const u32* in, u32* out, Planes K);
+// One merge round; in == nullptr reads the children in place (first round,
+// key planes of at most 8 words), and check verifies each child is sorted.
+// splits: one u32 per merge tile of the round (working memory).
+void launch_merge(KTimer& T, const MergePair* pairs, u32 npairs, u32 total_tiles, const u32* in, u32* out, Planes K,
+                  bool check, u32* splits, DevStatus* st);
+// The last round (one pair; W <= 8) fused with the drop rule, the kept-count
+// scan and the gather into the kept stream; with grandparents also gcnt,
+// kpos and the crossing slots (cleared to all ones before) for launch_stops.
+// Sets st->n_kept.
+void launch_merge_select(KTimer& T, ScanState& S, const MergePair* d_pair, MergePair pr, const u32* in, bool check,
+                         Planes K, const u64* vptr, const u32* vlen, const DropParams& dp, Kept out, u32* gcnt,
+                         u32* kpos, u32* slot, u32* splits, DevStatus* st);
 u32 merge_items_per_tile(u32 W);  // outputs per merge tile for key planes of W words
 // also writes iota[e] = e, the merge's starting permutation
 void launch_check_sorted(KTimer& T, Planes K, u64 n, const u32* child_start, u32 nchild, u32* iota, DevStatus* st);

@@ -508,7 +508,8 @@ u64 kernel_bytes(const std::string& name, const luda_stats& s, u32 W, u64 rounds
   const u64 soa = 8ull * W + 4 + 8 + 8 + 4;
   const u64 tail = s.bytes_written > s.data_block_bytes ? s.bytes_written - s.data_block_bytes : 0;
   if (name == "decode") return s.bytes_read + n * soa;               // input tables in, planes out
-  if (name == "merge") return rounds * n * (4 + 4 + 8ull * W + 12);  // per round: perm in/out + keys
+  if (name == "merge") return (rounds ? rounds - 1 : 0) * n * (4 + 4 + 8ull * W + 12);  // per round: perm in/out + keys
+  if (name == "merge_select") return n * (8ull * W + 12 + 12) + m * soa;  // keys and value refs in, kept stream out
   if (name == "drop") return n * (4 + 4 + 8 + 8ull * W + 4);
   if (name == "gather") return n * 8 + 2 * m * soa;
   if (name == "entry_sizes") return m * (2 * (8ull * W + 12) + 12);  // two keys in, three sizes out
@@ -813,7 +814,12 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     }
     std::vector<MergePair> allp;
     for (auto& r : rounds) allp.insert(allp.end(), r.begin(), r.end());
-    if (allp.empty()) allp.push_back(MergePair{});  // no rounds: a placeholder keeps the region non-empty
+    // the last round's pair again, at the end, for the fused last round (a
+    // single run is merged with an empty one)
+    MergePair fin{0, 0, 0, 0};
+    if (!rounds.empty()) fin = rounds.back().front();
+    else if (!runs.empty()) fin = MergePair{runs[0].first, runs[0].second, runs[0].second, 0};
+    allp.push_back(fin);
     std::vector<u8> blob;
     auto put = [&](const void* p, u64 bytes) {
       const u64 off = align_up(blob.size(), 16);
@@ -829,7 +835,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     // carve phase B
     Planes K;
     u64 *vptr, *gp_words, *gp_tag, *gp_prefix, *deep_words;
-    u32 *vlen, *permA, *permB, *keep, *kpos, *gcnt, *events, *gp_len, *deep_len, *d_cstart;
+    u32 *vlen, *permA, *permB, *keep, *kpos, *gcnt, *events, *gp_len, *deep_len, *d_cstart, *splits;
     MergePair* d_pairs;
     u8* d_tab;
     EncodeArrays EA;
@@ -847,6 +853,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
       kpos = a.take<u32>(n);
       gcnt = a.take<u32>(ngp ? n : 1);
       events = a.take<u32>(ngp + 2);
+      splits = a.take<u32>((u64)n / merge_items_per_tile(W) + runs.size() + 2);
       d_tab = a.take<u8>(blob.size());
       EA = carve_encode(a, es, W, true);
     };
@@ -895,10 +902,32 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
       HIPCHK(scan_reserve(c->scans,
                           std::max<u64>(std::max<u64>(n, es.m_cap), std::max<u64>(es.nbo_cap, std::max(es.nf_cap, es.nw_cap))) + 1,
                           c->s));
+      // the fused last merge round looks back over its merge tiles
+      HIPCHK(scan_reserve_tiles(c->scans, ((u64)n + merge_items_per_tile(W) - 1) / merge_items_per_tile(W) + 1, c->s));
       launch_decode(T, d_files, d_blks, nb, d_ebase, K, vptr, vlen, c->st, stA.max_blk, stA.max_nr, stA.max_ent);
-      launch_check_sorted(T, K, n, d_cstart, (u32)cstart.size() - 1, permA, c->st);
-      const u32* perm = permA;
-      {
+      if (W <= 8) {
+        // Every round but the last writes a permutation (the first reads the
+        // children in place and checks they are sorted); the last round
+        // selects and writes the kept stream itself.
+        const u32* in = nullptr;
+        u32* bufs[2] = {permA, permB};
+        u32 nbuf = 0;
+        u64 poff = 0;
+        const u32 items = merge_items_per_tile(W);
+        for (size_t ri = 0; ri + 1 < rounds.size(); ri++) {
+          const auto& r = rounds[ri];
+          const MergePair& last = r.back();
+          const u32 tiles = last.tile0 + (last.b1 - last.a0 + items - 1) / items;
+          launch_merge(T, d_pairs + poff, (u32)r.size(), tiles, in, bufs[nbuf], K, ri == 0, splits, c->st);
+          in = bufs[nbuf];
+          nbuf ^= 1;
+          poff += r.size();
+        }
+        if (ngp) fill_u32(T, events, 0xffffffffu, (u64)ngp + 1);  // no crossing yet
+        launch_merge_select(T, c->scans, d_pairs + (allp.size() - 1), fin, in, rounds.size() <= 1, K, vptr, vlen, dp,
+                            EA.kept, gcnt, kpos, events, splits, c->st);
+      } else {
+        launch_check_sorted(T, K, n, d_cstart, (u32)cstart.size() - 1, permA, c->st);
         u32* in = permA;
         u32* outp = permB;
         u64 poff = 0;
@@ -906,16 +935,14 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
           const MergePair& last = r.back();
           const u32 items = merge_items_per_tile(W);
           const u32 tiles = last.tile0 + (last.b1 - last.a0 + items - 1) / items;
-          // runs not touched by this round keep their positions: copy them
-          launch_merge(T, d_pairs + poff, (u32)r.size(), tiles, in, outp, K);
+          launch_merge(T, d_pairs + poff, (u32)r.size(), tiles, in, outp, K, false, splits, c->st);
           poff += r.size();
           std::swap(in, outp);
         }
-        perm = in;
+        launch_drop(T, in, n, K, dp, keep, gcnt, events, c->st);
+        scan_keep(T, c->scans, keep, kpos, n, dp, gcnt, events, ngp, c->st);
+        launch_gather(T, in, n, keep, kpos, K, vptr, vlen, EA.kept);
       }
-      launch_drop(T, perm, n, K, dp, keep, gcnt, events, c->st);
-      scan_keep(T, c->scans, keep, kpos, n, dp, gcnt, events, ngp, c->st);
-      launch_gather(T, perm, n, keep, kpos, K, vptr, vlen, EA.kept);
       if (ngp) launch_stops(T, gcnt, kpos, keep, n, dp, events, ngp, EA.kept, c->st);
       run_encode(c, es, EA, (u8*)out);
     }

@@ -30,8 +30,9 @@ hipError_t zero_words(u64* p, u64 n, hipStream_t s) {
 
 u64 scan_tiles_for(u64 n_cap) { return std::max<u64>(1, (n_cap + lbscan::kTile - 1) / lbscan::kTile); }
 
-hipError_t scan_reserve(ScanState& S, u64 n_cap, hipStream_t s) {
-  const u64 tiles = scan_tiles_for(n_cap);
+hipError_t scan_reserve(ScanState& S, u64 n_cap, hipStream_t s) { return scan_reserve_tiles(S, scan_tiles_for(n_cap), s); }
+
+hipError_t scan_reserve_tiles(ScanState& S, u64 tiles, hipStream_t s) {
   if (tiles <= S.tiles) return hipSuccess;
   if (S.words) {
     const hipError_t e = hipStreamSynchronize(s);  // no scan of the stream still reads the old words

@@ -15,35 +15,9 @@
 // parallel and the short sequential budget walk runs over them alone.
 #include "engine.h"
 #include "scan.h"
+#include "select.h"
 
 namespace {
-
-// Internal-key compare of plane entry e against a stored key.
-__device__ __forceinline__ int ikey_cmp_fixed(const Planes& K, u64 e, const u64* words, u32 len, u64 tag) {
-  int c = ukey_cmp_fixed(K, e, words, len);
-  if (c) return c;
-  const u64 te = K.tag[e];
-  if (te != tag) return te > tag ? -1 : 1;
-  return 0;
-}
-
-__device__ bool is_base_level_for_key(const Planes& K, u64 e, const DropParams& dp) {
-  for (u32 L = dp.first_level; L < 7; L++) {
-    const u32 b = dp.lvl_begin[L], en = dp.lvl_end[L];
-    if (b >= en) continue;
-    u32 lo = b, hi = en;
-    while (lo < hi) {
-      const u32 m = (lo + hi) >> 1;
-      if (ukey_cmp_fixed(K, e, dp.deep_words + (2ull * m + 1) * K.W, dp.deep_len[2 * m + 1]) > 0) lo = m + 1;
-      else hi = m;
-    }
-    if (lo < en && ukey_cmp_fixed(K, e, dp.deep_words + 2ull * lo * K.W, dp.deep_len[2 * lo]) >= 0) return false;
-  }
-  return true;
-}
-
-// slot[g] of the grandparent crossings (g = 1..ngp): no crossing yet
-constexpr u32 kNoEvent = 0xffffffffu;
 
 __global__ void k_drop(const u32* perm, u32 n, Planes K, DropParams dp, u32* keep, u32* gcnt, u32* slot) {
   if (dp.ngp) GRID_STRIDE(g, (u64)dp.ngp + 1) slot[g] = kNoEvent;  // filled by the keep scan
@@ -62,16 +36,7 @@ __global__ void k_drop(const u32* perm, u32 n, Planes K, DropParams dp, u32* kee
       if (!drop && type == 0 && seq <= dp.S && is_base_level_for_key(K, e, dp)) drop = true;
     }
     keep[p] = drop ? 0u : 1u;
-    if (dp.ngp) {
-      // number of grandparents whose largest key is below this key
-      u32 lo = 0, hi = dp.ngp;
-      while (lo < hi) {
-        const u32 m = (lo + hi) >> 1;
-        if (ikey_cmp_fixed(K, e, dp.gp_words + (u64)m * K.W, dp.gp_len[m], dp.gp_tag[m]) > 0) lo = m + 1;
-        else hi = m;
-      }
-      gcnt[p] = lo;
-    }
+    if (dp.ngp) gcnt[p] = grandparents_below(K, e, dp);
   }
 }
 
