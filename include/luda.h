@@ -234,6 +234,100 @@ int luda_poll(luda_ctx* ctx);
 /* Waits for every submitted job and runs its callback; returns how many. */
 int luda_drain(luda_ctx* ctx);
 
+/* ---- File I/O around a compaction ------------------------------------------
+ * The reference reads each input file whole with fread (cuda/cuda_common.h:
+ * 302-311) and writes each output on a pthread of its own with fwrite and
+ * fsync (db/db_impl.cc:55-65,1490-1565).  Here input files go from disk
+ * through two pinned chunk buffers into HBM on the context's I/O stream, a
+ * stream of its own: reading chunk k+1 from disk overlaps the copy of chunk
+ * k, and both overlap kernels a compaction on another thread is running.
+ * Outputs go from HBM to pinned memory in one copy, then to their files on up
+ * to 16 writer threads. */
+typedef struct luda_file_set {
+  uint32_t n;
+  uint32_t reserved;
+  luda_file* files;     /* files[i].data: device address of file i, .size, .number */
+  void* device_data;    /* the one device buffer holding them */
+  uint64_t total_bytes; /* sum of file sizes */
+  double read_ms;       /* whole call */
+} luda_file_set;
+/* Reads `n` table files into HBM (numbers[i] goes to files[i].number; numbers
+ * may be NULL).  The result feeds luda_compact with LUDA_JOB_INPUTS_ON_DEVICE;
+ * free it with luda_free_files.  LUDA_IO_ERROR when a file cannot be read. */
+int luda_read_files(luda_ctx* ctx, const char* const* paths, const uint64_t* numbers, uint32_t n,
+                    luda_file_set** out, char* err, size_t errlen);
+void luda_free_files(luda_ctx* ctx, luda_file_set* set);
+/* Writes output table i of `res` to paths[i] (created or truncated), with
+ * fsync when sync is non-zero, as FinishCompactionOutputFile's Sync/Close do
+ * (db/db_impl.cc:1000-1014).  Works for results kept on the device and for
+ * host results. */
+int luda_write_outputs(luda_ctx* ctx, const luda_result* res, const char* const* paths, int sync, char* err,
+                       size_t errlen);
+
+/* ---- The install record ------------------------------------------------------
+ * A VersionEdit as VersionEdit::EncodeTo serialises it (db/version_edit.cc:
+ * 41-89), and the MANIFEST log record that carries it (log::Writer::AddRecord,
+ * db/log_writer.cc:35-105).  Host-only; no context needed. */
+#define LUDA_EDIT_HAS_COMPARATOR 0x1u
+#define LUDA_EDIT_HAS_LOG_NUMBER 0x2u
+#define LUDA_EDIT_HAS_PREV_LOG_NUMBER 0x4u
+#define LUDA_EDIT_HAS_NEXT_FILE 0x8u
+#define LUDA_EDIT_HAS_LAST_SEQUENCE 0x10u
+typedef struct luda_edit_pointer {
+  int32_t level;
+  uint32_t reserved;
+  luda_slice key; /* an internal key */
+} luda_edit_pointer;
+typedef struct luda_edit_deleted {
+  int32_t level;
+  uint32_t reserved;
+  uint64_t number;
+} luda_edit_deleted;
+typedef struct luda_edit_file {
+  int32_t level;
+  uint32_t reserved;
+  uint64_t number;
+  uint64_t file_size;
+  luda_slice smallest; /* internal keys */
+  luda_slice largest;
+} luda_edit_file;
+/* VersionEdit's fields (db/version_edit.h:86-100): the deleted files are a
+ * set (sorted by level, then number; duplicates once), the others keep their
+ * order. */
+typedef struct luda_edit {
+  uint32_t has; /* LUDA_EDIT_HAS_* */
+  uint32_t num_compact_pointers;
+  luda_slice comparator;
+  uint64_t log_number;
+  uint64_t prev_log_number;
+  uint64_t next_file_number;
+  uint64_t last_sequence;
+  const luda_edit_pointer* compact_pointers;
+  const luda_edit_deleted* deleted;
+  uint32_t num_deleted;
+  uint32_t num_new_files;
+  const luda_edit_file* new_files;
+} luda_edit;
+/* Serialises e into dst[0 .. *len).  If cap is too small, returns
+ * LUDA_INVALID_ARGUMENT with *len = the bytes needed. */
+int luda_encode_edit(const luda_edit* e, uint8_t* dst, uint64_t cap, uint64_t* len);
+/* The edit DBImpl::InstallCompactionResults applies (db/db_impl.cc:1040-1056):
+ * every input file deleted from its level (Compaction::AddInputDeletions,
+ * db/version_set.cc:1527-1533), then output i added at level + 1 as file
+ * output_numbers[i] with its size and boundary keys.  `base` (may be NULL)
+ * supplies the header fields and compact pointers LogAndApply and
+ * SetupOtherInputs put into the same edit. */
+int luda_encode_compaction_edit(const luda_job* job, const luda_result* res, const uint64_t* output_numbers,
+                                const luda_edit* base, uint8_t* dst, uint64_t cap, uint64_t* len);
+/* Appends `payload` as one log record to a log whose current block offset is
+ * *block_offset (the file length mod 32768), updating it: the record split
+ * into FULL/FIRST/MIDDLE/LAST fragments with 7-byte headers (masked CRC-32C
+ * of type and data, length, type) and block trailers zero-filled.  Needs at
+ * most n + 7 * (n / 32761 + 2) + 6 bytes; returns LUDA_INVALID_ARGUMENT with
+ * *len = the bytes needed if cap is too small. */
+int luda_log_record(uint64_t* block_offset, const uint8_t* payload, uint64_t n, uint8_t* dst, uint64_t cap,
+                    uint64_t* len);
+
 /* Device-buffer helpers so that a caller with no HIP code of its own can keep
  * inputs resident in HBM (LUDA_JOB_INPUTS_ON_DEVICE) and read outputs back. */
 int luda_device_alloc(luda_ctx* ctx, uint64_t bytes, void** dptr);

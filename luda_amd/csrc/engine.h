@@ -76,6 +76,16 @@ struct KTimer {
   } while (0)
 
 // ---------------------------------------------------------------------------
+// what io.hip needs of a context: its device and the slot for its I/O state
+// (created on first use, closed with the context)
+
+struct luda_ctx;
+struct IoState;
+int ctx_device_of(luda_ctx* c);
+IoState** ctx_io_slot(luda_ctx* c);
+void io_close(IoState* io);
+
+// ---------------------------------------------------------------------------
 // input side
 
 struct FileDesc {

@@ -226,7 +226,11 @@ struct luda_ctx {
     }
   }
   std::map<void*, u64> pool_sizes;
+  IoState* io = nullptr;  // file I/O stream and staging (io.hip), on first use
 };
+
+int ctx_device_of(luda_ctx* c) { return c->device; }
+IoState** ctx_io_slot(luda_ctx* c) { return &c->io; }
 
 namespace {
 
@@ -1617,6 +1621,8 @@ int ctx_open(int device, uint64_t hbm_budget_bytes, luda_ctx** out, char* err, s
 void ctx_close(luda_ctx* c) {
   if (!c) return;
   if (c->s) hipStreamSynchronize(c->s);
+  if (c->io) io_close(c->io);
+  c->io = nullptr;
   c->in_buf.release();
   c->a_buf.release();
   c->b_buf.release();

@@ -161,6 +161,41 @@ class lw_workload(C.Structure):
 
 RESULT_P = C.POINTER(luda_result)
 
+
+class luda_file_set(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("reserved", C.c_uint32), ("files", C.POINTER(luda_file)),
+                ("device_data", C.c_void_p), ("total_bytes", C.c_uint64), ("read_ms", C.c_double)]
+
+
+FILESET_P = C.POINTER(luda_file_set)
+
+LUDA_EDIT_HAS_COMPARATOR = 0x1
+LUDA_EDIT_HAS_LOG_NUMBER = 0x2
+LUDA_EDIT_HAS_PREV_LOG_NUMBER = 0x4
+LUDA_EDIT_HAS_NEXT_FILE = 0x8
+LUDA_EDIT_HAS_LAST_SEQUENCE = 0x10
+
+
+class luda_edit_pointer(C.Structure):
+    _fields_ = [("level", C.c_int32), ("reserved", C.c_uint32), ("key", luda_slice)]
+
+
+class luda_edit_deleted(C.Structure):
+    _fields_ = [("level", C.c_int32), ("reserved", C.c_uint32), ("number", C.c_uint64)]
+
+
+class luda_edit_file(C.Structure):
+    _fields_ = [("level", C.c_int32), ("reserved", C.c_uint32), ("number", C.c_uint64), ("file_size", C.c_uint64),
+                ("smallest", luda_slice), ("largest", luda_slice)]
+
+
+class luda_edit(C.Structure):
+    _fields_ = [("has", C.c_uint32), ("num_compact_pointers", C.c_uint32), ("comparator", luda_slice),
+                ("log_number", C.c_uint64), ("prev_log_number", C.c_uint64), ("next_file_number", C.c_uint64),
+                ("last_sequence", C.c_uint64), ("compact_pointers", C.POINTER(luda_edit_pointer)),
+                ("deleted", C.POINTER(luda_edit_deleted)), ("num_deleted", C.c_uint32),
+                ("num_new_files", C.c_uint32), ("new_files", C.POINTER(luda_edit_file))]
+
 BUILD_ARGTYPES = [C.POINTER(luda_options), C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
 
 # Every symbol include/luda.h declares, with its ctypes signature.
@@ -185,6 +220,16 @@ ENGINE_SIGNATURES = {
     "luda_submit": (C.c_int, [C.c_void_p, C.POINTER(luda_job), C.c_void_p, C.c_void_p]),
     "luda_submit_build": (C.c_int, [C.c_void_p, C.POINTER(luda_options), C.c_uint64, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "luda_read_files": (C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), C.POINTER(C.c_uint64), C.c_uint32,
+                                  C.POINTER(FILESET_P), C.c_char_p, C.c_size_t]),
+    "luda_free_files": (None, [C.c_void_p, FILESET_P]),
+    "luda_write_outputs": (C.c_int, [C.c_void_p, RESULT_P, C.POINTER(C.c_char_p), C.c_int, C.c_char_p, C.c_size_t]),
+    "luda_encode_edit": (C.c_int, [C.POINTER(luda_edit), C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    "luda_encode_compaction_edit": (C.c_int, [C.POINTER(luda_job), RESULT_P, C.POINTER(C.c_uint64),
+                                              C.POINTER(luda_edit), C.c_void_p, C.c_uint64,
+                                              C.POINTER(C.c_uint64)]),
+    "luda_log_record": (C.c_int, [C.POINTER(C.c_uint64), C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
+                                  C.POINTER(C.c_uint64)]),
     "luda_poll": (C.c_int, [C.c_void_p]),
     "luda_drain": (C.c_int, [C.c_void_p]),
 }

@@ -247,14 +247,17 @@ class Engine:
             if not keep_on_device:
                 self.lib.luda_free_result(self.ctx, res)
 
-    def compact(self, job, keep_on_device=False):
+    def compact(self, job, keep_on_device=False, keep_result=False):
         """(status, (tables, stats, result_ptr) | None, message).  With
-        keep_on_device the result stays alive; free it with free()."""
+        keep_on_device (outputs left in HBM) or keep_result (host copies read,
+        result kept) the result stays alive; free it with free()."""
         res = ffi.RESULT_P()
         err = C.create_string_buffer(512)
         if keep_on_device:
             job.flags |= ffi.LUDA_JOB_KEEP_ON_DEVICE
         rc = self.lib.luda_compact(self.ctx, C.byref(job), C.byref(res), err, 512)
+        if keep_result and rc == ffi.LUDA_OK:
+            return rc, (result_tables(res), _stats_dict(res.contents.stats), res), ""
         return self._finish(rc, res, err, keep_on_device)
 
     def build_tables(self, opts_p, n, keys, key_off, vals, val_off, keep_on_device=False):
@@ -385,6 +388,26 @@ class Engine:
         vb = C.create_string_buffer(vals, len(vals) + 1)
         return self.build_tables(C.byref(opts), n, C.cast(kb, C.c_void_p), C.cast(ko, C.c_void_p),
                                  C.cast(vb, C.c_void_p), C.cast(vo, C.c_void_p))
+
+    # -- file I/O (luda_read_files, luda_write_outputs) --------------------
+    def read_files(self, paths, numbers=None):
+        """(status, luda_file_set pointer | None, message); free the set with
+        free_files."""
+        arr = (C.c_char_p * max(1, len(paths)))(*[p.encode() for p in paths])
+        nums = (C.c_uint64 * max(1, len(paths)))(*(numbers or [])) if numbers else None
+        out = ffi.FILESET_P()
+        err = C.create_string_buffer(512)
+        rc = self.lib.luda_read_files(self.ctx, arr, nums, len(paths), C.byref(out), err, 512)
+        return rc, (out if rc == ffi.LUDA_OK else None), err.value.decode()
+
+    def free_files(self, fs):
+        self.lib.luda_free_files(self.ctx, fs)
+
+    def write_outputs(self, res, paths, sync=True):
+        arr = (C.c_char_p * max(1, len(paths)))(*[p.encode() for p in paths])
+        err = C.create_string_buffer(512)
+        rc = self.lib.luda_write_outputs(self.ctx, res, arr, 1 if sync else 0, err, 512)
+        return rc, err.value.decode()
 
     def device_alloc(self, nbytes):
         p = C.c_void_p()

@@ -75,6 +75,11 @@ def load_ref():
     if _ref is None:
         lib = C.CDLL(REF_LIB)
         _bind_common(lib, "ref_")
+        lib.ref_encode_edit.restype = C.c_int
+        lib.ref_encode_edit.argtypes = [C.POINTER(ffi.luda_edit), C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]
+        lib.ref_log_record.restype = C.c_int
+        lib.ref_log_record.argtypes = [C.POINTER(C.c_uint64), C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
+                                       C.POINTER(C.c_uint64)]
         _ref = lib
     return _ref
 

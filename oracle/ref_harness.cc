@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "db/dbformat.h"
+#include "db/log_writer.h"
 #include "db/table_cache.h"
 #include "db/version_edit.h"
 #include "db/version_set.h"
@@ -359,6 +360,53 @@ int ref_compact(const luda_job* job, luda_result** out) {
   *out = res.Finish(bytes_read);
   (*out)->stats.entries_in = total;
   (*out)->stats.entries_kept = kept;
+  return LUDA_OK;
+}
+
+// VersionEdit::EncodeTo (db/version_edit.cc:41-89) of an edit described by
+// a luda_edit, built with VersionEdit's own setters.
+int ref_encode_edit(const luda_edit* e, uint8_t* dst, uint64_t cap, uint64_t* len) {
+  leveldb::VersionEdit ve;
+  if (e->has & LUDA_EDIT_HAS_COMPARATOR)
+    ve.SetComparatorName(Slice((const char*)e->comparator.data, e->comparator.size));
+  if (e->has & LUDA_EDIT_HAS_LOG_NUMBER) ve.SetLogNumber(e->log_number);
+  if (e->has & LUDA_EDIT_HAS_PREV_LOG_NUMBER) ve.SetPrevLogNumber(e->prev_log_number);
+  if (e->has & LUDA_EDIT_HAS_NEXT_FILE) ve.SetNextFile(e->next_file_number);
+  if (e->has & LUDA_EDIT_HAS_LAST_SEQUENCE) ve.SetLastSequence(e->last_sequence);
+  for (uint32_t i = 0; i < e->num_compact_pointers; i++) {
+    leveldb::InternalKey k;
+    k.DecodeFrom(Slice((const char*)e->compact_pointers[i].key.data, e->compact_pointers[i].key.size));
+    ve.SetCompactPointer(e->compact_pointers[i].level, k);
+  }
+  for (uint32_t i = 0; i < e->num_deleted; i++) ve.DeleteFile(e->deleted[i].level, e->deleted[i].number);
+  for (uint32_t i = 0; i < e->num_new_files; i++) {
+    const luda_edit_file& f = e->new_files[i];
+    leveldb::InternalKey a, b;
+    a.DecodeFrom(Slice((const char*)f.smallest.data, f.smallest.size));
+    b.DecodeFrom(Slice((const char*)f.largest.data, f.largest.size));
+    ve.AddFile(f.level, f.number, f.file_size, a, b);
+  }
+  std::string out;
+  ve.EncodeTo(&out);
+  *len = out.size();
+  if (out.size() > cap) return LUDA_INVALID_ARGUMENT;
+  memcpy(dst, out.data(), out.size());
+  return LUDA_OK;
+}
+
+// log::Writer::AddRecord (db/log_writer.cc:35-80) of one record appended to a
+// log of length *block_offset (only its offset in the block matters).
+int ref_log_record(uint64_t* block_offset, const uint8_t* payload, uint64_t n, uint8_t* dst, uint64_t cap,
+                   uint64_t* len) {
+  StringSink sink;
+  {
+    leveldb::log::Writer w(&sink, *block_offset);
+    w.AddRecord(Slice((const char*)payload, n));
+  }
+  *len = sink.contents.size();
+  if (sink.contents.size() > cap) return LUDA_INVALID_ARGUMENT;
+  memcpy(dst, sink.contents.data(), sink.contents.size());
+  *block_offset = (*block_offset + sink.contents.size()) % 32768;
   return LUDA_OK;
 }
 

@@ -84,3 +84,9 @@ def test_exceptions_never_cross_the_abi():
     assert rc == ffi.LUDA_IO_ERROR and not ctx and "injected" in msg
     rc, ctx, msg = _init_with_fault("other")
     assert rc == ffi.LUDA_IO_ERROR and not ctx and msg == "internal error"
+
+
+def test_edit_layouts():
+    assert C.sizeof(ffi.luda_edit) == 88
+    assert C.sizeof(ffi.luda_edit_file) == 56
+    assert C.sizeof(ffi.luda_file_set) == 40
