@@ -202,6 +202,29 @@ int luda_verify_tables(luda_ctx* ctx, const luda_file* tables, uint32_t n, uint3
 int luda_verify_tables_noalloc(luda_ctx* ctx, const luda_file* tables, uint32_t n, uint32_t flags,
                                uint32_t* first_bad, char* err, size_t errlen);
 
+/* ---- Reading tables ------------------------------------------------------------
+ * Every entry of the given tables, in order, as Table::Open and a full
+ * iteration read them (table/table.cc:38-214: the footer, the index block, a
+ * TwoLevelIterator over each data block's Block::Iter, table/block.cc:
+ * 224-251), decoded on the device and returned as plain host arrays.  With
+ * LUDA_READ_VERIFY every block's checksum is checked first, as luda_verify_
+ * tables does (the fork's ReadBlock skips it, table/format.cc:88); a failure
+ * is LUDA_CORRUPTION.  `flags` also takes LUDA_JOB_INPUTS_ON_DEVICE. */
+#define LUDA_READ_VERIFY 0x4u
+typedef struct luda_entries {
+  uint64_t n;              /* entries */
+  uint32_t num_tables;
+  uint32_t reserved;
+  uint8_t* keys;           /* entry i's internal key: keys[key_off[i] .. key_off[i+1]) */
+  uint64_t* key_off;       /* n + 1 */
+  uint8_t* vals;           /* entry i's value: vals[val_off[i] .. val_off[i+1]) */
+  uint64_t* val_off;       /* n + 1 */
+  uint64_t* table_first;   /* num_tables + 1: first entry of each table, then n */
+} luda_entries;
+int luda_read_tables(luda_ctx* ctx, const luda_file* tables, uint32_t n, uint32_t flags, luda_entries** out,
+                     char* err, size_t errlen);
+void luda_free_entries(luda_entries* e);
+
 /* Batched and served compaction: jobs queued on the context run on worker
  * slots, each with its own HIP stream and working memory, so that the
  * kernels of several jobs overlap on the device.  A queued job starts when a

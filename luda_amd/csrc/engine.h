@@ -342,3 +342,9 @@ void launch_write_filters(KTimer& T, Kept k, const u32* nw_dev, u32 nw_cap, cons
                           u8* out, u64 out_cap, DevStatus* st);
 void launch_write_tail(KTimer& T, Kept k, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b, u8* out,
                        u64 out_cap, DevStatus* st);
+
+// k_read.hip: ksz[e] = internal-key bytes of entry e (e < n), ksz[n] = vlen[n]
+// = 0; then the entries to plain arrays
+void launch_entry_out_sizes(KTimer& T, Planes K, u64 n, u32* ksz, u32* vlen);
+void launch_emit(KTimer& T, Planes K, const u64* vptr, const u32* vlen, u64 n, const u64* key_off, const u64* val_off,
+                 u8* keys, u8* vals);

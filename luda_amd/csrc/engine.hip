@@ -37,6 +37,7 @@ constexpr u32 kMaxChunk = 8192;       // orbit chunk: its LDS footprint is 5 byt
 constexpr u32 kMaxAttempts = 4;
 
 u64 align_up(u64 x, u64 a) { return (x + a - 1) / a * a; }
+constexpr u64 align_up_c(u64 x, u64 a) { return (x + a - 1) / a * a; }
 
 double now_ms() {
   using namespace std::chrono;
@@ -1085,6 +1086,177 @@ void verify_tables(luda_ctx* c, const luda_file* tables, u32 n, u32 flags, u32* 
 }
 
 // ---------------------------------------------------------------------------
+// luda_read_tables
+
+luda_entries* read_tables(luda_ctx* c, const luda_file* tables, u32 nfiles, u32 flags) {
+  luda_entries* r = (luda_entries*)calloc(1, sizeof(luda_entries));
+  if (!r) throw std::bad_alloc();
+  struct Owner {
+    luda_entries* r;
+    ~Owner() {
+      if (r) luda_free_entries(r);
+    }
+  } own{r};
+  r->num_tables = nfiles;
+  r->table_first = (uint64_t*)calloc(nfiles + 1, sizeof(uint64_t));
+  r->key_off = (uint64_t*)calloc(1, sizeof(uint64_t));
+  r->val_off = (uint64_t*)calloc(1, sizeof(uint64_t));
+  if (!r->table_first || !r->key_off || !r->val_off) throw std::bad_alloc();
+  if (nfiles == 0) {
+    own.r = nullptr;
+    return r;
+  }
+  HIPCHK(c->pin.reserve(1024 + 4ull * (nfiles + 1) + sizeof(DevStatus)));
+  init_ctx_timer(c);
+  KTimer& T = c->T;
+  std::vector<FileDesc> fdesc(nfiles);
+  u64 bytes_in = 0, nb_cap = 0;
+  for (u32 i = 0; i < nfiles; i++) {
+    bytes_in += tables[i].size;
+    nb_cap += tables[i].size / 41 + 1;
+  }
+  if (nb_cap > 0xffffffffull / 2) throw Fail{LUDA_NOT_SUPPORTED, "tables too large"};
+  if (flags & LUDA_JOB_INPUTS_ON_DEVICE) {
+    for (u32 i = 0; i < nfiles; i++) fdesc[i] = {tables[i].data, tables[i].size};
+  } else {
+    u64 need = 0;
+    for (u32 i = 0; i < nfiles; i++) need += align_up(tables[i].size + 16, 256);
+    HIPCHK(c->in_buf.reserve(need));
+    u64 off = 0;
+    for (u32 i = 0; i < nfiles; i++) {
+      HIPCHK(hipMemcpyAsync(c->in_buf.p + off, tables[i].data, tables[i].size, hipMemcpyHostToDevice, c->s));
+      fdesc[i] = {c->in_buf.p + off, tables[i].size};
+      off += align_up(tables[i].size + 16, 256);
+    }
+  }
+  // footers, index blocks and restart arrays: blocks and entries per table
+  FileDesc* d_files;
+  FileInfo* d_info;
+  u32 *d_nblk, *d_bbase, *d_nent, *d_ebase, *d_febase;
+  BlockDesc* d_blks;
+  auto carve_a = [&](Arena& a) {
+    d_files = a.take<FileDesc>(nfiles);
+    d_info = a.take<FileInfo>(nfiles);
+    d_nblk = a.take<u32>(nfiles + 1);
+    d_bbase = a.take<u32>(nfiles + 1);
+    d_blks = a.take<BlockDesc>(nb_cap);
+    d_nent = a.take<u32>(nb_cap + 1);
+    d_ebase = a.take<u32>(nb_cap + 1);
+    d_febase = a.take<u32>(nfiles + 1);
+  };
+  {
+    Arena dry;
+    dry.dry = true;
+    carve_a(dry);
+    HIPCHK(c->a_buf.reserve(dry.used));
+    Arena a;
+    a.base = c->a_buf.p;
+    a.cap = c->a_buf.cap;
+    carve_a(a);
+  }
+  DevStatus st0;
+  memset(&st0, 0, sizeof(st0));
+  st0.bad_file = 0xffffffffu;
+  stage_in(c, d_files, fdesc.data(), nfiles * sizeof(FileDesc), &st0);
+  launch_files(T, d_files, nfiles, d_info, d_nblk, c->st);
+  HIPCHK(scan_reserve(c->scans, std::max<u64>(nb_cap, nfiles + 1), c->s));
+  scan_u32(T, c->scans, d_nblk, d_bbase, nullptr, nfiles + 1, nfiles + 1, nullptr, c->st);
+  launch_index(T, d_files, d_info, d_bbase, nfiles, (u32)nb_cap, d_blks, d_nent, c->st);
+  if (flags & LUDA_READ_VERIFY) launch_verify(T, d_files, nfiles, d_blks, (u32)nb_cap, c->st);
+  scan_entries(T, c->scans, d_blks, d_nent, d_ebase, (u32)nb_cap, nfiles, d_febase, c->st);
+  stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)}, StageSeg{d_febase, 1024, (nfiles + 1) * 4ull});
+  stream_sync(c);
+  DevStatus stA;
+  memcpy(&stA, c->pin.p, sizeof(DevStatus));
+  std::vector<u32> febase(nfiles + 1);
+  memcpy(febase.data(), c->pin.p + 1024, (nfiles + 1) * 4);
+  if (status_code_of(stA) != LUDA_OK)
+    throw Fail{status_code_of(stA), std::string("table rejected: ") + site_name(stA.site)};
+  const u32 nb = stA.n_blocks_in;
+  const u64 n = stA.n_entries;
+  if (nb > nb_cap) throw Fail{LUDA_CORRUPTION, "more data blocks than the tables can hold"};
+  for (u32 i = 0; i <= nfiles; i++) r->table_first[i] = febase[i];
+  // decode, then the entries' bytes to plain arrays
+  u32 W = std::max<u32>(1, (std::max<u32>(stA.max_rulen, 8 * c->kw_hint) + 7) / 8);
+  for (u32 attempt = 1;; attempt++) {
+    if (W > kMaxKeyWords) throw Fail{LUDA_NOT_SUPPORTED, "user keys longer than the device key planes"};
+    Planes K;
+    u64 *vptr, *key_off, *val_off;
+    u32 *vlen, *ksz;
+    u8 *keys, *vals;
+    auto carve_b = [&](Arena& a) {
+      K.P = a.take<u64>((u64)W * std::max<u64>(n, 1));
+      K.klen = a.take<u32>(n + 1);
+      K.tag = a.take<u64>(n + 1);
+      K.stride = std::max<u64>(n, 1);
+      K.W = W;
+      vptr = a.take<u64>(n + 1);
+      vlen = a.take<u32>(n + 1);
+      ksz = a.take<u32>(n + 1);
+      key_off = a.take<u64>(n + 1);
+      val_off = a.take<u64>(n + 1);
+      keys = a.take<u8>(n * (8ull * W + 8) + 16);
+      vals = a.take<u8>(bytes_in + 16);
+    };
+    {
+      Arena dry;
+      dry.dry = true;
+      carve_b(dry);
+      if (dry.used > c->budget) throw Fail{LUDA_NOT_SUPPORTED, "tables exceed the HBM budget"};
+      HIPCHK(c->b_buf.reserve(dry.used));
+      Arena a;
+      a.base = c->b_buf.p;
+      a.cap = c->b_buf.cap;
+      carve_b(a);
+    }
+    DevStatus sb = stA;
+    sb.err = 0;
+    sb.site = 0;
+    sb.max_ulen = 0;
+    stage_in(c, nullptr, nullptr, 0, &sb);
+    HIPCHK(scan_reserve(c->scans, n + 2, c->s));
+    launch_decode(T, d_files, d_blks, nb, d_ebase, K, vptr, vlen, c->st, stA.max_blk, stA.max_nr, stA.max_ent);
+    launch_entry_out_sizes(T, K, n, ksz, vlen);
+    scan_u32_to_u64(T, c->scans, ksz, key_off, nullptr, (u32)n + 1, (u32)n + 1, nullptr, c->st);
+    scan_u32_to_u64(T, c->scans, vlen, val_off, nullptr, (u32)n + 1, (u32)n + 1, nullptr, c->st);
+    launch_emit(T, K, vptr, vlen, n, key_off, val_off, keys, vals);
+    HIPCHK(hipGetLastError());
+    constexpr u64 o_kb = align_up_c(sizeof(DevStatus), 64), o_vb = o_kb + 64;
+    stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)}, StageSeg{key_off + n, o_kb, 8}, StageSeg{val_off + n, o_vb, 8});
+    stream_sync(c);
+    DevStatus stB;
+    memcpy(&stB, c->pin.p, sizeof(DevStatus));
+    if (stB.err & ERR_OVERFLOW) {
+      if (attempt >= kMaxAttempts || stB.max_ulen <= 8 * W)
+        throw Fail{LUDA_NOT_SUPPORTED, std::string("capacity: ") + site_name(stB.site)};
+      W = (stB.max_ulen + 7) / 8;
+      continue;
+    }
+    if (status_code_of(stB) != LUDA_OK)
+      throw Fail{status_code_of(stB), std::string("table rejected: ") + site_name(stB.site)};
+    c->kw_hint = std::max<u32>(c->kw_hint, (stB.max_ulen + 7) / 8);
+    u64 kb = 0, vb = 0;
+    memcpy(&kb, c->pin.p + o_kb, 8);
+    memcpy(&vb, c->pin.p + o_vb, 8);
+    free(r->key_off);
+    free(r->val_off);
+    r->key_off = (uint64_t*)malloc((n + 1) * 8);
+    r->val_off = (uint64_t*)malloc((n + 1) * 8);
+    r->keys = (uint8_t*)malloc(kb ? kb : 1);
+    r->vals = (uint8_t*)malloc(vb ? vb : 1);
+    if (!r->key_off || !r->val_off || !r->keys || !r->vals) throw std::bad_alloc();
+    HIPCHK(hipMemcpyAsync(r->key_off, key_off, (n + 1) * 8, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(r->val_off, val_off, (n + 1) * 8, hipMemcpyDeviceToHost, c->s));
+    if (kb) HIPCHK(hipMemcpyAsync(r->keys, keys, kb, hipMemcpyDeviceToHost, c->s));
+    if (vb) HIPCHK(hipMemcpyAsync(r->vals, vals, vb, hipMemcpyDeviceToHost, c->s));
+    stream_sync(c);
+    r->n = n;
+    own.r = nullptr;
+    return r;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // luda_verify_tables_noalloc: luda_verify_tables with its working memory in
 // the code object.  Every device the library runs on holds its own copy of
 // `scratch` until the process ends; host code reaches it only through the
@@ -1679,6 +1851,28 @@ int luda_verify_tables(luda_ctx* c, const luda_file* tables, uint32_t n, uint32_
     fault_point("verify");
     verify_tables(c, tables, n, flags, first_bad);
   });
+}
+
+int luda_read_tables(luda_ctx* c, const luda_file* tables, uint32_t n, uint32_t flags, luda_entries** out, char* err,
+                     size_t errlen) {
+  if (!c || !out || (n && !tables)) return LUDA_INVALID_ARGUMENT;
+  *out = nullptr;
+  std::lock_guard<std::mutex> lk(c->mu);
+  hipSetDevice(c->device);
+  return guarded(c, err, errlen, [&] {
+    fault_point("read");
+    *out = read_tables(c, tables, n, flags);
+  });
+}
+
+void luda_free_entries(luda_entries* e) {
+  if (!e) return;
+  free(e->keys);
+  free(e->key_off);
+  free(e->vals);
+  free(e->val_off);
+  free(e->table_first);
+  free(e);
 }
 
 int luda_verify_tables_noalloc(luda_ctx* c, const luda_file* tables, uint32_t n, uint32_t flags, uint32_t* first_bad,

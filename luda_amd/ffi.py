@@ -169,6 +169,18 @@ class luda_file_set(C.Structure):
 
 FILESET_P = C.POINTER(luda_file_set)
 
+LUDA_READ_VERIFY = 0x4
+
+
+class luda_entries(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("num_tables", C.c_uint32), ("reserved", C.c_uint32),
+                ("keys", C.POINTER(C.c_uint8)), ("key_off", C.POINTER(C.c_uint64)),
+                ("vals", C.POINTER(C.c_uint8)), ("val_off", C.POINTER(C.c_uint64)),
+                ("table_first", C.POINTER(C.c_uint64))]
+
+
+ENTRIES_P = C.POINTER(luda_entries)
+
 LUDA_EDIT_HAS_COMPARATOR = 0x1
 LUDA_EDIT_HAS_LOG_NUMBER = 0x2
 LUDA_EDIT_HAS_PREV_LOG_NUMBER = 0x4
@@ -230,6 +242,9 @@ ENGINE_SIGNATURES = {
                                               C.POINTER(C.c_uint64)]),
     "luda_log_record": (C.c_int, [C.POINTER(C.c_uint64), C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
                                   C.POINTER(C.c_uint64)]),
+    "luda_read_tables": (C.c_int, [C.c_void_p, C.POINTER(luda_file), C.c_uint32, C.c_uint32, C.POINTER(ENTRIES_P),
+                                   C.c_char_p, C.c_size_t]),
+    "luda_free_entries": (None, [ENTRIES_P]),
     "luda_poll": (C.c_int, [C.c_void_p]),
     "luda_drain": (C.c_int, [C.c_void_p]),
 }

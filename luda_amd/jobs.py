@@ -389,6 +389,38 @@ class Engine:
         return self.build_tables(C.byref(opts), n, C.cast(kb, C.c_void_p), C.cast(ko, C.c_void_p),
                                  C.cast(vb, C.c_void_p), C.cast(vo, C.c_void_p))
 
+    def read_tables(self, tables, verify=False, on_device=False):
+        """luda_read_tables: (status, ([entries per table of (key, value)]) |
+        None, message); tables are bytes, or (device pointer, size) pairs with
+        on_device."""
+        arr = (ffi.luda_file * max(1, len(tables)))()
+        keep = []
+        for i, t in enumerate(tables):
+            if on_device:
+                arr[i].data, arr[i].size = t
+            else:
+                b = C.create_string_buffer(bytes(t), len(t))
+                keep.append(b)
+                arr[i].data, arr[i].size = C.cast(b, C.c_void_p), len(t)
+        flags = (ffi.LUDA_READ_VERIFY if verify else 0) | (ffi.LUDA_JOB_INPUTS_ON_DEVICE if on_device else 0)
+        out = ffi.ENTRIES_P()
+        err = C.create_string_buffer(512)
+        rc = self.lib.luda_read_tables(self.ctx, arr, len(tables), flags, C.byref(out), err, 512)
+        if rc != ffi.LUDA_OK:
+            return rc, None, err.value.decode()
+        try:
+            e = out.contents
+            n = e.n
+            ko = [e.key_off[i] for i in range(n + 1)]
+            vo = [e.val_off[i] for i in range(n + 1)]
+            kb = C.string_at(e.keys, ko[n]) if ko[n] else b""
+            vb = C.string_at(e.vals, vo[n]) if vo[n] else b""
+            tf = [e.table_first[i] for i in range(e.num_tables + 1)]
+            ents = [(kb[ko[i]:ko[i + 1]], vb[vo[i]:vo[i + 1]]) for i in range(n)]
+            return rc, [ents[tf[t]:tf[t + 1]] for t in range(e.num_tables)], ""
+        finally:
+            self.lib.luda_free_entries(out)
+
     # -- file I/O (luda_read_files, luda_write_outputs) --------------------
     def read_files(self, paths, numbers=None):
         """(status, luda_file_set pointer | None, message); free the set with
