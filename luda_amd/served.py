@@ -24,14 +24,13 @@ Rules followed (reference files):
     (VersionSet::Finalize db/version_set.cc:1050-1086, MaxBytesForLevel :42-53);
   * picking: the first file past compact_pointer_[level] (level 0: widened
     to every overlapping level-0 file), boundary inputs, the overlapping files
-    of level+1, grandparents of level+2 (PickCompaction / SetupOtherInputs /
+    of level+1, the level's inputs grown when that takes no more level+1
+    files, grandparents of level+2 (PickCompaction / SetupOtherInputs /
     AddBoundaryInputs / GetOverlappingInputs, db/version_set.cc:1270-1464,
     :1380-1403, Version::GetOverlappingInputs :519-555);
   * trivial moves (Compaction::IsTrivialMove :1517-1525, DBImpl::
     BackgroundCompaction db/db_impl.cc:694-711);
   * install: inputs deleted, outputs added at level+1 (db/db_impl.cc:1040-1056).
-SetupOtherInputs' optional widening of inputs_[0] (version_set.cc:1420-1450)
-is left out: it changes which files a job takes, not what a job computes.
 
 The served form may run compactions of disjoint level pairs at the same time
 (the reference runs one at a time, db/db_impl.cc:865-871); a job's inputs,
@@ -371,10 +370,23 @@ class Lsm:
         inputs0 = [first]
         if L == 0:
             inputs0 = self.overlapping(0, ukey(first.smallest), ukey(first.largest))
+        # SetupOtherInputs (db/version_set.cc:1405-1464)
         inputs0 = self.add_boundary(L, inputs0)
         lo, hi = self.key_range(inputs0)
-        inputs1 = self.add_boundary(L + 1, self.overlapping(L + 1, ukey(lo), ukey(hi)))
+        inputs1 = self.overlapping(L + 1, ukey(lo), ukey(hi))
         alo, ahi = self.key_range(inputs0 + inputs1)
+        if inputs1:
+            # grow the level's inputs when that takes in no more level+1 files
+            # and stays under ExpandedCompactionByteSizeLimit (:37-39)
+            exp0 = self.add_boundary(L, self.overlapping(L, ukey(alo), ukey(ahi)))
+            size1 = sum(f.size for f in inputs1)
+            if len(exp0) > len(inputs0) and size1 + sum(f.size for f in exp0) < 25 * self.max_file_size:
+                nlo, nhi = self.key_range(exp0)
+                exp1 = self.overlapping(L + 1, ukey(nlo), ukey(nhi))
+                if len(exp1) == len(inputs1):
+                    lo, hi = nlo, nhi
+                    inputs0, inputs1 = exp0, exp1
+                    alo, ahi = self.key_range(inputs0 + inputs1)
         gps = self.overlapping(L + 2, ukey(alo), ukey(ahi)) if L + 2 < NUM_LEVELS else []
         self.compact_pointer[L] = hi
         if len(inputs0) == 1 and not inputs1 and sum(f.size for f in gps) <= 10 * self.max_file_size:

@@ -24,6 +24,7 @@
 #include "db/table_cache.h"
 #include "db/version_edit.h"
 #include "db/version_set.h"
+#include "port/port.h"
 #include "leveldb/comparator.h"
 #include "leveldb/env.h"
 #include "leveldb/filter_policy.h"
@@ -391,6 +392,61 @@ int ref_encode_edit(const luda_edit* e, uint8_t* dst, uint64_t cap, uint64_t* le
   *len = out.size();
   if (out.size() > cap) return LUDA_INVALID_ARGUMENT;
   memcpy(dst, out.data(), out.size());
+  return LUDA_OK;
+}
+
+// VersionSet::PickCompaction (db/version_set.cc:1270-1464) over a tree given
+// as an edit (its new files and compact pointers), applied to an empty
+// VersionSet in directory `dbdir` with LogAndApply (which also runs Finalize,
+// :1050-1086).  Out: the picked level (-1 when none), the file numbers of
+// inputs_[0], inputs_[1] and grandparents_ in the Compaction's order,
+// IsTrivialMove, and the Compaction's own edit (its compact pointer) encoded.
+int ref_pick(const char* dbdir, const luda_edit* tree, uint64_t max_file_size, int* level, uint64_t* in0,
+             uint32_t* n0, uint64_t* in1, uint32_t* n1, uint64_t* gp, uint32_t* ngp, int* trivial, uint8_t* edit,
+             uint64_t edit_cap, uint64_t* edit_len) {
+  leveldb::Options options;
+  options.max_file_size = max_file_size;
+  leveldb::Env* env = leveldb::Env::Default();
+  options.env = env;
+  env->CreateDir(dbdir);
+  leveldb::InternalKeyComparator icmp(leveldb::BytewiseComparator());
+  leveldb::TableCache tc(dbdir, options, 10);
+  leveldb::VersionSet vs(dbdir, &options, &tc, &icmp);
+  leveldb::VersionEdit ve;
+  for (uint32_t i = 0; i < tree->num_compact_pointers; i++) {
+    leveldb::InternalKey k;
+    k.DecodeFrom(Slice((const char*)tree->compact_pointers[i].key.data, tree->compact_pointers[i].key.size));
+    ve.SetCompactPointer(tree->compact_pointers[i].level, k);
+  }
+  for (uint32_t i = 0; i < tree->num_new_files; i++) {
+    const luda_edit_file& f = tree->new_files[i];
+    leveldb::InternalKey a, b;
+    a.DecodeFrom(Slice((const char*)f.smallest.data, f.smallest.size));
+    b.DecodeFrom(Slice((const char*)f.largest.data, f.largest.size));
+    ve.AddFile(f.level, f.number, f.file_size, a, b);
+  }
+  leveldb::port::Mutex mu;
+  mu.Lock();
+  Status st = vs.LogAndApply(&ve, &mu);
+  mu.Unlock();
+  if (!st.ok()) return LUDA_IO_ERROR;
+  leveldb::Compaction* c = vs.PickCompaction();
+  *level = -1;
+  *n0 = *n1 = *ngp = 0;
+  *trivial = 0;
+  *edit_len = 0;
+  if (!c) return LUDA_OK;
+  *level = c->level();
+  for (auto* f : c->inputs_[0]) in0[(*n0)++] = f->number;
+  for (auto* f : c->inputs_[1]) in1[(*n1)++] = f->number;
+  for (auto* f : c->grandparents_) gp[(*ngp)++] = f->number;
+  *trivial = c->IsTrivialMove() ? 1 : 0;
+  std::string enc;
+  c->edit()->EncodeTo(&enc);
+  *edit_len = enc.size();
+  if (enc.size() <= edit_cap) memcpy(edit, enc.data(), enc.size());
+  c->ReleaseInputs();
+  delete c;
   return LUDA_OK;
 }
 

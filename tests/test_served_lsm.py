@@ -108,11 +108,14 @@ def test_deeper_rows_match_the_job():
                         (5, "a", "z"), (6, "h5", "h9")]:
         lsm.add_file(lvl, f(lo, hi))
     spec = lsm.pick()
-    assert spec.level == 1 and [x.smallest[:-8] for x in spec.inputs[0]] == [b"c"]
-    # the job spans user keys c .. g (its level-1 file and the level-2 file);
-    # "g0" sorts after "g", so the level-3 file g0 .. g9 is outside
+    # c .. e takes the level-2 file d .. g; growing the level-1 inputs to
+    # f .. h takes no other level-2 file, so SetupOtherInputs takes both
+    # (db/version_set.cc:1420-1450)
+    assert spec.level == 1 and [x.smallest[:-8] for x in spec.inputs[0]] == [b"c", b"f"]
+    # the job spans user keys c .. h; "h5" sorts after "h", so the level-6
+    # file h5 .. h9 is outside
     got = [(d, x.smallest[:-8].decode()) for d, x in spec.deeper]
-    assert got == [(3, "c5"), (4, "e0"), (5, "a")]
+    assert got == [(3, "c5"), (3, "g0"), (4, "e0"), (5, "a")]
     opts = small_options()
 
     def rows(job):
@@ -120,11 +123,11 @@ def test_deeper_rows_match_the_job():
     with_rows, _k1 = served._job_struct(spec, opts)
     rebuilt = served.JobSpec(spec.level, spec.inputs, spec.grandparents, spec.deeper_files, spec.smallest_snapshot)
     from_files, _k2 = served._job_struct(rebuilt, opts)
-    assert rows(with_rows) == rows(from_files) and with_rows.num_deeper == 3
+    assert rows(with_rows) == rows(from_files) and with_rows.num_deeper == 4
     lsm.add_file(4, f("ea", "eb"))  # a deeper level changes after the pick
     again, _k3 = served._job_struct(spec, opts)
     assert rows(again) == rows(with_rows)
-    assert np.frombuffer(rows(with_rows), dtype=served._LEVEL_FILE)["level"].tolist() == [3, 4, 5]
+    assert np.frombuffer(rows(with_rows), dtype=served._LEVEL_FILE)["level"].tolist() == [3, 3, 4, 5]
 
 
 @pytest.mark.parametrize("mode", ["plain", "served"])
