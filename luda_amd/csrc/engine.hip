@@ -31,7 +31,9 @@
 namespace {
 
 constexpr u64 kAlign = 256;
-constexpr u32 kMaxKeyWords = 32;     // user keys up to 256 bytes
+// user keys up to 2 KiB: the wide-key decoder holds one key per lane in LDS
+// (64 x (8 * 256 + 8) bytes beside its block stage, within a CU's 160 KiB)
+constexpr u32 kMaxKeyWords = 256;
 constexpr u64 kMaxBlockSize = 8192;   // block_size handled on the device
 constexpr u32 kMaxChunk = 8192;       // orbit chunk: its LDS footprint is 5 bytes per node
 constexpr u32 kMaxAttempts = 4;

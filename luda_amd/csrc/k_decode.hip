@@ -984,7 +984,18 @@ void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 
     return;
   }
   const u32 grid = nb < 65536 ? nb : 65536;
+  // the stage and one key buffer per lane: 138 KiB for 2 KiB user keys, more
+  // than the default limit of dynamic LDS per workgroup
   const size_t lds = kStage + 16 + 64 * (8 * K.W + 8);
+  if (lds > 65536) {
+    static bool raised = false;
+    if (!raised) {
+      const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_decode_groups),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      if (e != hipSuccess) launch_refused("decode (wide keys: LDS limit)", e);
+      raised = true;
+    }
+  }
   KLAUNCH(T, "decode", k_decode_groups, grid, 64, lds, files, blks, nb, blk_ebase, K, vptr, vlen, st);
 }
 void launch_ingest(KTimer& T, u64 n, const u8* keys, const u64* key_off, const u8* vals, const u64* val_off, Kept out,

@@ -214,10 +214,22 @@ def test_wide_keys_match_oracle(eng, seed):
     assert out[1]["key_width"] > 64
 
 
-def test_key_over_256_bytes_is_left_to_the_cpu(eng):
-    tj = wide_key_job(9, lo=200, hi=257)
+@pytest.mark.parametrize("seed", range(3))
+def test_keys_of_257_to_2048_bytes_match_oracle(eng, seed):
+    """User keys of 257-2048 bytes: planes of up to 256 words, the grouped
+    decoder with more than 64 KiB of LDS."""
+    print("very wide keys", seed, flush=True)
+    out = check_against_oracle(eng, wide_key_job(seed, lo=257, hi=2048, nl0=3), seed)
+    assert out[1]["key_width"] > 256
+
+
+def test_key_over_2048_bytes_is_left_to_the_cpu(eng):
+    tj = wide_key_job(9, lo=1900, hi=2048)
+    long_key = ikey(b"\xfe" * 2049, 10 ** 6, 1)  # sorts after every other key
+    tj.files[0][0].append((long_key, b"v"))
     job, keep = tj.make_job(oracle_build)
-    assert max(len(k) - 8 for f in tj.files[0] + tj.files[1] for k, _ in f) == 257
+    assert max(len(k) - 8 for f in tj.files[0] + tj.files[1] for k, _ in f) == 2049
+    assert oracle.compact(job, "oracle")[0] == 0
     rc, out, msg = eng.compact(job)
     assert rc == ffi.LUDA_NOT_SUPPORTED, msg
 
