@@ -42,6 +42,16 @@ extern "C" {
 /* luda_job.flags */
 #define LUDA_JOB_INPUTS_ON_DEVICE 0x1u /* luda_file.data are device pointers */
 #define LUDA_JOB_KEEP_ON_DEVICE 0x2u   /* leave output bytes in HBM only (result.data == NULL) */
+/* Keep the output tables' decoded entries (key planes, tags, value addresses
+ * in the tables) resident in HBM with the result, until luda_free_result.  A
+ * later luda_compact whose inputs are all such tables, passed by their device
+ * addresses (LUDA_JOB_INPUTS_ON_DEVICE), takes their entries from there and
+ * decodes nothing: the chained level compactions of a served DB.  The
+ * reference keeps the decoded entries of flushed level-0 tables in host memory
+ * for the next L0->L1 compaction the same way (cuda/cuda_common.h:253-255,
+ * 339-355; db/db_impl.cc:146-149,1292-1295).  Also a flag of
+ * luda_build_tables. */
+#define LUDA_JOB_KEEP_DECODED 0x8u
 
 /* The subset of leveldb::Options (include/leveldb/options.h:33-143) that
  * changes table bytes. */

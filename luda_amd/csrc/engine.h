@@ -104,6 +104,22 @@ struct BlockDesc {
   u32 file;
 };
 
+// One input table whose decoded entries are resident: entries [first,
+// first + count) of the arrays below go to [dst, dst + count) of the job.
+struct DecodedSrc {
+  const u64* P;    // plane words, W planes of `stride`
+  u64 stride;
+  u32 W;
+  u32 pad;
+  const u32* klen;
+  const u64* tag;
+  const u64* vptr;  // device addresses of the values in the table itself
+  const u32* vlen;
+  u64 first;
+  u64 count;
+  u64 dst;
+};
+
 // Two adjacent runs of an index array merged into the same positions:
 // [a0, a1) and [a1, b1); tile0 = first thread-tile of this pair.
 struct MergePair {
@@ -267,6 +283,8 @@ void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 
                    u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr, u32 max_ent);
 void launch_ingest(KTimer& T, u64 n, const u8* keys, const u64* key_off, const u8* vals, const u64* val_off, Kept out,
                    DevStatus* st);
+// entries 0 .. n-1 from resident decoded sources (src sorted by dst)
+void launch_gather_decoded(KTimer& T, const DecodedSrc* src, u32 nsrc, u64 n, Planes K, u64* vptr, u32* vlen);
 
 // k_merge.hip
 // One merge round; in == nullptr reads the children in place (first round,
@@ -338,6 +356,9 @@ void launch_windows(KTimer& T, const u32* nbo_dev, u32 nbo_cap, const u32* nf_de
                     EncBufs b);
 void launch_write_blocks(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u8* out,
                          u64 out_cap, DevStatus* st);
+// vaddr[q] = device address of kept entry q's value in the written tables
+void launch_value_addrs(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u8* out,
+                        u64* vaddr);
 void launch_write_filters(KTimer& T, Kept k, const u32* nw_dev, u32 nw_cap, const u32* nf_dev, EncParams ep, EncBufs b,
                           u8* out, u64 out_cap, DevStatus* st);
 void launch_write_tail(KTimer& T, Kept k, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b, u8* out,

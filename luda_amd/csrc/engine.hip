@@ -311,10 +311,11 @@ struct EncodeArrays {
   u32* ent_b;
   u16* ftab_f;
   u32* ent_f;
+  u64* vaddr;  // with LUDA_JOB_KEEP_DECODED: each kept value's address in the output
 };
 
 // Carves the encoder's arrays for a setup out of an arena (dry or real).
-EncodeArrays carve_encode(Arena& a, const EncodeSetup& s, u32 W, bool with_kept) {
+EncodeArrays carve_encode(Arena& a, const EncodeSetup& s, u32 W, bool with_kept, bool with_vaddr = false) {
   EncodeArrays r;
   memset(&r, 0, sizeof(r));
   const u64 m = s.m_cap, nb = s.nbo_cap, nf = s.nf_cap, nw = s.nw_cap;
@@ -366,6 +367,7 @@ EncodeArrays carve_encode(Arena& a, const EncodeSetup& s, u32 W, bool with_kept)
     r.ftab_f = a.take<u16>(s.of.table_entries);
     r.ent_f = a.take<u32>(s.of.entry_entries);
   }
+  if (with_vaddr) r.vaddr = a.take<u64>(m);
   return r;
 }
 
@@ -416,6 +418,7 @@ void run_encode(luda_ctx* c, const EncodeSetup& s, const EncodeArrays& A, u8* ou
   launch_write_blocks(T, k, nbo_dev, s.nbo_cap, s.ep, b, out, s.out_cap, st);
   if (s.ep.bloom_bits) launch_write_filters(T, k, nw_dev, s.nw_cap, nf_dev, s.ep, b, out, s.out_cap, st);
   launch_write_tail(T, k, nf_dev, s.nf_cap, s.ep, b, out, s.out_cap, st);
+  if (A.vaddr) launch_value_addrs(T, k, nbo_dev, s.nbo_cap, s.ep, b, out, A.vaddr);
 }
 
 // Orbit chunk for successor steps of at most `step` nodes: a chunk must hold
@@ -606,6 +609,127 @@ luda_result* finish_result(luda_ctx* c, const EncodeSetup& s, const EncodeArrays
 }
 
 // ---------------------------------------------------------------------------
+// Resident decoded entries (LUDA_JOB_KEEP_DECODED).  A result's kept stream
+// (planes, tags, value lengths) and its values' addresses in the output
+// tables are copied to a buffer the result owns; each output table is
+// registered by its device address, so that a later job whose inputs are all
+// registered tables gathers their entries instead of decoding them.
+
+struct DecodedSet {
+  int device;
+  void* buf;
+  u32 W;
+  u64 m;  // entries (the stride of every array)
+  u64* P;
+  u32* klen;
+  u64* tag;
+  u64* vptr;
+  u32* vlen;
+};
+struct ResidentTable {
+  DecodedSet* d;
+  u64 first, count, size;
+};
+std::mutex g_resident_mu;
+std::map<std::pair<int, u64>, ResidentTable> g_resident;  // (device, table address)
+std::map<const luda_result*, DecodedSet*> g_decoded_of;
+
+// A result this context just built, released on a failure after it was
+// made (its device buffer goes back to the context's pool).
+void free_result_now(luda_ctx* c, luda_result* r) {
+  if (r->device_data) c->give_out(r->device_data);
+  if (r->data) hipHostFree(r->data);
+  free(r->outputs);
+  free(r->keys);
+  free(r);
+}
+
+// After the job's last synchronisation: its m kept entries copied out of the
+// context's working memory, and its tables registered.
+void keep_decoded(luda_ctx* c, const EncodeArrays& A, u32 W, u64 m, luda_result* r) {
+  DecodedSet* d = new DecodedSet;
+  d->device = c->device;
+  d->W = W;
+  d->m = m;
+  const u64 mm = std::max<u64>(m, 1);
+  const u64 bytes = mm * (8ull * W + 4 + 8 + 8 + 4) + 5 * kAlign;
+  if (hipMalloc(&d->buf, bytes) != hipSuccess) {
+    delete d;
+    throw Fail{LUDA_IO_ERROR, "device memory for the resident decoded entries"};
+  }
+  Arena a;
+  a.base = static_cast<u8*>(d->buf);
+  a.cap = bytes;
+  d->P = a.take<u64>((u64)W * mm);
+  d->klen = a.take<u32>(mm);
+  d->tag = a.take<u64>(mm);
+  d->vptr = a.take<u64>(mm);
+  d->vlen = a.take<u32>(mm);
+  const Kept& k = A.kept;
+  hipError_t e = hipSuccess;
+  for (u32 w = 0; w < W && m && e == hipSuccess; w++)
+    e = hipMemcpyAsync(d->P + (u64)w * m, k.K.P + (u64)w * k.K.stride, m * 8, hipMemcpyDeviceToDevice, c->s);
+  if (m && e == hipSuccess) e = hipMemcpyAsync(d->klen, k.K.klen, m * 4, hipMemcpyDeviceToDevice, c->s);
+  if (m && e == hipSuccess) e = hipMemcpyAsync(d->tag, k.K.tag, m * 8, hipMemcpyDeviceToDevice, c->s);
+  if (m && e == hipSuccess) e = hipMemcpyAsync(d->vptr, A.vaddr, m * 8, hipMemcpyDeviceToDevice, c->s);
+  if (m && e == hipSuccess) e = hipMemcpyAsync(d->vlen, k.vlen, m * 4, hipMemcpyDeviceToDevice, c->s);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->s);
+  if (e != hipSuccess) {
+    hipFree(d->buf);
+    delete d;
+    throw Fail{LUDA_IO_ERROR, std::string("resident decoded entries: ") + hipGetErrorString(e)};
+  }
+  std::lock_guard<std::mutex> lk(g_resident_mu);
+  u64 first = 0;
+  for (u32 i = 0; i < r->num_outputs; i++) {
+    const luda_output& o = r->outputs[i];
+    g_resident[{c->device, (u64)r->device_data + o.offset}] = ResidentTable{d, first, o.num_entries, o.file_size};
+    first += o.num_entries;
+  }
+  g_decoded_of[r] = d;
+}
+
+void forget_decoded(const luda_result* r) {
+  DecodedSet* d = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_resident_mu);
+    auto it = g_decoded_of.find(r);
+    if (it == g_decoded_of.end()) return;
+    d = it->second;
+    g_decoded_of.erase(it);
+    for (u32 i = 0; i < r->num_outputs; i++) g_resident.erase({d->device, (u64)r->device_data + r->outputs[i].offset});
+  }
+  hipSetDevice(d->device);
+  hipFree(d->buf);
+  delete d;
+}
+
+// Every input table registered (same device, same size): their sources, in
+// input order.
+bool find_resident(int device, const std::vector<const luda_file*>& files, std::vector<DecodedSrc>* out) {
+  std::lock_guard<std::mutex> lk(g_resident_mu);
+  out->clear();
+  for (const luda_file* f : files) {
+    auto it = g_resident.find({device, (u64)f->data});
+    if (it == g_resident.end() || it->second.size != f->size) return false;
+    const ResidentTable& t = it->second;
+    DecodedSrc s;
+    memset(&s, 0, sizeof(s));
+    s.P = t.d->P;
+    s.stride = t.d->m;
+    s.W = t.d->W;
+    s.klen = t.d->klen;
+    s.tag = t.d->tag;
+    s.vptr = t.d->vptr;
+    s.vlen = t.d->vlen;
+    s.first = t.first;
+    s.count = t.count;
+    out->push_back(s);
+  }
+  return !files.empty();
+}
+
+// ---------------------------------------------------------------------------
 // luda_compact
 
 struct Child {
@@ -691,58 +815,79 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
   }
   HIPCHK(hipEventRecord(c->ev0, c->s));
 
-  // ---- phase A: footers, index blocks, restart arrays
-  u64 nb_cap = 0;
-  for (auto* f : files) nb_cap += f->size / 41 + 1;
-  if (nb_cap > 0xffffffffull / 2) throw Fail{LUDA_NOT_SUPPORTED, "inputs too large"};
-  auto carve_a = [&](Arena& a, FileDesc*& d_files, FileInfo*& d_info, u32*& d_nblk, u32*& d_bbase, BlockDesc*& d_blks,
-                     u32*& d_nent, u32*& d_ebase, u32*& d_febase) {
-    d_files = a.take<FileDesc>(nfiles);
-    d_info = a.take<FileInfo>(nfiles);
-    d_nblk = a.take<u32>(nfiles + 1);
-    d_bbase = a.take<u32>(nfiles + 1);
-    d_blks = a.take<BlockDesc>(nb_cap);
-    d_nent = a.take<u32>(nb_cap + 1);
-    d_ebase = a.take<u32>(nb_cap + 1);
-    d_febase = a.take<u32>(nfiles + 1);
-  };
-  FileDesc* d_files;
-  FileInfo* d_info;
-  u32 *d_nblk, *d_bbase, *d_nent, *d_ebase, *d_febase;
-  BlockDesc* d_blks;
-  {
-    Arena dry;
-    dry.dry = true;
-    carve_a(dry, d_files, d_info, d_nblk, d_bbase, d_blks, d_nent, d_ebase, d_febase);
-    HIPCHK(c->a_buf.reserve(dry.used));
-    Arena a;
-    a.base = c->a_buf.p;
-    a.cap = c->a_buf.cap;
-    carve_a(a, d_files, d_info, d_nblk, d_bbase, d_blks, d_nent, d_ebase, d_febase);
-  }
-  DevStatus st0;
-  memset(&st0, 0, sizeof(st0));
-  stage_in(c, d_files, fdesc.data(), nfiles * sizeof(FileDesc), &st0);
-  KTimer& T = c->T;
-  launch_files(T, d_files, nfiles, d_info, d_nblk, c->st);
-  HIPCHK(scan_reserve(c->scans, std::max<u64>(nb_cap, nfiles + 1), c->s));
-  scan_u32(T, c->scans, d_nblk, d_bbase, nullptr, nfiles + 1, nfiles + 1, nullptr, c->st);
-  launch_index(T, d_files, d_info, d_bbase, nfiles, (u32)nb_cap, d_blks, d_nent, c->st);  // sets n_blocks_in
-  scan_entries(T, c->scans, d_blks, d_nent, d_ebase, (u32)nb_cap, nfiles, d_febase, c->st);
+  // Inputs whose decoded entries an earlier call left resident
+  // (LUDA_JOB_KEEP_DECODED): when every input is one, phase A and the decode
+  // are skipped and their entries gathered instead.
+  std::vector<DecodedSrc> dsrc;
+  const bool resident = (job->flags & LUDA_JOB_INPUTS_ON_DEVICE) && find_resident(c->device, files, &dsrc);
+  FileDesc* d_files = nullptr;
+  FileInfo* d_info = nullptr;
+  u32 *d_nblk = nullptr, *d_bbase = nullptr, *d_nent = nullptr, *d_ebase = nullptr, *d_febase = nullptr;
+  BlockDesc* d_blks = nullptr;
   DevStatus stA;
+  memset(&stA, 0, sizeof(stA));
   std::vector<u32> febase(nfiles + 1);
-  HIPCHK(hipGetLastError());
-  stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)}, StageSeg{d_febase, 1024, (nfiles + 1) * 4ull});
-  stream_sync(c);
-  memcpy(&stA, c->pin.p, sizeof(DevStatus));
-  memcpy(febase.data(), c->pin.p + 1024, (nfiles + 1) * 4);
-  if (status_code_of(stA) != LUDA_OK)
-    throw Fail{status_code_of(stA), std::string("input rejected: ") + site_name(stA.site)};
+  KTimer& T = c->T;
+  if (resident) {
+    u64 e = 0;
+    u32 wmax = 1;
+    for (u32 i = 0; i < nfiles; i++) {
+      febase[i] = (u32)e;
+      dsrc[i].dst = e;
+      e += dsrc[i].count;
+      wmax = std::max<u32>(wmax, dsrc[i].W);
+    }
+    if (e > 0xffffffffull / 2) throw Fail{LUDA_NOT_SUPPORTED, "inputs too large"};
+    febase[nfiles] = (u32)e;
+    stA.n_entries = (u32)e;
+    stA.max_rulen = 8 * wmax;
+  } else {
+    // ---- phase A: footers, index blocks, restart arrays
+    u64 nb_cap = 0;
+    for (auto* f : files) nb_cap += f->size / 41 + 1;
+    if (nb_cap > 0xffffffffull / 2) throw Fail{LUDA_NOT_SUPPORTED, "inputs too large"};
+    auto carve_a = [&](Arena& a, FileDesc*& d_files, FileInfo*& d_info, u32*& d_nblk, u32*& d_bbase, BlockDesc*& d_blks,
+                       u32*& d_nent, u32*& d_ebase, u32*& d_febase) {
+      d_files = a.take<FileDesc>(nfiles);
+      d_info = a.take<FileInfo>(nfiles);
+      d_nblk = a.take<u32>(nfiles + 1);
+      d_bbase = a.take<u32>(nfiles + 1);
+      d_blks = a.take<BlockDesc>(nb_cap);
+      d_nent = a.take<u32>(nb_cap + 1);
+      d_ebase = a.take<u32>(nb_cap + 1);
+      d_febase = a.take<u32>(nfiles + 1);
+    };
+    {
+      Arena dry;
+      dry.dry = true;
+      carve_a(dry, d_files, d_info, d_nblk, d_bbase, d_blks, d_nent, d_ebase, d_febase);
+      HIPCHK(c->a_buf.reserve(dry.used));
+      Arena a;
+      a.base = c->a_buf.p;
+      a.cap = c->a_buf.cap;
+      carve_a(a, d_files, d_info, d_nblk, d_bbase, d_blks, d_nent, d_ebase, d_febase);
+    }
+    DevStatus st0;
+    memset(&st0, 0, sizeof(st0));
+    stage_in(c, d_files, fdesc.data(), nfiles * sizeof(FileDesc), &st0);
+    launch_files(T, d_files, nfiles, d_info, d_nblk, c->st);
+    HIPCHK(scan_reserve(c->scans, std::max<u64>(nb_cap, nfiles + 1), c->s));
+    scan_u32(T, c->scans, d_nblk, d_bbase, nullptr, nfiles + 1, nfiles + 1, nullptr, c->st);
+    launch_index(T, d_files, d_info, d_bbase, nfiles, (u32)nb_cap, d_blks, d_nent, c->st);  // sets n_blocks_in
+    scan_entries(T, c->scans, d_blks, d_nent, d_ebase, (u32)nb_cap, nfiles, d_febase, c->st);
+    HIPCHK(hipGetLastError());
+    stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)}, StageSeg{d_febase, 1024, (nfiles + 1) * 4ull});
+    stream_sync(c);
+    memcpy(&stA, c->pin.p, sizeof(DevStatus));
+    memcpy(febase.data(), c->pin.p + 1024, (nfiles + 1) * 4);
+    if (status_code_of(stA) != LUDA_OK)
+      throw Fail{status_code_of(stA), std::string("input rejected: ") + site_name(stA.site)};
+    if (stA.n_blocks_in > nb_cap) throw Fail{LUDA_CORRUPTION, "more data blocks than the tables can hold"};
+  }
   const u32 nb = stA.n_blocks_in;
   const u32 n = stA.n_entries;
   stats.blocks_in = nb;
   stats.entries_in = n;
-  if (nb > nb_cap) throw Fail{LUDA_CORRUPTION, "more data blocks than the tables can hold"};
 
   // ---- phase B
   u32 W = std::max<u32>(1, (std::max<u32>(stA.max_rulen, 8 * c->kw_hint) + 7) / 8);
@@ -838,7 +983,8 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
               o_gt = put(gt.data(), gt.size() * 8), o_gp = put(gpre.data(), gpre.size() * 8),
               o_dw = put(dw.data(), dw.size() * 8), o_dl = put(dl.data(), dl.size() * 4),
               o_cs = put(cstart.data(), cstart.size() * 4),
-              o_mp = put(allp.data(), allp.size() * sizeof(MergePair));
+              o_mp = put(allp.data(), allp.size() * sizeof(MergePair)),
+              o_ds = put(dsrc.data(), dsrc.size() * sizeof(DecodedSrc));
     // carve phase B
     Planes K;
     u64 *vptr, *gp_words, *gp_tag, *gp_prefix, *deep_words;
@@ -862,7 +1008,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
       events = a.take<u32>(ngp + 2);
       splits = a.take<u32>((u64)n / merge_items_per_tile(W) + runs.size() + 2);
       d_tab = a.take<u8>(blob.size());
-      EA = carve_encode(a, es, W, true);
+      EA = carve_encode(a, es, W, true, (job->flags & LUDA_JOB_KEEP_DECODED) != 0);
     };
     {
       Arena dry;
@@ -883,6 +1029,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     deep_len = reinterpret_cast<u32*>(d_tab + o_dl);
     d_cstart = reinterpret_cast<u32*>(d_tab + o_cs);
     d_pairs = reinterpret_cast<MergePair*>(d_tab + o_mp);
+    const DecodedSrc* d_dsrc = reinterpret_cast<const DecodedSrc*>(d_tab + o_ds);
     void* out = c->take_out(std::max<u64>(out_cap, 1));
     if (!out) throw Fail{LUDA_IO_ERROR, "output allocation failed"};
     {
@@ -911,7 +1058,8 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
                           c->s));
       // the fused last merge round looks back over its merge tiles
       HIPCHK(scan_reserve_tiles(c->scans, ((u64)n + merge_items_per_tile(W) - 1) / merge_items_per_tile(W) + 1, c->s));
-      launch_decode(T, d_files, d_blks, nb, d_ebase, K, vptr, vlen, c->st, stA.max_blk, stA.max_nr, stA.max_ent);
+      if (resident) launch_gather_decoded(T, d_dsrc, nfiles, n, K, vptr, vlen);
+      else launch_decode(T, d_files, d_blks, nb, d_ebase, K, vptr, vlen, c->st, stA.max_blk, stA.max_nr, stA.max_ent);
       if (W <= 8) {
         // Every round but the last writes a permutation (the first reads the
         // children in place and checks they are sorted); the last round
@@ -1015,6 +1163,14 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     fill_kernel_stats(c, &stats, W, rounds.size());
     luda_result* r = finish_result(c, es, EA, out, stB, job->flags, stats,
                                    reinterpret_cast<const FileMeta*>(c->pin.p + o_meta), c->pin.p + o_keys);
+    if (job->flags & LUDA_JOB_KEEP_DECODED) {
+      try {
+        keep_decoded(c, EA, W, stB.n_kept, r);
+      } catch (...) {
+        free_result_now(c, r);
+        throw;
+      }
+    }
     r->stats.total_ms = now_ms() - t_call;
     r->stats.end_clock_ms = now_ms();
     return r;
@@ -1428,13 +1584,13 @@ luda_result* build_tables(luda_ctx* c, const luda_options* opts, u64 n, const u8
     {
       Arena dry;
       dry.dry = true;
-      carve_encode(dry, es, W, true);
+      carve_encode(dry, es, W, true, (flags & LUDA_JOB_KEEP_DECODED) != 0);
       if (dry.used > c->budget) throw Fail{LUDA_NOT_SUPPORTED, "job exceeds the HBM budget"};
       HIPCHK(c->b_buf.reserve(dry.used));
       Arena a;
       a.base = c->b_buf.p;
       a.cap = c->b_buf.cap;
-      EA = carve_encode(a, es, W, true);
+      EA = carve_encode(a, es, W, true, (flags & LUDA_JOB_KEEP_DECODED) != 0);
     }
     void* out = c->take_out(out_cap);
     if (!out) throw Fail{LUDA_IO_ERROR, "output allocation failed"};
@@ -1483,6 +1639,14 @@ luda_result* build_tables(luda_ctx* c, const luda_options* opts, u64 n, const u8
     fill_kernel_stats(c, &stats, W, 0);
     luda_result* r = finish_result(c, es, EA, out, stB, flags, stats,
                                    reinterpret_cast<const FileMeta*>(c->pin.p + o_meta), c->pin.p + o_keys);
+    if (flags & LUDA_JOB_KEEP_DECODED) {
+      try {
+        keep_decoded(c, EA, W, stB.n_kept, r);
+      } catch (...) {
+        free_result_now(c, r);
+        throw;
+      }
+    }
     r->stats.total_ms = now_ms() - t_call;
     r->stats.end_clock_ms = now_ms();
     return r;
@@ -1660,6 +1824,7 @@ void pool_close(JobPool* P) {
   for (auto& d : P->done)
     if (d.result) {
       // undelivered results: return their buffers before the slots go
+      forget_decoded(d.result);
       if (d.result->device_data) {
         std::lock_guard<std::mutex> lk(d.owner->mu);
         d.owner->give_out(d.result->device_data);
@@ -1897,6 +2062,7 @@ int luda_build_tables(luda_ctx* c, const luda_options* opts, uint64_t n, const u
 
 void luda_free_result(luda_ctx* c, luda_result* r) {
   if (!r) return;
+  forget_decoded(r);
   if (c && r->device_data) {
     // a result of luda_submit returns its buffer to the slot that made it
     luda_ctx* owner = c;

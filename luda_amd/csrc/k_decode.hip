@@ -892,6 +892,28 @@ __global__ __launch_bounds__(64) void k_decode_groups(const FileDesc* files, con
   if (maxu) atomicMax(&st->max_ulen, maxu);
 }
 
+// Decoded entries a previous call left resident (luda_job inputs produced
+// with LUDA_JOB_KEEP_DECODED) copied into this job's arrays, in place of
+// decoding their tables: entry e belongs to the source whose range holds it;
+// plane words past the source's width are zero, as the decoder pads them.
+__global__ void k_gather_decoded(const DecodedSrc* src, u32 nsrc, u64 n, Planes K, u64* vptr, u32* vlen) {
+  GRID_STRIDE(e, n) {
+    u32 lo = 0, hi = nsrc;
+    while (hi - lo > 1) {
+      const u32 mid = (lo + hi) >> 1;
+      if (src[mid].dst <= e) lo = mid;
+      else hi = mid;
+    }
+    const DecodedSrc& s = src[lo];
+    const u64 j = s.first + (e - s.dst);
+    for (u32 w = 0; w < K.W; w++) K.P[(u64)w * K.stride + e] = w < s.W ? s.P[(u64)w * s.stride + j] : 0;
+    K.klen[e] = s.klen[j];
+    K.tag[e] = s.tag[j];
+    vptr[e] = s.vptr[j];
+    vlen[e] = s.vlen[j];
+  }
+}
+
 // Host entries -> kept stream (luda_build_tables).
 __global__ void k_ingest(u64 n, const u8* keys, const u64* key_off, const u8* vals, const u64* val_off, Kept out,
                          DevStatus* st) {
@@ -997,6 +1019,10 @@ void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 
     }
   }
   KLAUNCH(T, "decode", k_decode_groups, grid, 64, lds, files, blks, nb, blk_ebase, K, vptr, vlen, st);
+}
+void launch_gather_decoded(KTimer& T, const DecodedSrc* src, u32 nsrc, u64 n, Planes K, u64* vptr, u32* vlen) {
+  if (!n) return;
+  KLAUNCH(T, "gather_decoded", k_gather_decoded, grid_for(n, 256), 256, 0, src, nsrc, n, K, vptr, vlen);
 }
 void launch_ingest(KTimer& T, u64 n, const u8* keys, const u64* key_off, const u8* vals, const u64* val_off, Kept out,
                    DevStatus* st) {

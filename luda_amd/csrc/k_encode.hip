@@ -1406,6 +1406,38 @@ __global__ __launch_bounds__(kWbThreads) void k_write_blocks(Kept k, const u32* 
   }
 }
 
+// Where each kept entry's value lands in the output (for results kept
+// decoded): one wavefront per output block walks its entries as
+// assemble_block lays them out, entry header (three varints), key delta,
+// value, and writes the value's device address.
+__global__ __launch_bounds__(256) void k_value_addrs(Kept k, const u32* nbo_dev, EncParams ep, EncBufs b, u8* out,
+                                                     u64* vaddr) {
+  const u64 nbo = *nbo_dev;
+  const u32 lane = threadIdx.x & 63;
+  const u32 R = ep.interval;
+  for (u64 i = (u64)blockIdx.x * 4 + (threadIdx.x >> 6); i < nbo; i += (u64)gridDim.x * 4) {
+    const BlockMeta m = block_meta(b, i);
+    const u64 base_addr = (u64)out + b.f_off[m.file] + m.off;
+    u64 base = 0;
+    for (u32 j0 = 0; j0 < m.n; j0 += 64) {
+      const u32 j = j0 + lane;
+      const bool act = j < m.n;
+      const u64 q = (u64)m.first + j;
+      const bool rs = (j % R) == 0;
+      const u32 sz = act ? (rs ? b.szr[q] : b.sznr[q]) : 0;
+      const u32 incl = wave_incl_scan(sz);
+      if (act) {
+        const u32 il = k.K.klen[q] + 8;
+        const u32 sh = rs ? 0 : b.shared[q];
+        const u32 vl = k.vlen[q];
+        const u64 off = base + incl - sz;
+        vaddr[q] = base_addr + off + varint_len(sh) + varint_len(il - sh) + varint_len(vl) + (il - sh);
+      }
+      base += wave_last(incl);
+    }
+  }
+}
+
 // util/hash.cc over a user key held as big-endian words uk (klen <= 8 KW).
 template <u32 KW>
 __device__ __forceinline__ u32 hash_words(const u64 (&uk)[KW], u32 klen, u32 seed) {
@@ -1795,6 +1827,11 @@ void launch_write_blocks(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, Enc
     KLAUNCH(T, "write_blocks", k_write_blocks<9>, g, kWbThreads, 0, k, nbo_dev, ep, b, out, out_cap, st);
   else
     KLAUNCH(T, "write_blocks", k_write_blocks<0>, g, kWbThreads, 0, k, nbo_dev, ep, b, out, out_cap, st);
+}
+void launch_value_addrs(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u8* out,
+                        u64* vaddr) {
+  KLAUNCH(T, "value_addrs", k_value_addrs, std::min<u32>(grid_for(nbo_cap, 4), 8192), 256, 0, k, nbo_dev, ep, b, out,
+          vaddr);
 }
 void launch_write_filters(KTimer& T, Kept k, const u32* nw_dev, u32 nw_cap, const u32* nf_dev, EncParams ep, EncBufs b,
                           u8* out, u64 out_cap, DevStatus* st) {

@@ -22,6 +22,7 @@ LUDA_IO_ERROR = 5
 
 LUDA_JOB_INPUTS_ON_DEVICE = 0x1
 LUDA_JOB_KEEP_ON_DEVICE = 0x2
+LUDA_JOB_KEEP_DECODED = 0x8
 
 U64_MAX = (1 << 64) - 1
 

@@ -247,23 +247,27 @@ class Engine:
             if not keep_on_device:
                 self.lib.luda_free_result(self.ctx, res)
 
-    def compact(self, job, keep_on_device=False, keep_result=False):
+    def compact(self, job, keep_on_device=False, keep_result=False, keep_decoded=False):
         """(status, (tables, stats, result_ptr) | None, message).  With
         keep_on_device (outputs left in HBM) or keep_result (host copies read,
-        result kept) the result stays alive; free it with free()."""
+        result kept) the result stays alive; free it with free().
+        keep_decoded: LUDA_JOB_KEEP_DECODED."""
         res = ffi.RESULT_P()
         err = C.create_string_buffer(512)
         if keep_on_device:
             job.flags |= ffi.LUDA_JOB_KEEP_ON_DEVICE
+        if keep_decoded:
+            job.flags |= ffi.LUDA_JOB_KEEP_DECODED
         rc = self.lib.luda_compact(self.ctx, C.byref(job), C.byref(res), err, 512)
         if keep_result and rc == ffi.LUDA_OK:
             return rc, (result_tables(res), _stats_dict(res.contents.stats), res), ""
         return self._finish(rc, res, err, keep_on_device)
 
-    def build_tables(self, opts_p, n, keys, key_off, vals, val_off, keep_on_device=False):
+    def build_tables(self, opts_p, n, keys, key_off, vals, val_off, keep_on_device=False, keep_decoded=False):
         res = ffi.RESULT_P()
         err = C.create_string_buffer(512)
-        flags = ffi.LUDA_JOB_KEEP_ON_DEVICE if keep_on_device else 0
+        flags = (ffi.LUDA_JOB_KEEP_ON_DEVICE if keep_on_device else 0) | \
+            (ffi.LUDA_JOB_KEEP_DECODED if keep_decoded else 0)
         rc = self.lib.luda_build_tables(self.ctx, opts_p, n, keys, key_off, vals, val_off, flags, C.byref(res), err,
                                         512)
         return self._finish(rc, res, err, keep_on_device)
@@ -296,18 +300,21 @@ class Engine:
         if rc != ffi.LUDA_OK:
             raise RuntimeError(f"luda_set_slots failed ({rc})")
 
-    def submit(self, job, on_done, keep_on_device=False):
+    def submit(self, job, on_done, keep_on_device=False, keep_decoded=False):
         """Queues `job`; on_done(status, (tables, stats, result_ptr) | None,
         message) runs from poll()/drain().  The job must stay alive until then."""
         if keep_on_device:
             job.flags |= ffi.LUDA_JOB_KEEP_ON_DEVICE
+        if keep_decoded:
+            job.flags |= ffi.LUDA_JOB_KEEP_DECODED
         return self._queue(lambda uid: self.lib.luda_submit(self.ctx, C.byref(job), C.cast(self._done_fn, C.c_void_p),
                                                             C.c_void_p(uid)), on_done, job, keep_on_device)
 
-    def submit_build(self, opts, n, keys, key_off, vals, val_off, on_done, keep_on_device=False):
+    def submit_build(self, opts, n, keys, key_off, vals, val_off, on_done, keep_on_device=False, keep_decoded=False):
         """Queues a flush (luda_submit_build); the arrays must stay alive until
         on_done runs."""
-        flags = ffi.LUDA_JOB_KEEP_ON_DEVICE if keep_on_device else 0
+        flags = (ffi.LUDA_JOB_KEEP_ON_DEVICE if keep_on_device else 0) | \
+            (ffi.LUDA_JOB_KEEP_DECODED if keep_decoded else 0)
         hold = (opts, keys, key_off, vals, val_off)
         return self._queue(lambda uid: self.lib.luda_submit_build(
             self.ctx, C.byref(opts), n, keys, key_off, vals, val_off, flags, C.cast(self._done_fn, C.c_void_p),

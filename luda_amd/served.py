@@ -515,11 +515,14 @@ def _job_struct(spec, opts):
 
 
 class EngineBackend:
-    """Flushes and compactions on a luda context; tables stay in HBM."""
+    """Flushes and compactions on a luda context; tables stay in HBM, and with
+    `resident` their decoded entries too (LUDA_JOB_KEEP_DECODED), so that a
+    compaction of tables the engine wrote decodes nothing."""
 
-    def __init__(self, engine, opts):
+    def __init__(self, engine, opts, resident=True):
         self.eng = engine
         self.opts = opts
+        self.resident = resident
         self.flush_opts = ffi.luda_options()
         C.pointer(self.flush_opts)[0] = opts
         self.flush_opts.max_file_size = (1 << 64) - 1  # BuildTable writes one table
@@ -554,7 +557,8 @@ class EngineBackend:
     def flush(self, lsm, mem):
         k, ko, v, vo = mem.arrays()
         rc, out, msg = self.eng.build_tables(C.byref(self.flush_opts), mem.n, k.ctypes.data, ko.ctypes.data,
-                                             v.ctypes.data, vo.ctypes.data, keep_on_device=True)
+                                             v.ctypes.data, vo.ctypes.data, keep_on_device=True,
+                                             keep_decoded=self.resident)
         if rc != ffi.LUDA_OK:
             raise RuntimeError(f"luda_build_tables failed ({rc}): {msg}")
         self._account(out[1])
@@ -562,7 +566,7 @@ class EngineBackend:
 
     def compact(self, lsm, spec):
         job, keep = _job_struct(spec, self.opts)
-        rc, out, msg = self.eng.compact(job, keep_on_device=True)
+        rc, out, msg = self.eng.compact(job, keep_on_device=True, keep_decoded=self.resident)
         if rc != ffi.LUDA_OK:
             raise RuntimeError(f"luda_compact failed ({rc}): {msg}")
         self._account(out[1])
@@ -579,7 +583,7 @@ class EngineBackend:
             self._account(out[1])
             done(self._files(lsm, out[2]), out[1]["end_clock_ms"], hold)
         self.eng.submit_build(self.flush_opts, mem.n, k.ctypes.data, ko.ctypes.data, v.ctypes.data, vo.ctypes.data,
-                              cb, keep_on_device=True)
+                              cb, keep_on_device=True, keep_decoded=self.resident)
 
     def submit_compact(self, lsm, spec, done):
         job, keep = _job_struct(spec, self.opts)
@@ -590,7 +594,7 @@ class EngineBackend:
                 raise RuntimeError(f"queued compaction failed ({rc}): {msg}")
             self._account(out[1])
             done(self._files(lsm, out[2]), out[1]["end_clock_ms"])
-        self.eng.submit(job, cb, keep_on_device=True)
+        self.eng.submit(job, cb, keep_on_device=True, keep_decoded=self.resident)
 
     def poll(self):
         return self.eng.poll()
