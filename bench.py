@@ -277,8 +277,10 @@ def run_serve(args):
     peng = jobs.Engine(0)
     seng = jobs.Engine(0)
     seng.set_slots(args.slots)
-    pbe = served.EngineBackend(peng, opts)
-    sbe = served.EngineBackend(seng, opts)
+    # the plain path is the drop-in as it is: every job decodes its inputs;
+    # the served form keeps decoded entries resident between chained jobs
+    pbe = served.EngineBackend(peng, opts, resident=False)
+    sbe = served.EngineBackend(seng, opts, resident=True)
     # warm-up: the whole stream through both paths, so that every working
     # buffer and output buffer is allocated before anything is timed
     for be, run in ((pbe, served.run_plain), (sbe, served.run_served)):

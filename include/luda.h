@@ -45,8 +45,10 @@ extern "C" {
 /* Keep the output tables' decoded entries (key planes, tags, value addresses
  * in the tables) resident in HBM with the result, until luda_free_result.  A
  * later luda_compact whose inputs are all such tables, passed by their device
- * addresses (LUDA_JOB_INPUTS_ON_DEVICE), takes their entries from there and
- * decodes nothing: the chained level compactions of a served DB.  The
+ * addresses (LUDA_JOB_INPUTS_ON_DEVICE) to the context that wrote them (its
+ * worker slots included; other contexts decode them as usual), takes their
+ * entries from there and decodes nothing: the chained level compactions of a
+ * served DB.  The
  * reference keeps the decoded entries of flushed level-0 tables in host memory
  * for the next L0->L1 compaction the same way (cuda/cuda_common.h:253-255,
  * 339-355; db/db_impl.cc:146-149,1292-1295).  Also a flag of

@@ -230,6 +230,7 @@ struct luda_ctx {
   }
   std::map<void*, u64> pool_sizes;
   IoState* io = nullptr;  // file I/O stream and staging (io.hip), on first use
+  luda_ctx* family = nullptr;  // the context luda_init returned: itself, or a worker slot's owner
 };
 
 int ctx_device_of(luda_ctx* c) { return c->device; }
@@ -616,6 +617,7 @@ luda_result* finish_result(luda_ctx* c, const EncodeSetup& s, const EncodeArrays
 // registered tables gathers their entries instead of decoding them.
 
 struct DecodedSet {
+  const luda_ctx* family;
   int device;
   void* buf;
   u32 W;
@@ -630,8 +632,15 @@ struct ResidentTable {
   DecodedSet* d;
   u64 first, count, size;
 };
+// Tables are registered for the context family that wrote them (a context
+// and its worker slots): another context's jobs decode them as usual.
+struct ResidentKey {
+  const luda_ctx* family;
+  u64 addr;
+  bool operator<(const ResidentKey& o) const { return family != o.family ? family < o.family : addr < o.addr; }
+};
 std::mutex g_resident_mu;
-std::map<std::pair<int, u64>, ResidentTable> g_resident;  // (device, table address)
+std::map<ResidentKey, ResidentTable> g_resident;
 std::map<const luda_result*, DecodedSet*> g_decoded_of;
 
 // A result this context just built, released on a failure after it was
@@ -648,6 +657,7 @@ void free_result_now(luda_ctx* c, luda_result* r) {
 // context's working memory, and its tables registered.
 void keep_decoded(luda_ctx* c, const EncodeArrays& A, u32 W, u64 m, luda_result* r) {
   DecodedSet* d = new DecodedSet;
+  d->family = c->family;
   d->device = c->device;
   d->W = W;
   d->m = m;
@@ -683,7 +693,7 @@ void keep_decoded(luda_ctx* c, const EncodeArrays& A, u32 W, u64 m, luda_result*
   u64 first = 0;
   for (u32 i = 0; i < r->num_outputs; i++) {
     const luda_output& o = r->outputs[i];
-    g_resident[{c->device, (u64)r->device_data + o.offset}] = ResidentTable{d, first, o.num_entries, o.file_size};
+    g_resident[ResidentKey{c->family, (u64)r->device_data + o.offset}] = ResidentTable{d, first, o.num_entries, o.file_size};
     first += o.num_entries;
   }
   g_decoded_of[r] = d;
@@ -697,20 +707,20 @@ void forget_decoded(const luda_result* r) {
     if (it == g_decoded_of.end()) return;
     d = it->second;
     g_decoded_of.erase(it);
-    for (u32 i = 0; i < r->num_outputs; i++) g_resident.erase({d->device, (u64)r->device_data + r->outputs[i].offset});
+    for (u32 i = 0; i < r->num_outputs; i++) g_resident.erase(ResidentKey{d->family, (u64)r->device_data + r->outputs[i].offset});
   }
   hipSetDevice(d->device);
   hipFree(d->buf);
   delete d;
 }
 
-// Every input table registered (same device, same size): their sources, in
-// input order.
-bool find_resident(int device, const std::vector<const luda_file*>& files, std::vector<DecodedSrc>* out) {
+// Every input table registered for this context family (same size): their
+// sources, in input order.
+bool find_resident(const luda_ctx* family, const std::vector<const luda_file*>& files, std::vector<DecodedSrc>* out) {
   std::lock_guard<std::mutex> lk(g_resident_mu);
   out->clear();
   for (const luda_file* f : files) {
-    auto it = g_resident.find({device, (u64)f->data});
+    auto it = g_resident.find(ResidentKey{family, (u64)f->data});
     if (it == g_resident.end() || it->second.size != f->size) return false;
     const ResidentTable& t = it->second;
     DecodedSrc s;
@@ -819,7 +829,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
   // (LUDA_JOB_KEEP_DECODED): when every input is one, phase A and the decode
   // are skipped and their entries gathered instead.
   std::vector<DecodedSrc> dsrc;
-  const bool resident = (job->flags & LUDA_JOB_INPUTS_ON_DEVICE) && find_resident(c->device, files, &dsrc);
+  const bool resident = (job->flags & LUDA_JOB_INPUTS_ON_DEVICE) && find_resident(c->family, files, &dsrc);
   FileDesc* d_files = nullptr;
   FileInfo* d_info = nullptr;
   u32 *d_nblk = nullptr, *d_bbase = nullptr, *d_nent = nullptr, *d_ebase = nullptr, *d_febase = nullptr;
@@ -1803,6 +1813,7 @@ JobPool* pool_of(luda_ctx* c) {
     luda_ctx* w = nullptr;
     char err[256];
     if (ctx_open(c->device, c->budget, &w, err, sizeof(err)) != LUDA_OK) break;
+    w->family = c;
     P->ctxs.push_back(w);
   }
   if (P->ctxs.empty()) {
@@ -1933,6 +1944,7 @@ int ctx_open(int device, uint64_t hbm_budget_bytes, luda_ctx** out, char* err, s
     return LUDA_IO_ERROR;
   }
   c->device = device;
+  c->family = c;
   c->budget = hbm_budget_bytes ? hbm_budget_bytes : (64ull << 30);
   int rc = guarded(c, err, errlen, [&] {
     fault_point("init");
