@@ -356,6 +356,13 @@ void launch_windows(KTimer& T, const u32* nbo_dev, u32 nbo_cap, const u32* nf_de
                     EncBufs b);
 void launch_write_blocks(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u8* out,
                          u64 out_cap, DevStatus* st);
+// Jobs of at most cut_small_max_blocks() output blocks: block offsets and
+// table cuts (O, f_first, n_files_out, data_bytes) in one workgroup, the
+// work of scan_block_offsets, scan_forced, launch_fnext, the table orbit and
+// scan_files.
+u32 cut_small_max_blocks();
+void launch_cut_small(KTimer& T, const u32* nbo_dev, EncParams ep, EncBufs b, u32 nf_cap, bool size_cuts,
+                      DevStatus* st);
 // vaddr[q] = device address of kept entry q's value in the written tables
 void launch_value_addrs(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u8* out,
                         u64* vaddr);

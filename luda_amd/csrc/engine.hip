@@ -382,6 +382,8 @@ u32 grid_for(u64 n, u32 per) {
   return (u32)std::min<u64>(std::max<u64>(g, 1), 16384);
 }
 
+void run_cuts_large(luda_ctx* c, const EncodeSetup& s, const EncodeArrays& A);
+
 // The encoder over the kept stream, counts in st.
 void run_encode(luda_ctx* c, const EncodeSetup& s, const EncodeArrays& A, u8* out) {
   KTimer& T = c->T;
@@ -399,15 +401,12 @@ void run_encode(luda_ctx* c, const EncodeSetup& s, const EncodeArrays& A, u8* ou
   launch_bnext(T, k, m_dev, s.m_cap, s.ep, b, st);
   run_orbit(T, s.ob, b.bnext, m_dev, &st->blk_step_max, b.bs, A.ftab_b, A.ent_b, st);
   scan_blocks(T, SC, k, m_dev, s.m_cap, b, s.nbo_cap, st);
-  scan_block_offsets(T, SC, nbo_dev, s.nbo_cap, b, st);
-  if (s.size_cuts) {
-    scan_forced(T, SC, nbo_dev, s.nbo_cap, b, st);
-    launch_fnext(T, nbo_dev, s.nbo_cap, &st->n_forced, s.ep, b, st);
-    run_orbit(T, s.of, b.fnext, nbo_dev, &st->file_step_max, b.fs, A.ftab_f, A.ent_f, st);
+  if (s.nbo_cap <= cut_small_max_blocks()) {
+    // a small job: block offsets and table cuts in one launch
+    launch_cut_small(T, nbo_dev, s.ep, b, s.nf_cap, s.size_cuts, st);
   } else {
-    KLAUNCH(T, "table_starts", k_table_starts_forced, grid_for(s.nbo_cap, 256), 256, 0, nbo_dev, b);
+    run_cuts_large(c, s, A);
   }
-  scan_files(T, SC, nbo_dev, s.nbo_cap, b, s.nf_cap, st);
   launch_block_layout(T, k, nbo_dev, s.nbo_cap, nf_dev, s.ep, b);
   scan_u64(T, SC, b.ob_isz, b.ob_isz, nbo_dev, 1, s.nbo_cap + 1, nullptr, st);
   scan_windows(T, SC, nf_dev, s.nf_cap, s.ep, b, s.nw_cap, st);
@@ -420,6 +419,25 @@ void run_encode(luda_ctx* c, const EncodeSetup& s, const EncodeArrays& A, u8* ou
   if (s.ep.bloom_bits) launch_write_filters(T, k, nw_dev, s.nw_cap, nf_dev, s.ep, b, out, s.out_cap, st);
   launch_write_tail(T, k, nf_dev, s.nf_cap, s.ep, b, out, s.out_cap, st);
   if (A.vaddr) launch_value_addrs(T, k, nbo_dev, s.nbo_cap, s.ep, b, out, A.vaddr);
+}
+
+// Block offsets and table cuts of a larger job: scans, table successors and
+// the table orbit over the whole device.
+void run_cuts_large(luda_ctx* c, const EncodeSetup& s, const EncodeArrays& A) {
+  KTimer& T = c->T;
+  DevStatus* st = c->st;
+  const EncBufs& b = A.eb;
+  const u32* nbo_dev = &st->n_blocks_out;
+  ScanState& SC = c->scans;
+  scan_block_offsets(T, SC, nbo_dev, s.nbo_cap, b, st);
+  if (s.size_cuts) {
+    scan_forced(T, SC, nbo_dev, s.nbo_cap, b, st);
+    launch_fnext(T, nbo_dev, s.nbo_cap, &st->n_forced, s.ep, b, st);
+    run_orbit(T, s.of, b.fnext, nbo_dev, &st->file_step_max, b.fs, A.ftab_f, A.ent_f, st);
+  } else {
+    KLAUNCH(T, "table_starts", k_table_starts_forced, grid_for(s.nbo_cap, 256), 256, 0, nbo_dev, b);
+  }
+  scan_files(T, SC, nbo_dev, s.nbo_cap, b, s.nf_cap, st);
 }
 
 // Orbit chunk for successor steps of at most `step` nodes: a chunk must hold

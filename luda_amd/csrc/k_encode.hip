@@ -18,6 +18,7 @@
 namespace {
 
 constexpr u32 kThreads = 256;
+constexpr u32 kCutSmallBlocks = 16384;  // output blocks of a job whose table cuts run in one workgroup
 constexpr u32 kWaves = kThreads / 64;
 constexpr u32 kStage = 6144;   // bytes of one data block assembled in LDS per wave
 constexpr u32 kFBits = 4096;   // bytes of one bloom filter assembled in LDS per wave
@@ -440,22 +441,25 @@ __device__ __forceinline__ u32 file_of_block(const EncBufs& b, u32 nf, u64 i) {
   return lo;
 }
 
+// Block i's table, offset in it and index entry size (nf >= 1 tables).
+__device__ __forceinline__ void block_layout_item(const Kept& k, const EncBufs& b, u32 nf, u64 i) {
+  const u32 f = file_of_block(b, nf, i);
+  const u64 off = b.O[i] - b.O[b.f_first[f]];
+  b.ob_file[i] = f;
+  b.ob_off[i] = off;
+  const u64 last = (u64)b.ob_first[i] + b.ob_n[i] - 1;
+  const u32 kl = (i + 1 < b.f_first[f + 1]) ? index_sep(k.K, last, b.ob_first[i + 1], nullptr)
+                                            : index_succ(k.K, last, nullptr);
+  const u32 hl = varint_len(off) + varint_len(b.ob_bytes[i]);
+  b.ob_isz[i] = 1 + varint_len(kl) + varint_len(hl) + kl + hl;
+}
+
 __global__ void k_block_layout(Kept k, const u32* nbo_dev, const u32* nf_dev, EncParams ep, EncBufs b) {
   const u64 nbo = *nbo_dev;
   const u32 nf = *nf_dev;
   if (blockIdx.x == 0 && threadIdx.x == 0) b.ob_isz[nbo] = 0;  // the end of the index size scan
   if (nf == 0) return;  // no tables, or their count overflowed (reported)
-  GRID_STRIDE(i, nbo) {
-    const u32 f = file_of_block(b, nf, i);
-    const u64 off = b.O[i] - b.O[b.f_first[f]];
-    b.ob_file[i] = f;
-    b.ob_off[i] = off;
-    const u64 last = (u64)b.ob_first[i] + b.ob_n[i] - 1;
-    const u32 kl = (i + 1 < b.f_first[f + 1]) ? index_sep(k.K, last, b.ob_first[i + 1], nullptr)
-                                              : index_succ(k.K, last, nullptr);
-    const u32 hl = varint_len(off) + varint_len(b.ob_bytes[i]);
-    b.ob_isz[i] = 1 + varint_len(kl) + varint_len(hl) + kl + hl;
-  }
+  GRID_STRIDE(i, nbo) block_layout_item(k, b, nf, i);
 }
 
 // Filter windows of a table: FilterBlockBuilder emits one filter per 2 KiB of
@@ -490,6 +494,8 @@ __device__ __forceinline__ u64 filter_bytes(u64 keys, u32 bits_per_key) {
 // first block of a window writes the window (its keys run to the first block
 // of a later window); the last block of a window also writes the empty
 // windows up to the next block's window, or to the end of its table.
+__device__ __forceinline__ void windows_item(const EncParams& ep, const EncBufs& b, u64 nbo, u32 nw, u64 i);
+
 __global__ void k_windows(const u32* nbo_dev, const u32* nf_dev, const u32* nw_dev, EncParams ep, EncBufs b) {
   const u64 nbo = *nbo_dev;
   const u32 nw = *nw_dev;  // 0 when the window count overflowed its capacity
@@ -499,7 +505,12 @@ __global__ void k_windows(const u32* nbo_dev, const u32* nf_dev, const u32* nw_d
     b.w_fsz[nw] = 0;
   }
   if (*nf_dev == 0 || nw == 0) return;
-  GRID_STRIDE(i, nbo) {
+  GRID_STRIDE(i, nbo) windows_item(ep, b, nbo, nw, i);
+}
+
+// The windows block i starts or ends (see k_windows).
+__device__ __forceinline__ void windows_item(const EncParams& ep, const EncBufs& b, u64 nbo, u32 nw, u64 i) {
+  {
     const u32 f = b.ob_file[i];
     const u32 wf = b.f_win[f];
     const u64 sw = b.ob_off[i] >> 11;
@@ -558,6 +569,139 @@ struct FileSizesLoad {
     return size + meta_block_size(ep, dend, fb) + 5 + ib + 5 + 48;
   }
 };
+
+// ---------------------------------------------------------------------------
+// The block offsets and the table cuts of a job of at most kCutSmallBlocks
+// output blocks, in one workgroup of one launch: the same results (O, the
+// tables' first blocks, their count) as the scans, table successors and
+// table orbit that larger jobs run over the device in seven launches of a
+// few microseconds each, whatever the size.  The offsets are scanned over the
+// workgroup with a carry from chunk to chunk; the cuts walk the tables in
+// order, each found by the whole workgroup testing up to 1024 blocks at once
+// against the block offsets and forced-block bits held in LDS.
+
+constexpr u32 kCutThreads = 1024;
+constexpr u32 kCutWaves = kCutThreads / 64;
+
+// Exclusive scan of ld(i), i < n, by the whole workgroup; out(i, prefix) for
+// every item, in any order (every item of a chunk is loaded before any of its
+// outputs is written, so the scan may run in place); returns the total.
+template <typename T, typename Load, typename Out>
+__device__ T wg_scan(u64 n, Load ld, Out out, T* s_w) {
+  const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  T carry = 0;
+  for (u64 base = 0; base < n; base += 4ull * kCutThreads) {
+    T v[4];
+    T sum = 0;
+#pragma unroll
+    for (u32 j = 0; j < 4; j++) {
+      const u64 i = base + 4ull * tid + j;
+      v[j] = i < n ? (T)ld(i) : T(0);
+      sum += v[j];
+    }
+    const T incl = wave_incl_sum(sum);
+    if (lane == 63) s_w[wave] = incl;
+    __syncthreads();
+    T before = 0, tot = 0;
+    for (u32 w = 0; w < kCutWaves; w++) {
+      if (w < wave) before += s_w[w];
+      tot += s_w[w];
+    }
+    T ex = carry + before + incl - sum;
+#pragma unroll
+    for (u32 j = 0; j < 4; j++) {
+      const u64 i = base + 4ull * tid + j;
+      if (i < n) out(i, ex);
+      ex += v[j];
+    }
+    carry += tot;
+    __syncthreads();
+  }
+  return carry;
+}
+
+__global__ __launch_bounds__(kCutThreads) void k_cut_small(const u32* nbo_dev, EncParams ep, EncBufs b, u32 nf_cap,
+                                                           u32 size_cuts, DevStatus* st) {
+  __shared__ u32 sO[kCutSmallBlocks + 1];         // block offsets from block 0, when they fit 32 bits
+  __shared__ u32 sF[kCutSmallBlocks / 32 + 1];    // forced blocks, one bit each
+  __shared__ u64 s_w64[kCutWaves];
+  __shared__ u32 s_w32[kCutWaves];
+  __shared__ u32 s_hit;
+  const u32 tid = threadIdx.x;
+  const u64 nbo = *nbo_dev;
+  if (nbo > kCutSmallBlocks) {  // the host only picks this kernel for capacities this small
+    if (tid == 0) set_err(st, ERR_UNSUPPORTED, SITE_CAPACITY);
+    return;
+  }
+  for (u64 w = tid; w <= nbo / 32; w += kCutThreads) sF[w] = 0;
+  __syncthreads();
+  // block offsets (the data block bytes with their trailers), and the forced
+  // blocks' bits from the same loads
+  const u64 data = wg_scan<u64>(
+      nbo + 1,
+      [&](u64 i) -> u64 {
+        if (i >= nbo) return 0;
+        const u32 by = b.ob_bytes[i];
+        if (b.ob_forced[i]) atomicOr(&sF[i >> 5], 1u << (i & 31));
+        return (u64)by + 5;
+      },
+      [&](u64 i, u64 ex) {
+        b.O[i] = ex;
+        sO[i] = (u32)ex;
+      },
+      s_w64);
+  const bool wide = data >> 32;  // offsets past 4 GiB: read them from global memory
+  if (tid == 0) st->data_bytes = data;
+  __syncthreads();
+  auto offset = [&](u64 i) -> u64 { return wide ? b.O[i] : (u64)sO[i]; };
+  auto forced = [&](u64 i) -> bool { return (sF[i >> 5] >> (i & 31)) & 1u; };
+  // tables: a table starting at block t ends before the first block j > t
+  // with O[j] >= O[t] + max_file_size (FileSize() reached the limit after
+  // block j - 1, db/db_impl.cc:1173-1180) or that ShouldStopBefore starts,
+  // or at the end
+  u64 nf = 0;
+  if (size_cuts) {
+    u64 t = 0;
+    while (t < nbo) {
+      if (tid == 0 && nf < nf_cap) b.f_first[nf] = (u32)t;
+      nf++;
+      const u64 ot = offset(t);
+      const u64 target = ep.max_file_size > ~0ull - ot ? ~0ull : ot + ep.max_file_size;
+      for (u64 j0 = t + 1;; j0 += kCutThreads) {
+        if (tid == 0) s_hit = 0xffffffffu;
+        __syncthreads();
+        const u64 j = j0 + tid;
+        if (j >= nbo || offset(j) >= target || forced(j)) atomicMin(&s_hit, tid);
+        __syncthreads();
+        const u32 h = s_hit;
+        __syncthreads();  // everyone has read it before it is reset
+        if (h != 0xffffffffu) {
+          t = min(j0 + h, nbo);
+          break;
+        }
+      }
+    }
+  } else {
+    // tables start at block 0 and at every block ShouldStopBefore starts
+    nf = wg_scan<u32>(
+        nbo, [&](u64 i) { return (u32)(i == 0 || forced(i)); },
+        [&](u64 i, u32 ex) {
+          if ((i == 0 || forced(i)) && ex < nf_cap) b.f_first[ex] = (u32)i;
+        },
+        s_w32);
+  }
+  if (nf > nf_cap) {  // reported for the host's retry; nothing is written
+    if (tid == 0) {
+      st->need_files = (u32)nf;
+      set_err(st, ERR_OVERFLOW, SITE_CAPACITY);
+    }
+    nf = 0;
+  }
+  if (tid == 0) {
+    st->n_files_out = (u32)nf;
+    b.f_first[nf] = (u32)nbo;
+  }
+}
 
 // ---------------------------------------------------------------------------
 // writers
@@ -1827,6 +1971,11 @@ void launch_write_blocks(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, Enc
     KLAUNCH(T, "write_blocks", k_write_blocks<9>, g, kWbThreads, 0, k, nbo_dev, ep, b, out, out_cap, st);
   else
     KLAUNCH(T, "write_blocks", k_write_blocks<0>, g, kWbThreads, 0, k, nbo_dev, ep, b, out, out_cap, st);
+}
+u32 cut_small_max_blocks() { return kCutSmallBlocks; }
+void launch_cut_small(KTimer& T, const u32* nbo_dev, EncParams ep, EncBufs b, u32 nf_cap, bool size_cuts,
+                      DevStatus* st) {
+  KLAUNCH(T, "cut_small", k_cut_small, 1, kCutThreads, 0, nbo_dev, ep, b, nf_cap, size_cuts ? 1u : 0u, st);
 }
 void launch_value_addrs(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u8* out,
                         u64* vaddr) {
