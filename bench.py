@@ -139,6 +139,23 @@ def run_mix(args):
 
     def summary(res):
         return jobs.device_result_tables(eng, res)
+    tuning = None
+    slots = args.slots
+    if args.tune:
+        # the autotuner measures on a different mix (another seed), then the
+        # batched form below runs with what it chose
+        from luda_amd import autotune
+        t = time.time()
+        tmix = mix_jobs(eng, opts, args.tune, args.mix_seed + 1)
+        ttotal = sum(b for _, b, _, _, _ in tmix)
+        tuning = autotune.autotune_batched([j.job for _, _, j, _, _ in tmix], ttotal, log=log)
+        tuning["sample"] = f"{len(tmix)} jobs of the mix with seed {args.mix_seed + 1}, {ttotal / 2**30:.2f} GiB"
+        slots = tuning["slots"]
+        log(f"autotune: {tuning['slots']} slots, rules {tuning['rules']} ({tuning['gbps']} GB/s on the sample) "
+            f"in {time.time() - t:.1f}s")
+        for _, _, _, keep, _ in tmix:
+            for res in keep:
+                eng.free(res)
     # plain: one luda_compact after another
     for shape, b, job, keep, wl in mix[:2]:  # warm-up
         rc, o, msg = eng.compact(job.job, keep_on_device=True)
@@ -156,8 +173,12 @@ def run_mix(args):
     golden = mix_golden(args.mix, args.mix_seed)
     # batched: everything queued at once on the worker slots
     beng = jobs.Engine(0)
-    beng.set_slots(args.slots)
-    for shape, b, job, keep, wl in mix[:2]:  # warm-up of every slot's buffers
+    beng.set_slots(slots)
+    if tuning:
+        beng.set_tuning(tuning["rules"])
+    # warm-up: the largest jobs, one per slot, so that every slot's working
+    # memory has its final size before the timed batch
+    for shape, b, job, keep, wl in sorted(mix, key=lambda x: -x[1])[:slots]:
         beng.submit(job.job, lambda rc, o, msg: beng.free(o[2]), keep_on_device=True)
     beng.drain()
     got = {}
@@ -179,7 +200,7 @@ def run_mix(args):
         cpu = mix_cpu_baseline(eng, opts, mix, plain_hash)
     line = {
         "metric": "batched compaction input GB/s vs the plain path; p50/p99 job latency",
-        "jobs": len(mix), "input_bytes": total, "slots": args.slots,
+        "jobs": len(mix), "input_bytes": total, "slots": slots, "tuning": tuning,
         "plain_gbps": round(total / t_plain / 1e9, 3), "batched_gbps": round(total / t_batch / 1e9, 3),
         "speedup": round(t_plain / t_batch, 3),
         "plain_p50_ms": round(percentile(plain_lat, 50), 3), "plain_p99_ms": round(percentile(plain_lat, 99), 3),
@@ -379,6 +400,9 @@ def main():
                          "plain path, one JSON line of aggregate GB/s and job latency")
     ap.add_argument("--mix-seed", type=int, default=3)
     ap.add_argument("--slots", type=int, default=6)
+    ap.add_argument("--tune", type=int, default=0, metavar="JOBS",
+                    help="with --mix: first autotune the batched form (slots, launch shapes per job class) on a "
+                         "mix of JOBS jobs with the next seed, then measure with its choice")
     ap.add_argument("--serve", type=int, default=0, metavar="PUTS",
                     help="instead: a fillrandom stream of PUTS puts through the served form against the plain "
                          "path at --rates multiples of the plain path's sustainable rate, one JSON line")

@@ -237,6 +237,27 @@ int luda_read_tables(luda_ctx* ctx, const luda_file* tables, uint32_t n, uint32_
                      char* err, size_t errlen);
 void luda_free_entries(luda_entries* e);
 
+/* ---- Tuning ------------------------------------------------------------------
+ * Launch shapes a caller (or luda_amd/autotune.py, which measures them) can
+ * set per class of job.  A rule applies to jobs whose input bytes lie in
+ * [min_bytes, max_bytes) and whose longest user key, rounded up to 8 bytes,
+ * lies in [min_key_bytes, max_key_bytes]; the last matching rule wins.  A
+ * zero field keeps the engine's default. */
+typedef struct luda_tuning {
+  uint32_t write_blocks_groups; /* workgroups (of 16 waves) of the data block writer; default one per CU (256) */
+  uint32_t decode_groups;       /* wavefronts of the block decoder; default min(blocks / 4, 4096) */
+  uint32_t reserved[2];
+} luda_tuning;
+typedef struct luda_tuning_rule {
+  uint64_t min_bytes;
+  uint64_t max_bytes;
+  uint32_t min_key_bytes;
+  uint32_t max_key_bytes;
+  luda_tuning tuning;
+} luda_tuning_rule;
+/* Replaces the context's rules (n = 0 clears them); worker slots follow. */
+int luda_set_tuning(luda_ctx* ctx, const luda_tuning_rule* rules, uint32_t n);
+
 /* Batched and served compaction: jobs queued on the context run on worker
  * slots, each with its own HIP stream and working memory, so that the
  * kernels of several jobs overlap on the device.  A queued job starts when a

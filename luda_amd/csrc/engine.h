@@ -279,8 +279,9 @@ void launch_index(KTimer& T, const FileDesc* files, const FileInfo* info, const 
 // st->n_entries, and file_ebase[f] = the first entry of table f (f <= nfiles)
 void scan_entries(KTimer& T, ScanState& S, const BlockDesc* blks, const u32* blk_nent, u32* blk_ebase, u32 nb_cap,
                   u32 nfiles, u32* file_ebase, DevStatus* st);
+// groups: wavefronts of the decoder, 0 = the default
 void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 nb, const u32* blk_ebase, Planes K,
-                   u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr, u32 max_ent);
+                   u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr, u32 max_ent, u32 groups = 0);
 void launch_ingest(KTimer& T, u64 n, const u8* keys, const u64* key_off, const u8* vals, const u64* val_off, Kept out,
                    DevStatus* st);
 // entries 0 .. n-1 from resident decoded sources (src sorted by dst)
@@ -354,8 +355,9 @@ void launch_block_layout(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, con
                          EncBufs b);
 void launch_windows(KTimer& T, const u32* nbo_dev, u32 nbo_cap, const u32* nf_dev, const u32* nw_dev, EncParams ep,
                     EncBufs b);
+// groups: workgroups of the block writer, 0 = one per CU
 void launch_write_blocks(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u8* out,
-                         u64 out_cap, DevStatus* st);
+                         u64 out_cap, DevStatus* st, u32 groups = 0);
 // Jobs of at most cut_small_max_blocks() output blocks: block offsets and
 // table cuts (O, f_first, n_files_out, data_bytes) in one workgroup, the
 // work of scan_block_offsets, scan_forced, launch_fnext, the table orbit and

@@ -178,6 +178,7 @@ struct EncodeSetup {
   u64 out_cap;    // output bytes
   bool size_cuts; // max_file_size can cut a table before the stream ends
   OrbitPlan ob, of;
+  u32 wb_groups;   // block writer workgroups (0 = default)
 };
 
 }  // namespace
@@ -231,6 +232,7 @@ struct luda_ctx {
   std::map<void*, u64> pool_sizes;
   IoState* io = nullptr;  // file I/O stream and staging (io.hip), on first use
   luda_ctx* family = nullptr;  // the context luda_init returned: itself, or a worker slot's owner
+  std::vector<luda_tuning_rule> rules;  // luda_set_tuning
 };
 
 int ctx_device_of(luda_ctx* c) { return c->device; }
@@ -274,7 +276,15 @@ int status_code_of(const DevStatus& st) {
 void stream_sync(luda_ctx* c) {
   const double t0 = now_ms();
   hipError_t e;
-  while ((e = hipStreamQuery(c->s)) == hipErrorNotReady && now_ms() - t0 < 2.0) {
+  if (c->family != c) {
+    // a worker slot: the other slots' threads are enqueueing work meanwhile,
+    // and a tight polling loop would contend with them for the runtime, so
+    // this one sleeps between polls
+    while ((e = hipStreamQuery(c->s)) == hipErrorNotReady && now_ms() - t0 < 2.0)
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+  } else {
+    while ((e = hipStreamQuery(c->s)) == hipErrorNotReady && now_ms() - t0 < 2.0) {
+    }
   }
   if (e == hipErrorNotReady) e = hipStreamSynchronize(c->s);
   HIPCHK(e);
@@ -415,7 +425,7 @@ void run_encode(luda_ctx* c, const EncodeSetup& s, const EncodeArrays& A, u8* ou
     scan_u64(T, SC, b.w_fsz, b.w_fsz, nw_dev, 1, s.nw_cap + 1, nullptr, st);
   }
   scan_file_offsets(T, SC, nf_dev, s.nf_cap, s.ep, b, st);
-  launch_write_blocks(T, k, nbo_dev, s.nbo_cap, s.ep, b, out, s.out_cap, st);
+  launch_write_blocks(T, k, nbo_dev, s.nbo_cap, s.ep, b, out, s.out_cap, st, s.wb_groups);
   if (s.ep.bloom_bits) launch_write_filters(T, k, nw_dev, s.nw_cap, nf_dev, s.ep, b, out, s.out_cap, st);
   launch_write_tail(T, k, nf_dev, s.nf_cap, s.ep, b, out, s.out_cap, st);
   if (A.vaddr) launch_value_addrs(T, k, nbo_dev, s.nbo_cap, s.ep, b, out, A.vaddr);
@@ -455,6 +465,7 @@ EncodeSetup plan_encode(const luda_options& o, u32 W, u64 m, u64 out_cap, u64 ex
                         u64 nw_cap_min, u64 nbo_cap_min) {
   EncodeSetup s;
   memset(&s.ep, 0, sizeof(s.ep));
+  s.wb_groups = 0;
   s.ep.block_size = o.block_size;
   s.ep.max_file_size = o.max_file_size;
   s.ep.interval = o.block_restart_interval;
@@ -757,6 +768,17 @@ bool find_resident(const luda_ctx* family, const std::vector<const luda_file*>& 
   return !files.empty();
 }
 
+// The context's tuning for a job of `bytes` input bytes and user keys of at
+// most key_bytes (a multiple of 8): the last matching rule.
+luda_tuning pick_tuning(const luda_ctx* c, u64 bytes, u32 key_bytes) {
+  luda_tuning t;
+  memset(&t, 0, sizeof(t));
+  for (const luda_tuning_rule& r : c->rules)
+    if (bytes >= r.min_bytes && bytes < r.max_bytes && key_bytes >= r.min_key_bytes && key_bytes <= r.max_key_bytes)
+      t = r.tuning;
+  return t;
+}
+
 // ---------------------------------------------------------------------------
 // luda_compact
 
@@ -962,6 +984,8 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     if (W > kMaxKeyWords) throw Fail{LUDA_NOT_SUPPORTED, "user keys longer than the device key planes"};
     plan_rounds(W);
     EncodeSetup es = plan_encode(job->options, W, n, out_cap, ngp, nf_min, nw_min, nbo_min);
+    const luda_tuning tune = pick_tuning(c, bytes_in, 8 * W);
+    es.wb_groups = tune.write_blocks_groups;
     // host-side tables (grandparents, deeper levels, children, merge pairs)
     // staged in one blob that a single copy uploads
     std::vector<u64> gw((u64)std::max<u32>(ngp, 1) * W, 0), gt(std::max<u32>(ngp, 1), 0), gpre(ngp + 1, 0);
@@ -1087,7 +1111,9 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
       // the fused last merge round looks back over its merge tiles
       HIPCHK(scan_reserve_tiles(c->scans, ((u64)n + merge_items_per_tile(W) - 1) / merge_items_per_tile(W) + 1, c->s));
       if (resident) launch_gather_decoded(T, d_dsrc, nfiles, n, K, vptr, vlen);
-      else launch_decode(T, d_files, d_blks, nb, d_ebase, K, vptr, vlen, c->st, stA.max_blk, stA.max_nr, stA.max_ent);
+      else
+        launch_decode(T, d_files, d_blks, nb, d_ebase, K, vptr, vlen, c->st, stA.max_blk, stA.max_nr, stA.max_ent,
+                      tune.decode_groups);
       if (W <= 8) {
         // Every round but the last writes a permutation (the first reads the
         // children in place and checks they are sorted); the last round
@@ -1608,6 +1634,7 @@ luda_result* build_tables(luda_ctx* c, const luda_options* opts, u64 n, const u8
   u64 nf_min = 0, nw_min = 0, nbo_min = 0;
   for (u32 attempt = 1;; attempt++) {
     EncodeSetup es = plan_encode(*opts, W, n, out_cap, 0, nf_min, nw_min, nbo_min);
+    es.wb_groups = pick_tuning(c, kbytes + vbytes, 8 * W).write_blocks_groups;
     EncodeArrays EA;
     {
       Arena dry;
@@ -1832,6 +1859,7 @@ JobPool* pool_of(luda_ctx* c) {
     char err[256];
     if (ctx_open(c->device, c->budget, &w, err, sizeof(err)) != LUDA_OK) break;
     w->family = c;
+    w->rules = c->rules;
     P->ctxs.push_back(w);
   }
   if (P->ctxs.empty()) {
@@ -1936,6 +1964,22 @@ int luda_submit_build(luda_ctx* c, const luda_options* opts, uint64_t n, const u
       P->submitted++;
     }
     P->cv_work.notify_one();
+  });
+}
+
+int luda_set_tuning(luda_ctx* c, const luda_tuning_rule* rules, uint32_t n) {
+  if (!c || (n && !rules)) return LUDA_INVALID_ARGUMENT;
+  return guarded(nullptr, nullptr, 0, [&] {
+    std::vector<luda_tuning_rule> r(rules, rules + n);
+    {
+      std::lock_guard<std::mutex> lk(c->mu);
+      c->rules = r;
+    }
+    if (c->pool)
+      for (luda_ctx* w : c->pool->ctxs) {
+        std::lock_guard<std::mutex> lk(w->mu);
+        w->rules = r;
+      }
   });
 }
 

@@ -980,7 +980,7 @@ void scan_entries(KTimer& T, ScanState& S, const BlockDesc* blks, const u32* blk
                         &st->n_entries, st, nullptr, FileEbaseEpi{blks, file_ebase, nfiles});
 }
 void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 nb, const u32* blk_ebase, Planes K,
-                   u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr, u32 max_ent) {
+                   u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr, u32 max_ent, u32 groups) {
   if (!nb) return;
   if (K.W <= 8) {
     // two stages of the largest block, its misalignment, and the 16 bytes an
@@ -993,7 +993,7 @@ void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 
     const u32 ecap = (u32)std::min<u64>((u64)max_ent, (65536 - lds) / 2);
     lds += ((2ull * ecap + 3) & ~3ull);
     // a few blocks per wavefront, so that the next descriptor's load overlaps
-    const u32 grid = std::max<u32>(1, std::min<u32>((nb + 3) / 4, 4096));
+    const u32 grid = std::max<u32>(1, std::min<u32>((nb + 3) / 4, groups ? groups : 4096));
     if (K.W <= 2)
       KLAUNCH(T, "decode", k_decode_entries<3>, grid, 64, lds, blks, nb, blk_ebase, K, vptr, vlen, st, stage, max_nr,
               ecap);

@@ -1960,9 +1960,9 @@ void scan_file_offsets(KTimer& T, ScanState& S, const u32* nf_dev, u32 nf_cap, E
                         &st->out_bytes, st);
 }
 void launch_write_blocks(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u8* out,
-                         u64 out_cap, DevStatus* st) {
+                         u64 out_cap, DevStatus* st, u32 groups) {
   // one workgroup of 16 waves per CU (its LDS holds one); each wave loops
-  const u32 g = std::min<u32>(grid_for(nbo_cap, kWbWaves), 256);
+  const u32 g = std::min<u32>(grid_for(nbo_cap, kWbWaves), groups ? groups : 256);
   if (k.K.W <= 2)
     KLAUNCH(T, "write_blocks", k_write_blocks<3>, g, kWbThreads, 0, k, nbo_dev, ep, b, out, out_cap, st);
   else if (k.K.W <= 4)

@@ -173,6 +173,15 @@ FILESET_P = C.POINTER(luda_file_set)
 LUDA_READ_VERIFY = 0x4
 
 
+class luda_tuning(C.Structure):
+    _fields_ = [("write_blocks_groups", C.c_uint32), ("decode_groups", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+
+
+class luda_tuning_rule(C.Structure):
+    _fields_ = [("min_bytes", C.c_uint64), ("max_bytes", C.c_uint64), ("min_key_bytes", C.c_uint32),
+                ("max_key_bytes", C.c_uint32), ("tuning", luda_tuning)]
+
+
 class luda_entries(C.Structure):
     _fields_ = [("n", C.c_uint64), ("num_tables", C.c_uint32), ("reserved", C.c_uint32),
                 ("keys", C.POINTER(C.c_uint8)), ("key_off", C.POINTER(C.c_uint64)),
@@ -246,6 +255,7 @@ ENGINE_SIGNATURES = {
     "luda_read_tables": (C.c_int, [C.c_void_p, C.POINTER(luda_file), C.c_uint32, C.c_uint32, C.POINTER(ENTRIES_P),
                                    C.c_char_p, C.c_size_t]),
     "luda_free_entries": (None, [ENTRIES_P]),
+    "luda_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(luda_tuning_rule), C.c_uint32]),
     "luda_poll": (C.c_int, [C.c_void_p]),
     "luda_drain": (C.c_int, [C.c_void_p]),
 }

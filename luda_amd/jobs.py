@@ -294,6 +294,22 @@ class Engine:
                                          C.byref(bad), err, 512)
         return rc, (None if bad.value == 0xffffffff else bad.value), err.value.decode()
 
+    def set_tuning(self, rules):
+        """luda_set_tuning from dicts with min_bytes, max_bytes,
+        min_key_bytes, max_key_bytes, write_blocks_groups, decode_groups
+        (missing bounds: all jobs; missing shapes: the defaults)."""
+        arr = (ffi.luda_tuning_rule * max(1, len(rules)))()
+        for i, r in enumerate(rules):
+            arr[i].min_bytes = r.get("min_bytes", 0)
+            arr[i].max_bytes = r.get("max_bytes", ffi.U64_MAX)
+            arr[i].min_key_bytes = r.get("min_key_bytes", 0)
+            arr[i].max_key_bytes = r.get("max_key_bytes", 0xFFFFFFFF)
+            arr[i].tuning.write_blocks_groups = r.get("write_blocks_groups", 0)
+            arr[i].tuning.decode_groups = r.get("decode_groups", 0)
+        rc = self.lib.luda_set_tuning(self.ctx, arr, len(rules))
+        if rc != ffi.LUDA_OK:
+            raise RuntimeError(f"luda_set_tuning failed ({rc})")
+
     # -- batched / served form (luda_submit, luda_poll, luda_drain) --------
     def set_slots(self, slots):
         rc = self.lib.luda_set_slots(self.ctx, slots)

@@ -90,3 +90,7 @@ def test_edit_layouts():
     assert C.sizeof(ffi.luda_edit) == 88
     assert C.sizeof(ffi.luda_edit_file) == 56
     assert C.sizeof(ffi.luda_file_set) == 40
+
+
+def test_tuning_layout():
+    assert C.sizeof(ffi.luda_tuning) == 16 and C.sizeof(ffi.luda_tuning_rule) == 40
