@@ -30,6 +30,14 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# The batched and served forms run one HIP stream per worker slot.  HIP maps
+# a process's streams onto GPU_MAX_HW_QUEUES hardware queues (4 unless set),
+# and streams that share a queue run their kernels in series, so these forms
+# ask for at least 16; the variable is read when the HIP runtime starts.
+if any(a.startswith(("--mix", "--serve")) for a in sys.argv[1:]):
+    if int(os.environ.get("GPU_MAX_HW_QUEUES") or 0) < 16:
+        os.environ["GPU_MAX_HW_QUEUES"] = "16"
+
 from luda_amd import ffi, jobs  # noqa: E402
 
 METRIC = "compaction input GB/s (byte-identical SSTables) vs CPU; p50/p99 job latency"

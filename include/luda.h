@@ -169,6 +169,11 @@ typedef struct luda_ctx luda_ctx;
  * device memory one call may hold (0 = 64 GiB).  Replaces the per-DB
  * gpu::HostAndDeviceMemory (cuda/decode_kv.cu:43-168, db/db_impl.h:215). */
 int luda_init(int device, uint64_t hbm_budget_bytes, luda_ctx** out, char* err, size_t errlen);
+/* Frees the context and its device memory: results' device tables live in
+ * it, so free every result first.  The context keeps device memory it has
+ * used (working buffers, freed outputs) for later calls, giving a wholly
+ * free 1 GiB-or-larger piece back only when 8 GiB more than is in use would
+ * still be held. */
 void luda_shutdown(luda_ctx* ctx);
 
 /* Runs one compaction.  Replaces db/db_impl.cc:1079-1197 (the merge, drop,
@@ -274,7 +279,11 @@ int luda_set_tuning(luda_ctx* ctx, const luda_tuning_rule* rules, uint32_t n);
  * inputs and `user` must stay valid until then.  Results are the same as
  * luda_compact's for the same job. */
 typedef void (*luda_done_fn)(void* user, int status, luda_result* result, const char* err);
-/* Worker slots (default 6); only before the first luda_submit. */
+/* Worker slots (default 6); only before the first luda_submit.  Each slot
+ * runs its jobs on a HIP stream of its own; HIP maps a process's streams onto
+ * GPU_MAX_HW_QUEUES hardware queues (4 unless the environment sets it before
+ * the process's first HIP call), and slots whose streams share a queue run
+ * their kernels in series: set it to at least slots + 2. */
 int luda_set_slots(luda_ctx* ctx, uint32_t slots);
 int luda_submit(luda_ctx* ctx, const luda_job* job, luda_done_fn done, void* user);
 /* Queues a memtable flush: the arguments of luda_build_tables, which must

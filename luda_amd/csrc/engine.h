@@ -265,7 +265,15 @@ struct StageSeg {
   u64 bytes;
 };
 void launch_stage_in(KTimer& T, const u8* src, u8* dst, u64 bytes, const DevStatus* init, DevStatus* st);
-void launch_stage_out(KTimer& T, u8* dst, StageSeg s0, StageSeg s1, StageSeg s2);
+// Completion flag of a stage_out: flag (device address of host memory) gets
+// seq once every segment is in place; ctr is a device word, zero between
+// launches.  flag null: no flag.
+struct StageDone {
+  u64* flag;
+  u32* ctr;
+  u64 seq;
+};
+void launch_stage_out(KTimer& T, u8* dst, StageSeg s0, StageSeg s1, StageSeg s2, StageDone done);
 
 void launch_files(KTimer& T, const FileDesc* files, u32 nfiles, FileInfo* info, u32* nblk, DevStatus* st);
 // k_verify.hip: every block's masked CRC (data blocks from launch_index's

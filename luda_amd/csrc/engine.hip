@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <chrono>
 #include <map>
+#include <memory>
 #include <condition_variable>
 #include <deque>
 #include <mutex>
@@ -62,35 +63,167 @@ struct Arena {
   }
 };
 
+// Device memory of a context family (a context and its worker slots):
+// chunks from hipMalloc, carved into blocks by best fit, free neighbours
+// merged again.  Jobs take their working buffers and output tables here
+// rather than from hipMalloc and hipFree: a hipFree waits for the whole
+// device, so one worker slot growing a buffer stalled every other slot, and
+// each fresh output allocation cost 0.1-0.5 ms of host time on the job's
+// path.  A chunk goes back to the runtime only when it is wholly free and
+// the family would still hold kRetain bytes beyond what is in use.
+class DevHeap {
+ public:
+  static constexpr u64 kAlign = 4096;
+  static constexpr u64 kChunkMin = 1ull << 30;
+  static constexpr u64 kRetain = 8ull << 30;
+
+  ~DevHeap() {
+    for (const Chunk& ch : chunks_)
+      if (ch.base) hipFree(ch.base);
+  }
+  // nullptr when the device has no room; the calling thread's current
+  // device is the family's
+  void* take(u64 bytes) {
+    const u64 need = align_up(std::max<u64>(bytes, 1), kAlign);
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = free_.lower_bound(need);
+    if (it == free_.end()) {
+      u64 size = std::max<u64>(align_up(need, 2ull << 20), kChunkMin);
+      u8* base = nullptr;
+      if (hipMalloc(&base, size) != hipSuccess) {
+        (void)hipGetLastError();
+        base = nullptr;
+        size = align_up(need, 2ull << 20);
+        if (hipMalloc(&base, size) != hipSuccess) {
+          (void)hipGetLastError();
+          return nullptr;
+        }
+      }
+      chunks_.push_back(Chunk{base, size});
+      blocks_[base] = Block{size, (u32)(chunks_.size() - 1), true};
+      it = free_.emplace(size, base);
+      held_ += size;
+    }
+    u8* p = it->second;
+    const u64 have = it->first;
+    free_.erase(it);
+    Block& b = blocks_.at(p);
+    b.free = false;
+    if (have - need >= kAlign) {
+      b.size = need;
+      blocks_[p + need] = Block{have - need, b.chunk, true};
+      free_.emplace(have - need, p + need);
+    }
+    in_use_ += b.size;
+    return p;
+  }
+  // false: p is not a block of this heap in use
+  bool give(void* q) {
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = blocks_.find(static_cast<u8*>(q));
+    if (it == blocks_.end() || it->second.free) return false;
+    in_use_ -= it->second.size;
+    it->second.free = true;
+    auto nx = std::next(it);
+    if (nx != blocks_.end() && nx->second.free && nx->second.chunk == it->second.chunk) {
+      erase_free(nx->first, nx->second.size);
+      it->second.size += nx->second.size;
+      blocks_.erase(nx);
+    }
+    if (it != blocks_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->second.free && pv->second.chunk == it->second.chunk) {
+        erase_free(pv->first, pv->second.size);
+        pv->second.size += it->second.size;
+        blocks_.erase(it);
+        it = pv;
+      }
+    }
+    Chunk& ch = chunks_[it->second.chunk];
+    if (it->first == ch.base && it->second.size == ch.size && held_ - ch.size >= in_use_ + kRetain) {
+      held_ -= ch.size;
+      hipFree(ch.base);
+      ch.base = nullptr;
+      blocks_.erase(it);
+      return true;
+    }
+    free_.emplace(it->second.size, it->first);
+    return true;
+  }
+  u64 held() {
+    std::lock_guard<std::mutex> lk(mu_);
+    return held_;
+  }
+
+ private:
+  struct Chunk {
+    u8* base;
+    u64 size;
+  };
+  struct Block {
+    u64 size;
+    u32 chunk;
+    bool free;
+  };
+  void erase_free(u8* p, u64 size) {
+    auto r = free_.equal_range(size);
+    for (auto i = r.first; i != r.second; ++i)
+      if (i->second == p) {
+        free_.erase(i);
+        return;
+      }
+  }
+  std::mutex mu_;
+  std::vector<Chunk> chunks_;
+  std::map<u8*, Block> blocks_;     // every block of every chunk, by address
+  std::multimap<u64, u8*> free_;    // free blocks by size
+  u64 in_use_ = 0, held_ = 0;
+};
+
+// A context's working buffer, grown (with a quarter of headroom) from the
+// family's heap.  Growing waits for the context's own stream first, so no
+// queued kernel still uses the old block when another slot takes it.
 struct DevBuf {
+  std::shared_ptr<DevHeap> heap;
+  hipStream_t s = nullptr;
   u8* p = nullptr;
   u64 cap = 0;
   hipError_t reserve(u64 need) {
     if (need <= cap) return hipSuccess;
-    if (p) hipFree(p);
+    if (p) {
+      const hipError_t e = hipStreamSynchronize(s);
+      if (e != hipSuccess) return e;
+      heap->give(p);
+    }
     p = nullptr;
     cap = 0;
-    u64 c = std::max<u64>(need, 1 << 20);
-    hipError_t e = hipMalloc(&p, c);
-    if (e == hipSuccess) cap = c;
-    return e;
+    u64 c = std::max<u64>(need + need / 4, 1 << 20);
+    void* q = heap->take(c);
+    if (!q) q = heap->take(c = need);
+    if (!q) return hipErrorOutOfMemory;
+    p = static_cast<u8*>(q);
+    cap = c;
+    return hipSuccess;
   }
   void release() {
-    if (p) hipFree(p);
+    if (p) heap->give(p);
     p = nullptr;
     cap = 0;
   }
 };
 
-// Pinned host memory; d is the address kernels use for it.
+// Pinned host memory; d is the address kernels use for it.  Grows with half
+// again as much headroom: hipHostFree waits for the whole device, which in
+// the batched form stalls every worker slot.
 struct PinnedBuf {
   u8* p = nullptr;
   u8* d = nullptr;
   u64 cap = 0;
   hipError_t reserve(u64 need) {
     if (need <= cap) return hipSuccess;
+    const bool grow = p != nullptr;
     release();
-    u64 c = std::max<u64>(need, 1 << 16);
+    u64 c = std::max<u64>(grow ? need + need / 2 : need, 1 << 16);
     hipError_t e = hipHostMalloc(&p, c, hipHostMallocDefault);
     if (e != hipSuccess) {
       p = nullptr;
@@ -203,33 +336,22 @@ struct luda_ctx {
   KTimer T;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool pin2_busy = false;  // a staging kernel that reads pin2 may not have run yet
-  std::multimap<u64, void*> out_pool;  // freed output buffers by size
+  // stage_out's completion flag: a word of host memory the kernel stores the
+  // sequence number of the staging into, and its workgroup counter
+  u64* done_h = nullptr;
+  u64* done_d = nullptr;
+  u32* done_ctr = nullptr;
+  u64 stage_seq = 0;
+  std::shared_ptr<DevHeap> heap;  // the family's device memory (working buffers, outputs)
 
-  void* take_out(u64 bytes) {
-    auto it = out_pool.lower_bound(bytes);
-    if (it != out_pool.end() && it->first <= 2 * bytes + (64 << 20)) {
-      void* p = it->second;
-      out_pool.erase(it);
-      return p;
-    }
-    void* p = nullptr;
-    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
-    pool_sizes[p] = bytes;
-    return p;
+  // output tables of a result; give_out from any thread
+  void* take_out(u64 bytes) { return heap->take(bytes); }
+  void give_out(void* p) { heap->give(p); }
+  // this context (a worker slot) draws on another family's heap
+  void use_heap(const std::shared_ptr<DevHeap>& h) {
+    heap = h;
+    for (DevBuf* b : {&in_buf, &a_buf, &b_buf}) b->heap = h;
   }
-  void give_out(void* p) {
-    auto it = pool_sizes.find(p);
-    if (it == pool_sizes.end()) return;
-    out_pool.emplace(it->second, p);
-    // keep the pool bounded
-    while (out_pool.size() > 8) {
-      auto first = out_pool.begin();
-      pool_sizes.erase(first->second);
-      hipFree(first->second);
-      out_pool.erase(first);
-    }
-  }
-  std::map<void*, u64> pool_sizes;
   IoState* io = nullptr;  // file I/O stream and staging (io.hip), on first use
   luda_ctx* family = nullptr;  // the context luda_init returned: itself, or a worker slot's owner
   std::vector<luda_tuning_rule> rules;  // luda_set_tuning
@@ -276,19 +398,30 @@ int status_code_of(const DevStatus& st) {
 void stream_sync(luda_ctx* c) {
   const double t0 = now_ms();
   hipError_t e;
-  if (c->family != c) {
-    // a worker slot: the other slots' threads are enqueueing work meanwhile,
-    // and a tight polling loop would contend with them for the runtime, so
-    // this one sleeps between polls
-    while ((e = hipStreamQuery(c->s)) == hipErrorNotReady && now_ms() - t0 < 2.0)
-      std::this_thread::sleep_for(std::chrono::microseconds(20));
-  } else {
-    while ((e = hipStreamQuery(c->s)) == hipErrorNotReady && now_ms() - t0 < 2.0) {
-    }
+  while ((e = hipStreamQuery(c->s)) == hipErrorNotReady && now_ms() - t0 < 2.0) {
   }
   if (e == hipErrorNotReady) e = hipStreamSynchronize(c->s);
   HIPCHK(e);
   c->pin2_busy = false;
+}
+
+// Waits for the stream after a stage_out, which ends it.  A worker slot
+// first watches the stage_out's completion flag in host memory (no runtime
+// call, so the slots' threads do not contend for the runtime while they
+// wait: polling the runtime from six threads at once left the device idle
+// for about 0.2 ms after each phase of a job); a busy wait for the first
+// millisecond, then 20 us naps.  Then a stream_sync, which finds the stream
+// done or nearly so.  The flag wait gives up after 200 ms and leaves the
+// rest to stream_sync.
+void stage_sync(luda_ctx* c) {
+  const double t0 = now_ms();
+  while (c->family != c && __atomic_load_n(c->done_h, __ATOMIC_ACQUIRE) < c->stage_seq) {
+    const double t = now_ms() - t0;
+    if (t > 200.0) break;
+    if (t < 1.0) __builtin_ia32_pause();
+    else std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
+  stream_sync(c);
 }
 
 // Host bytes to device memory, and with init the status word too, in one
@@ -307,12 +440,17 @@ void stage_in(luda_ctx* c, void* dst, const void* src, u64 bytes, const DevStatu
 }
 
 // Device bytes (up to three segments) into the pinned buffer pin, at the
-// segments' offsets, in one kernel; the caller synchronises the stream before
+// segments' offsets, in one kernel; the caller waits with stage_sync before
 // reading pin.  pin must already hold every segment.
 void stage_out(luda_ctx* c, StageSeg a, StageSeg b = {}, StageSeg d = {}) {
   for (const StageSeg& g : {a, b, d})
     if (g.bytes && g.off + g.bytes > c->pin.cap) throw Fail{LUDA_IO_ERROR, "read-back staging too small"};
-  launch_stage_out(c->T, c->pin.d, a, b, d);
+  // only worker slots watch the flag (stage_sync): a context alone on the
+  // device finds the stream done sooner by polling the runtime, and skips
+  // the flag's system-scope fences
+  const bool flag = c->family != c;
+  launch_stage_out(c->T, c->pin.d, a, b, d,
+                   StageDone{flag ? c->done_d : nullptr, c->done_ctr, flag ? ++c->stage_seq : 0});
 }
 
 struct EncodeArrays {
@@ -647,7 +785,7 @@ luda_result* finish_result(luda_ctx* c, const EncodeSetup& s, const EncodeArrays
 
 struct DecodedSet {
   const luda_ctx* family;
-  int device;
+  std::shared_ptr<DevHeap> heap;  // buf's
   void* buf;
   u32 W;
   u64 m;  // entries (the stride of every array)
@@ -687,12 +825,13 @@ void free_result_now(luda_ctx* c, luda_result* r) {
 void keep_decoded(luda_ctx* c, const EncodeArrays& A, u32 W, u64 m, luda_result* r) {
   DecodedSet* d = new DecodedSet;
   d->family = c->family;
-  d->device = c->device;
+  d->heap = c->heap;
   d->W = W;
   d->m = m;
   const u64 mm = std::max<u64>(m, 1);
   const u64 bytes = mm * (8ull * W + 4 + 8 + 8 + 4) + 5 * kAlign;
-  if (hipMalloc(&d->buf, bytes) != hipSuccess) {
+  d->buf = d->heap->take(bytes);
+  if (!d->buf) {
     delete d;
     throw Fail{LUDA_IO_ERROR, "device memory for the resident decoded entries"};
   }
@@ -714,7 +853,8 @@ void keep_decoded(luda_ctx* c, const EncodeArrays& A, u32 W, u64 m, luda_result*
   if (m && e == hipSuccess) e = hipMemcpyAsync(d->vlen, k.vlen, m * 4, hipMemcpyDeviceToDevice, c->s);
   if (e == hipSuccess) e = hipStreamSynchronize(c->s);
   if (e != hipSuccess) {
-    hipFree(d->buf);
+    (void)hipStreamSynchronize(c->s);  // no copy still writes the block
+    d->heap->give(d->buf);
     delete d;
     throw Fail{LUDA_IO_ERROR, std::string("resident decoded entries: ") + hipGetErrorString(e)};
   }
@@ -738,8 +878,7 @@ void forget_decoded(const luda_result* r) {
     g_decoded_of.erase(it);
     for (u32 i = 0; i < r->num_outputs; i++) g_resident.erase(ResidentKey{d->family, (u64)r->device_data + r->outputs[i].offset});
   }
-  hipSetDevice(d->device);
-  hipFree(d->buf);
+  d->heap->give(d->buf);
   delete d;
 }
 
@@ -927,7 +1066,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     scan_entries(T, c->scans, d_blks, d_nent, d_ebase, (u32)nb_cap, nfiles, d_febase, c->st);
     HIPCHK(hipGetLastError());
     stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)}, StageSeg{d_febase, 1024, (nfiles + 1) * 4ull});
-    stream_sync(c);
+    stage_sync(c);
     memcpy(&stA, c->pin.p, sizeof(DevStatus));
     memcpy(febase.data(), c->pin.p + 1024, (nfiles + 1) * 4);
     if (status_code_of(stA) != LUDA_OK)
@@ -1165,7 +1304,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     HIPCHK(c->pin.reserve(o_keys + key_bytes + 64));
     stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)}, StageSeg{EA.eb.meta, o_meta, meta_bytes},
               StageSeg{EA.eb.keys, o_keys, key_bytes});
-    stream_sync(c);
+    stage_sync(c);
     DevStatus stB;
     memcpy(&stB, c->pin.p, sizeof(DevStatus));
     float dms = 0;
@@ -1289,7 +1428,7 @@ void verify_tables(luda_ctx* c, const luda_file* tables, u32 n, u32 flags, u32* 
   launch_verify(T, d_files, n, d_blks, (u32)nb_cap, c->st);
   HIPCHK(hipGetLastError());
   stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)});
-  stream_sync(c);
+  stage_sync(c);
   DevStatus st;
   memcpy(&st, c->pin.p, sizeof(DevStatus));
   *first_bad = st.bad_file;
@@ -1377,7 +1516,7 @@ luda_entries* read_tables(luda_ctx* c, const luda_file* tables, u32 nfiles, u32 
   if (flags & LUDA_READ_VERIFY) launch_verify(T, d_files, nfiles, d_blks, (u32)nb_cap, c->st);
   scan_entries(T, c->scans, d_blks, d_nent, d_ebase, (u32)nb_cap, nfiles, d_febase, c->st);
   stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)}, StageSeg{d_febase, 1024, (nfiles + 1) * 4ull});
-  stream_sync(c);
+  stage_sync(c);
   DevStatus stA;
   memcpy(&stA, c->pin.p, sizeof(DevStatus));
   std::vector<u32> febase(nfiles + 1);
@@ -1435,7 +1574,7 @@ luda_entries* read_tables(luda_ctx* c, const luda_file* tables, u32 nfiles, u32 
     HIPCHK(hipGetLastError());
     constexpr u64 o_kb = align_up_c(sizeof(DevStatus), 64), o_vb = o_kb + 64;
     stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)}, StageSeg{key_off + n, o_kb, 8}, StageSeg{val_off + n, o_vb, 8});
-    stream_sync(c);
+    stage_sync(c);
     DevStatus stB;
     memcpy(&stB, c->pin.p, sizeof(DevStatus));
     if (stB.err & ERR_OVERFLOW) {
@@ -1563,8 +1702,8 @@ void verify_tables_noalloc(luda_ctx* c, const luda_file* tables, u32 n, u32 flag
   launch_index(T, d_files, d_info, d_bbase, n, (u32)nb_cap, d_blks, d_nent, c->st);
   launch_verify(T, d_files, n, d_blks, (u32)nb_cap, c->st);
   HIPCHK(hipGetLastError());
-  launch_stage_out(T, c->pin.d, StageSeg{c->st, 0, sizeof(DevStatus)}, StageSeg{}, StageSeg{});
-  stream_sync(c);
+  stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)});
+  stage_sync(c);
   DevStatus st;
   memcpy(&st, c->pin.p, sizeof(DevStatus));
   *first_bad = st.bad_file;
@@ -1666,7 +1805,7 @@ luda_result* build_tables(luda_ctx* c, const luda_options* opts, u64 n, const u8
     HIPCHK(c->pin.reserve(o_keys + key_bytes + 64));
     stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)}, StageSeg{EA.eb.meta, o_meta, meta_bytes},
               StageSeg{EA.eb.keys, o_keys, key_bytes});
-    stream_sync(c);
+    stage_sync(c);
     DevStatus stB;
     memcpy(&stB, c->pin.p, sizeof(DevStatus));
     float dms = 0;
@@ -1807,7 +1946,6 @@ struct JobPool {
   std::condition_variable cv_work, cv_done;
   std::deque<QueuedJob> queue;
   std::deque<DoneJob> done;
-  std::map<void*, luda_ctx*> owner;  // device output buffer -> the slot that owns it
   u64 budget = 0, inflight = 0;
   u32 running = 0, submitted = 0, delivered = 0;
   bool stop = false;
@@ -1843,7 +1981,6 @@ void pool_worker(JobPool* P, luda_ctx* w) {
     lk.lock();
     P->running--;
     P->inflight -= q.est;
-    if (r && r->device_data) P->owner[r->device_data] = w;
     P->done.push_back(DoneJob{q.done, q.user, rc, r, err, w});
     P->cv_done.notify_all();
     P->cv_work.notify_all();
@@ -1860,6 +1997,7 @@ JobPool* pool_of(luda_ctx* c) {
     if (ctx_open(c->device, c->budget, &w, err, sizeof(err)) != LUDA_OK) break;
     w->family = c;
     w->rules = c->rules;
+    w->use_heap(c->heap);
     P->ctxs.push_back(w);
   }
   if (P->ctxs.empty()) {
@@ -1882,10 +2020,7 @@ void pool_close(JobPool* P) {
     if (d.result) {
       // undelivered results: return their buffers before the slots go
       forget_decoded(d.result);
-      if (d.result->device_data) {
-        std::lock_guard<std::mutex> lk(d.owner->mu);
-        d.owner->give_out(d.result->device_data);
-      }
+      if (d.result->device_data) d.owner->give_out(d.result->device_data);
       if (d.result->data) hipHostFree(d.result->data);
       free(d.result->outputs);
       free(d.result->keys);
@@ -2015,11 +2150,18 @@ int ctx_open(int device, uint64_t hbm_budget_bytes, luda_ctx** out, char* err, s
       throw Fail{LUDA_NOT_SUPPORTED, "no HIP device " + std::to_string(device)};
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+    c->use_heap(std::make_shared<DevHeap>());
+    for (DevBuf* b : {&c->in_buf, &c->a_buf, &c->b_buf}) b->s = c->s;
     HIPCHK(hipMalloc(&c->st, sizeof(DevStatus)));
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));
-    HIPCHK(c->pin.reserve(1 << 16));
-    HIPCHK(c->pin2.reserve(128 << 10));
+    HIPCHK(c->pin.reserve(1 << 20));
+    HIPCHK(c->pin2.reserve(1 << 20));
+    HIPCHK(hipHostMalloc(&c->done_h, 64, hipHostMallocCoherent));
+    *c->done_h = 0;
+    HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&c->done_d), c->done_h, 0));
+    HIPCHK(hipMalloc(&c->done_ctr, 64));
+    HIPCHK(hipMemset(c->done_ctr, 0, 64));
     c->T.s = c->s;
     noalloc_setup(device, c->s);
   });
@@ -2042,9 +2184,9 @@ void ctx_close(luda_ctx* c) {
   scan_release(c->scans);
   c->pin.release();
   c->pin2.release();
-  for (auto& kv : c->out_pool) hipFree(kv.second);
-  c->out_pool.clear();
   if (c->st) hipFree(c->st);
+  if (c->done_ctr) hipFree(c->done_ctr);
+  if (c->done_h) hipHostFree(c->done_h);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->s) hipStreamDestroy(c->s);
@@ -2137,20 +2279,9 @@ int luda_build_tables(luda_ctx* c, const luda_options* opts, uint64_t n, const u
 void luda_free_result(luda_ctx* c, luda_result* r) {
   if (!r) return;
   forget_decoded(r);
-  if (c && r->device_data) {
-    // a result of luda_submit returns its buffer to the slot that made it
-    luda_ctx* owner = c;
-    if (c->pool) {
-      std::lock_guard<std::mutex> lk(c->pool->mu);
-      auto it = c->pool->owner.find(r->device_data);
-      if (it != c->pool->owner.end()) {
-        owner = it->second;
-        c->pool->owner.erase(it);
-      }
-    }
-    std::lock_guard<std::mutex> lk(owner->mu);
-    owner->give_out(r->device_data);
-  }
+  // the family's heap (the context and its worker slots share it) takes
+  // the tables back; its own lock, so this does not wait for running jobs
+  if (c && r->device_data) c->give_out(r->device_data);
   if (r->data) hipHostFree(r->data);
   free(r->outputs);
   free(r->keys);

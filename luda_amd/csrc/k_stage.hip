@@ -35,12 +35,30 @@ __global__ __launch_bounds__(kStageThreads) void k_stage_in(const u8* __restrict
   }
 }
 
+// With done.flag set, the last workgroup to finish its copies stores
+// done.seq into the flag (host memory) after every staged byte is visible
+// at system scope, so the host can see the copies complete by reading one
+// word, without asking the runtime.  done.ctr counts finished workgroups and
+// is reset by the last one (the next staging kernel of the stream starts
+// after this one ends).
 __global__ __launch_bounds__(kStageThreads) void k_stage_out(u8* __restrict__ dst, StageSeg s0, StageSeg s1,
-                                                             StageSeg s2) {
+                                                             StageSeg s2, StageDone done) {
   const u64 tid = (u64)blockIdx.x * blockDim.x + threadIdx.x, nth = (u64)gridDim.x * blockDim.x;
   if (s0.bytes) copy_bytes(reinterpret_cast<const u8*>(s0.src), dst + s0.off, s0.bytes, tid, nth);
   if (s1.bytes) copy_bytes(reinterpret_cast<const u8*>(s1.src), dst + s1.off, s1.bytes, tid, nth);
   if (s2.bytes) copy_bytes(reinterpret_cast<const u8*>(s2.src), dst + s2.off, s2.bytes, tid, nth);
+  if (done.flag) {
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const u32 prev = __hip_atomic_fetch_add(done.ctr, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      if (prev == gridDim.x - 1) {
+        __hip_atomic_store(done.ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence_system();
+        __hip_atomic_store(done.flag, done.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
+    }
+  }
 }
 
 u32 stage_grid(u64 bytes) {
@@ -56,7 +74,7 @@ void launch_stage_in(KTimer& T, const u8* src, u8* dst, u64 bytes, const DevStat
   KLAUNCH(T, "stage_in", k_stage_in, stage_grid(bytes), kStageThreads, 0, src, dst, bytes, init, st);
 }
 
-void launch_stage_out(KTimer& T, u8* dst, StageSeg s0, StageSeg s1, StageSeg s2) {
+void launch_stage_out(KTimer& T, u8* dst, StageSeg s0, StageSeg s1, StageSeg s2, StageDone done) {
   KLAUNCH(T, "stage_out", k_stage_out, stage_grid(std::max(s0.bytes, std::max(s1.bytes, s2.bytes))),
-          kStageThreads, 0, dst, s0, s1, s2);
+          kStageThreads, 0, dst, s0, s1, s2, done);
 }
