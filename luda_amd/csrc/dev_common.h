@@ -71,7 +71,7 @@ struct DevStatus {
   u64 data_bytes;     // output data blocks with trailers
   u64 keys_needed;    // bytes of smallest/largest keys
   u32 bad_file;       // luda_verify_tables: lowest table with a failed check
-  u32 pad;
+  u32 cut_fallback;   // the tiled block cuts cannot take this stream: run the orbit kernels
 };
 
 // ---------------------------------------------------------------------------

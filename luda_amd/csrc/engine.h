@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "cuts.h"
 #include "dev_common.h"
 
 // ---------------------------------------------------------------------------
@@ -265,6 +266,23 @@ struct StageSeg {
   u64 bytes;
 };
 void launch_stage_in(KTimer& T, const u8* src, u8* dst, u64 bytes, const DevStatus* init, DevStatus* st);
+// k_cuts.hip: the output block cuts by tiles of kCutTile entries (staged with
+// kCutSpan more), for chains with at most kCutCand entry points per tile.
+// ftab: kCutCand words per tile (exit | blocks << 12); cand, entry, base:
+// one word per tile.
+constexpr u32 kCutTile = cuts::kTile;
+constexpr u32 kCutSpan = cuts::kSpan;
+constexpr u32 kCutCand = cuts::kCand;
+struct CutBufs {
+  u32* ftab;
+  u32* cand;
+  u32* entry;
+  u32* base;
+};
+inline u32 cut_tiles_for(u64 m_cap) { return (u32)(m_cap / kCutTile + 1); }
+void run_block_cuts(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncParams ep, EncBufs b, CutBufs cb, u32 nbo_cap,
+                    DevStatus* st);
+
 // Completion flag of a stage_out: flag (device address of host memory) gets
 // seq once every segment is in place; ctr is a device word, zero between
 // launches.  flag null: no flag.

@@ -37,7 +37,7 @@ constexpr u64 kAlign = 256;
 constexpr u32 kMaxKeyWords = 256;
 constexpr u64 kMaxBlockSize = 8192;   // block_size handled on the device
 constexpr u32 kMaxChunk = 8192;       // orbit chunk: its LDS footprint is 5 bytes per node
-constexpr u32 kMaxAttempts = 4;
+constexpr u32 kMaxAttempts = 5;
 
 u64 align_up(u64 x, u64 a) { return (x + a - 1) / a * a; }
 constexpr u64 align_up_c(u64 x, u64 a) { return (x + a - 1) / a * a; }
@@ -312,6 +312,7 @@ struct EncodeSetup {
   bool size_cuts; // max_file_size can cut a table before the stream ends
   OrbitPlan ob, of;
   u32 wb_groups;   // block writer workgroups (0 = default)
+  bool generic_cuts;  // block cuts by the orbit kernels instead of the tiled ones
 };
 
 }  // namespace
@@ -461,6 +462,7 @@ struct EncodeArrays {
   u16* ftab_f;
   u32* ent_f;
   u64* vaddr;  // with LUDA_JOB_KEEP_DECODED: each kept value's address in the output
+  CutBufs cut;  // the tiled block cuts' functions and tile entry points
 };
 
 // Carves the encoder's arrays for a setup out of an arena (dry or real).
@@ -517,7 +519,22 @@ EncodeArrays carve_encode(Arena& a, const EncodeSetup& s, u32 W, bool with_kept,
     r.ent_f = a.take<u32>(s.of.entry_entries);
   }
   if (with_vaddr) r.vaddr = a.take<u64>(m);
+  const u64 tiles = cut_tiles_for(m);
+  r.cut.ftab = a.take<u32>(tiles * kCutCand);
+  r.cut.cand = a.take<u32>(tiles);
+  r.cut.entry = a.take<u32>(tiles);
+  r.cut.base = a.take<u32>(tiles);
   return r;
+}
+
+// Whether a job's block cuts start on the orbit kernels: the tiled cuts walk
+// each tile's blocks one after another, which for blocks of a few entries
+// (large values) is slower than the orbit's pointer jumping.
+// LUDA_BLOCK_CUTS=orbit forces the orbit kernels (both give the same cuts).
+bool start_generic_cuts(const luda_options& o, u64 entries, u64 bytes) {
+  const char* e = getenv("LUDA_BLOCK_CUTS");
+  if (e && strcmp(e, "orbit") == 0) return true;
+  return entries && (double)o.block_size * (double)entries < 16.0 * (double)std::max<u64>(bytes, 1);
 }
 
 __global__ void k_table_starts_forced(const u32* nbo_dev, EncBufs b) {
@@ -546,9 +563,13 @@ void run_encode(luda_ctx* c, const EncodeSetup& s, const EncodeArrays& A, u8* ou
   HIPCHK(scan_reserve(SC, std::max<u64>(std::max<u64>(s.m_cap, s.nbo_cap), std::max<u64>(s.nf_cap, s.nw_cap)) + 1,
                       c->s));
   launch_entry_sizes(T, k, m_dev, s.m_cap, b, st);
-  launch_bnext(T, k, m_dev, s.m_cap, s.ep, b, st);
-  run_orbit(T, s.ob, b.bnext, m_dev, &st->blk_step_max, b.bs, A.ftab_b, A.ent_b, st);
-  scan_blocks(T, SC, k, m_dev, s.m_cap, b, s.nbo_cap, st);
+  if (s.generic_cuts) {
+    launch_bnext(T, k, m_dev, s.m_cap, s.ep, b, st);
+    run_orbit(T, s.ob, b.bnext, m_dev, &st->blk_step_max, b.bs, A.ftab_b, A.ent_b, st);
+    scan_blocks(T, SC, k, m_dev, s.m_cap, b, s.nbo_cap, st);
+  } else {
+    run_block_cuts(T, k, m_dev, s.m_cap, s.ep, b, A.cut, s.nbo_cap, st);
+  }
   if (s.nbo_cap <= cut_small_max_blocks()) {
     // a small job: block offsets and table cuts in one launch
     launch_cut_small(T, nbo_dev, s.ep, b, s.nf_cap, s.size_cuts, st);
@@ -1118,6 +1139,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
   const u32 ngp = job->num_grandparents;
   u32 ndeep = 0;
   for (int L = 0; L < 7; L++) ndeep += (u32)lvl_files[L].size();
+  bool generic_cuts = start_generic_cuts(job->options, n, bytes_in);
 
   for (u32 attempt = 1;; attempt++) {
     if (W > kMaxKeyWords) throw Fail{LUDA_NOT_SUPPORTED, "user keys longer than the device key planes"};
@@ -1125,6 +1147,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     EncodeSetup es = plan_encode(job->options, W, n, out_cap, ngp, nf_min, nw_min, nbo_min);
     const luda_tuning tune = pick_tuning(c, bytes_in, 8 * W);
     es.wb_groups = tune.write_blocks_groups;
+    es.generic_cuts = generic_cuts;
     // host-side tables (grandparents, deeper levels, children, merge pairs)
     // staged in one blob that a single copy uploads
     std::vector<u64> gw((u64)std::max<u32>(ngp, 1) * W, 0), gt(std::max<u32>(ngp, 1), 0), gpre(ngp + 1, 0);
@@ -1340,6 +1363,10 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
       }
       if (stB.need_blocks) {
         nbo_min = (u64)stB.need_blocks + 16;
+        grew = true;
+      }
+      if (stB.cut_fallback && !generic_cuts) {
+        generic_cuts = true;
         grew = true;
       }
       if (!grew) {
@@ -1771,9 +1798,11 @@ luda_result* build_tables(luda_ctx* c, const luda_options* opts, u64 n, const u8
   HIPCHK(hipEventRecord(c->ev0, c->s));
   u64 out_cap = kbytes + vbytes + 16 * n + (1 << 20);
   u64 nf_min = 0, nw_min = 0, nbo_min = 0;
+  bool generic_cuts = start_generic_cuts(*opts, n, kbytes + vbytes);
   for (u32 attempt = 1;; attempt++) {
     EncodeSetup es = plan_encode(*opts, W, n, out_cap, 0, nf_min, nw_min, nbo_min);
     es.wb_groups = pick_tuning(c, kbytes + vbytes, 8 * W).write_blocks_groups;
+    es.generic_cuts = generic_cuts;
     EncodeArrays EA;
     {
       Arena dry;
@@ -1824,6 +1853,7 @@ luda_result* build_tables(luda_ctx* c, const luda_options* opts, u64 n, const u8
       if (stB.need_files) nf_min = (u64)stB.need_files + 16;
       if (stB.need_windows) nw_min = (u64)stB.need_windows + 16;
       if (stB.need_blocks) nbo_min = (u64)stB.need_blocks + 16;
+      if (stB.cut_fallback) generic_cuts = true;
       continue;
     }
     stats.attempts = attempt;

@@ -1,0 +1,109 @@
+"""The tiled output block cuts (k_cuts.hip) on the GPU: the same jobs with the
+tiled cuts and with the orbit kernels they replace (LUDA_BLOCK_CUTS=orbit)
+give identical tables, equal to the oracle's; kernel timing
+(LUDA_KERNEL_TIMING=cut_tiles) shows which path a job took, including the
+fallback of a job whose blocks hold more entries than a tile takes."""
+import contextlib
+import os
+import random
+
+import pytest
+
+import oracle
+from luda_amd import ffi, jobs
+from tests.tiny_jobs import TinyJob, ikey, many_grandparents_job
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = jobs.Engine(0)
+    yield e
+    e.close()
+
+
+@contextlib.contextmanager
+def env(**kv):
+    old = {k: os.environ.get(k) for k in kv}
+    os.environ.update({k: v for k, v in kv.items()})
+    try:
+        yield
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def tables_of(out):
+    return [(t.data, t.smallest, t.largest, t.num_entries) for t in out[0]]
+
+
+def build_both(eng, ents, opts):
+    """(tiled tables, orbit tables, tiled launches of cut_tiles)."""
+    with env(LUDA_KERNEL_TIMING="cut_tiles"):
+        rc, out, msg = eng.build_entries(ents, opts)
+    assert rc == 0, msg
+    with env(LUDA_BLOCK_CUTS="orbit"):
+        rc2, out2, msg2 = eng.build_entries(ents, opts)
+    assert rc2 == 0, msg2
+    st = out[1]
+    launches = st["top_kernel_launches"] if st["top_kernel"] == "cut_tiles" else 0
+    return tables_of(out), tables_of(out2), launches
+
+
+def oracle_tables(ents, opts):
+    return [(t.data, t.smallest, t.largest, t.num_entries) for t in oracle.build_tables(ents, opts, "oracle")]
+
+
+@pytest.mark.parametrize("n,vmin,vmax,interval,block_size,max_file",
+                         [(30000, 100, 100, 16, 4096, 2 << 20),   # fixed-size entries, many tiles
+                          (20000, 0, 400, 16, 4096, 1 << 20),     # varied, several tables
+                          (12000, 50, 150, 1, 4096, 300000),      # every entry a restart point
+                          (9000, 10, 40, 64, 1024, 100000),       # long restart runs, small blocks
+                          (4097, 100, 100, 16, 4096, 2 << 20),    # one entry past a tile
+                          (8192, 5, 20, 3, 1024, 50000)])         # ~20 entries per block
+def test_tiled_cuts_equal_orbit_and_oracle(eng, n, vmin, vmax, interval, block_size, max_file):
+    rng = random.Random(n * 31 + interval)
+    keys = sorted({b"%016d" % rng.randrange(10 ** 12) for _ in range(n)})
+    ents = [(ikey(k, 1000 + i, 1), bytes(rng.randrange(256) for _ in range(rng.randint(vmin, vmax))))
+            for i, k in enumerate(keys)]
+    opts = ffi.default_options(block_size=block_size, block_restart_interval=interval, max_file_size=max_file)
+    print("cuts", n, vmin, vmax, interval, block_size, max_file, flush=True)
+    tiled, orbit, launches = build_both(eng, ents, opts)
+    assert launches == 1  # the tiled cuts ran, once
+    assert tiled == orbit
+    assert tiled == oracle_tables(ents, opts)
+
+
+def test_blocks_of_many_entries_fall_back_to_the_orbit(eng):
+    """Entries of ~12 bytes in 8 KiB blocks: more entry points per tile than
+    the tiles take, so the job runs again on the orbit kernels."""
+    rng = random.Random(5)
+    ents = [(ikey(b"%06d" % i, 7, 1), bytes(rng.randrange(256) for _ in range(rng.randint(0, 3))))
+            for i in range(12000)]
+    opts = ffi.default_options(block_size=8192, max_file_size=1 << 20)
+    with env(LUDA_KERNEL_TIMING="cut_tiles"):
+        rc, out, msg = eng.build_entries(ents, opts)
+    assert rc == 0, msg
+    assert out[1]["attempts"] >= 2
+    assert tables_of(out) == oracle_tables(ents, opts)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_compactions_with_table_cuts_equal_orbit(eng, seed):
+    """Compactions whose ShouldStopBefore cuts start tables (forced block
+    starts in the kept stream), tiled against orbit and against the oracle."""
+    tj = many_grandparents_job(5 + 700 * seed) if seed % 2 else TinyJob(900 + seed)
+    job, keep = tj.make_job(lambda e, o: [t.data for t in oracle.build_tables(e, o, "oracle")])
+    rc1, want = oracle.compact(job, "oracle")
+    rc2, out, msg = eng.compact(job)
+    assert rc1 == 0 and rc2 == 0, msg
+    with env(LUDA_BLOCK_CUTS="orbit"):
+        rc3, out3, msg3 = eng.compact(job)
+    assert rc3 == 0, msg3
+    got = [(t.data, t.smallest, t.largest, t.num_entries) for t in out[0]]
+    assert got == [(t.data, t.smallest, t.largest, t.num_entries) for t in out3[0]]
+    assert got == [(t.data, t.smallest, t.largest, t.num_entries) for t in want]
