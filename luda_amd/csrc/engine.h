@@ -184,21 +184,23 @@ struct EncBufs {
   u32* ob_n;
   u32* ob_bytes;
   u32* ob_forced;  // 1 where ShouldStopBefore starts a table at this block
-  u64* O;          // exclusive scan of ob_bytes + 5: block offset in the output's data
-  u32* ff;         // exclusive scan of ob_forced
-  u32* flist;      // forced blocks in order
-  u32* fnext;      // first block after the table that would start here
-  u32* fs;         // 1 where a table starts
+  // the table that would start at each block (k_table_next)
+  u16* tnext;      // its end, as a distance (0: past the staged span)
+  u64* torel;      // the block's offset in its tile of kTableTile blocks
+  u64* tdend;      // its data bytes
+  u32* tnw;        // its filter windows
+  u64* tbytes;     // per tile: bytes of its blocks
+  u32* tforced;    // per tile: its first forced block, or ~0
   u32* ob_file;
   u64* ob_off;     // offset inside its table
-  u64* ob_isz;     // index entry size, then (scanned) its offset in the index block
+  u64* ob_isz;     // offset of its index entry in the index block
   // per output table (nf + 1 entries)
   u32* f_first;    // first block; f_first[nf] = nbo
-  u32* f_win;      // first filter window (exclusive scan of window counts)
+  u32* f_win;      // first filter window (windows are numbered across tables)
   u64* f_dend;     // data end offset
   u64* f_fbytes;   // filter data bytes
   u64* f_ibytes;   // index block size
-  u64* f_fbase;    // f_dend - (filter data offset of the table's first window)
+  u64* f_fbase;    // where the filter data starts in the table (its data end)
   u64* f_off;      // table size, then (scanned in place) table offset in the output
   FileMeta* meta;
   u8* keys;        // 2 x kw bytes per table: smallest, largest
@@ -206,7 +208,7 @@ struct EncBufs {
   u32* w_keys;
   u32* w_first;
   u32* w_file;     // its table (windows with keys only)
-  u64* w_fsz;      // filter bytes, then (scanned in place) offset in the filter data
+  u64* w_fsz;      // filter bytes, then (scanned in place) offset in its table's filter data
 };
 
 // ---------------------------------------------------------------------------
@@ -368,29 +370,19 @@ void launch_bnext(KTimer& T, Kept k, const u32* m_dev, u32 m_cap, EncParams ep, 
 // f_first, the window offsets f_win, the table offsets f_off)
 void scan_blocks(KTimer& T, ScanState& S, Kept k, const u32* m_dev, u32 m_cap, EncBufs b, u32 nbo_cap,
                  DevStatus* st);
-void scan_block_offsets(KTimer& T, ScanState& S, const u32* nbo_dev, u32 nbo_cap, EncBufs b, DevStatus* st);
-void scan_forced(KTimer& T, ScanState& S, const u32* nbo_dev, u32 nbo_cap, EncBufs b, DevStatus* st);
-void scan_files(KTimer& T, ScanState& S, const u32* nbo_dev, u32 nbo_cap, EncBufs b, u32 nf_cap, DevStatus* st);
-void scan_windows(KTimer& T, ScanState& S, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b, u32 nw_cap,
-                  DevStatus* st);
-void scan_file_offsets(KTimer& T, ScanState& S, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b,
-                       DevStatus* st);
-void launch_fnext(KTimer& T, const u32* nbo_dev, u32 nbo_cap, const u32* nforced_dev, EncParams ep, EncBufs b,
-                  DevStatus* st);
-void launch_block_layout(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, const u32* nf_dev, EncParams ep,
-                         EncBufs b);
-void launch_windows(KTimer& T, const u32* nbo_dev, u32 nbo_cap, const u32* nf_dev, const u32* nw_dev, EncParams ep,
-                    EncBufs b);
+// Table cuts (f_first, n_files_out), data_bytes, each table's data end and
+// first filter window (f_dend, f_win, n_windows); f_off[n_files_out] = 0 for
+// the table offset scan.  Two launches: the table that would start at each
+// block, then the chain of those from block 0 in one workgroup.
+constexpr u32 kTableTile = 4096;
+void launch_cut_tables(KTimer& T, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u32 nf_cap, u32 nw_cap,
+                       bool size_cuts, DevStatus* st);
+// Per table: ob_file, ob_off, ob_isz, the windows (w_*), f_fbytes, f_fbase,
+// f_ibytes and the table's size in f_off.
+void launch_table_layout(KTimer& T, Kept k, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b, DevStatus* st);
 // groups: workgroups of the block writer, 0 = one per CU
 void launch_write_blocks(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u8* out,
                          u64 out_cap, DevStatus* st, u32 groups = 0);
-// Jobs of at most cut_small_max_blocks() output blocks: block offsets and
-// table cuts (O, f_first, n_files_out, data_bytes) in one workgroup, the
-// work of scan_block_offsets, scan_forced, launch_fnext, the table orbit and
-// scan_files.
-u32 cut_small_max_blocks();
-void launch_cut_small(KTimer& T, const u32* nbo_dev, EncParams ep, EncBufs b, u32 nf_cap, bool size_cuts,
-                      DevStatus* st);
 // vaddr[q] = device address of kept entry q's value in the written tables
 void launch_value_addrs(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u8* out,
                         u64* vaddr);

@@ -310,7 +310,7 @@ struct EncodeSetup {
   u32 nw_cap;     // filter windows
   u64 out_cap;    // output bytes
   bool size_cuts; // max_file_size can cut a table before the stream ends
-  OrbitPlan ob, of;
+  OrbitPlan ob;
   u32 wb_groups;   // block writer workgroups (0 = default)
   bool generic_cuts;  // block cuts by the orbit kernels instead of the tiled ones
 };
@@ -459,8 +459,6 @@ struct EncodeArrays {
   EncBufs eb;
   u16* ftab_b;
   u32* ent_b;
-  u16* ftab_f;
-  u32* ent_f;
   u64* vaddr;  // with LUDA_JOB_KEEP_DECODED: each kept value's address in the output
   CutBufs cut;  // the tiled block cuts' functions and tile entry points
 };
@@ -491,11 +489,12 @@ EncodeArrays carve_encode(Arena& a, const EncodeSetup& s, u32 W, bool with_kept,
   b.ob_n = a.take<u32>(nb + 1);
   b.ob_bytes = a.take<u32>(nb + 1);
   b.ob_forced = a.take<u32>(nb + 1);
-  b.O = a.take<u64>(nb + 1);
-  b.ff = a.take<u32>(nb + 1);
-  b.flist = a.take<u32>(nb + 1);
-  b.fnext = a.take<u32>(nb + 1);
-  b.fs = a.take<u32>(nb + 1);
+  b.tnext = a.take<u16>(nb + 1);
+  b.torel = a.take<u64>(nb + 1);
+  b.tdend = a.take<u64>(nb + 1);
+  b.tnw = a.take<u32>(nb + 1);
+  b.tbytes = a.take<u64>(nb / kTableTile + 2);
+  b.tforced = a.take<u32>(nb / kTableTile + 2);
   b.ob_file = a.take<u32>(nb + 1);
   b.ob_off = a.take<u64>(nb + 1);
   b.ob_isz = a.take<u64>(nb + 1);
@@ -514,10 +513,6 @@ EncodeArrays carve_encode(Arena& a, const EncodeSetup& s, u32 W, bool with_kept,
   b.w_fsz = a.take<u64>(nw + 1);
   r.ftab_b = a.take<u16>(s.ob.table_entries);
   r.ent_b = a.take<u32>(s.ob.entry_entries);
-  if (s.size_cuts) {
-    r.ftab_f = a.take<u16>(s.of.table_entries);
-    r.ent_f = a.take<u32>(s.of.entry_entries);
-  }
   if (with_vaddr) r.vaddr = a.take<u64>(m);
   const u64 tiles = cut_tiles_for(m);
   r.cut.ftab = a.take<u32>(tiles * kCutCand);
@@ -537,17 +532,10 @@ bool start_generic_cuts(const luda_options& o, u64 entries, u64 bytes) {
   return entries && (double)o.block_size * (double)entries < 16.0 * (double)std::max<u64>(bytes, 1);
 }
 
-__global__ void k_table_starts_forced(const u32* nbo_dev, EncBufs b) {
-  const u64 nbo = *nbo_dev;
-  GRID_STRIDE(i, nbo) b.fs[i] = (i == 0 || b.ob_forced[i]) ? 1u : 0u;
-}
-
 u32 grid_for(u64 n, u32 per) {
   u64 g = (n + per - 1) / per;
   return (u32)std::min<u64>(std::max<u64>(g, 1), 16384);
 }
-
-void run_cuts_large(luda_ctx* c, const EncodeSetup& s, const EncodeArrays& A);
 
 // The encoder over the kept stream, counts in st.
 void run_encode(luda_ctx* c, const EncodeSetup& s, const EncodeArrays& A, u8* out) {
@@ -570,43 +558,13 @@ void run_encode(luda_ctx* c, const EncodeSetup& s, const EncodeArrays& A, u8* ou
   } else {
     run_block_cuts(T, k, m_dev, s.m_cap, s.ep, b, A.cut, s.nbo_cap, st);
   }
-  if (s.nbo_cap <= cut_small_max_blocks()) {
-    // a small job: block offsets and table cuts in one launch
-    launch_cut_small(T, nbo_dev, s.ep, b, s.nf_cap, s.size_cuts, st);
-  } else {
-    run_cuts_large(c, s, A);
-  }
-  launch_block_layout(T, k, nbo_dev, s.nbo_cap, nf_dev, s.ep, b);
-  scan_u64(T, SC, b.ob_isz, b.ob_isz, nbo_dev, 1, s.nbo_cap + 1, nullptr, st);
-  scan_windows(T, SC, nf_dev, s.nf_cap, s.ep, b, s.nw_cap, st);
-  if (s.ep.bloom_bits) {
-    launch_windows(T, nbo_dev, s.nbo_cap, nf_dev, nw_dev, s.ep, b);
-    scan_u64(T, SC, b.w_fsz, b.w_fsz, nw_dev, 1, s.nw_cap + 1, nullptr, st);
-  }
-  scan_file_offsets(T, SC, nf_dev, s.nf_cap, s.ep, b, st);
+  launch_cut_tables(T, nbo_dev, s.nbo_cap, s.ep, b, s.nf_cap, s.nw_cap, s.size_cuts, st);
+  launch_table_layout(T, k, nf_dev, s.nf_cap, s.ep, b, st);
+  scan_u64(T, SC, b.f_off, b.f_off, nf_dev, 1, s.nf_cap + 1, &st->out_bytes, st);
   launch_write_blocks(T, k, nbo_dev, s.nbo_cap, s.ep, b, out, s.out_cap, st, s.wb_groups);
   if (s.ep.bloom_bits) launch_write_filters(T, k, nw_dev, s.nw_cap, nf_dev, s.ep, b, out, s.out_cap, st);
   launch_write_tail(T, k, nf_dev, s.nf_cap, s.ep, b, out, s.out_cap, st);
   if (A.vaddr) launch_value_addrs(T, k, nbo_dev, s.nbo_cap, s.ep, b, out, A.vaddr);
-}
-
-// Block offsets and table cuts of a larger job: scans, table successors and
-// the table orbit over the whole device.
-void run_cuts_large(luda_ctx* c, const EncodeSetup& s, const EncodeArrays& A) {
-  KTimer& T = c->T;
-  DevStatus* st = c->st;
-  const EncBufs& b = A.eb;
-  const u32* nbo_dev = &st->n_blocks_out;
-  ScanState& SC = c->scans;
-  scan_block_offsets(T, SC, nbo_dev, s.nbo_cap, b, st);
-  if (s.size_cuts) {
-    scan_forced(T, SC, nbo_dev, s.nbo_cap, b, st);
-    launch_fnext(T, nbo_dev, s.nbo_cap, &st->n_forced, s.ep, b, st);
-    run_orbit(T, s.of, b.fnext, nbo_dev, &st->file_step_max, b.fs, A.ftab_f, A.ent_f, st);
-  } else {
-    KLAUNCH(T, "table_starts", k_table_starts_forced, grid_for(s.nbo_cap, 256), 256, 0, nbo_dev, b);
-  }
-  scan_files(T, SC, nbo_dev, s.nbo_cap, b, s.nf_cap, st);
 }
 
 // Orbit chunk for successor steps of at most `step` nodes: a chunk must hold
@@ -653,15 +611,6 @@ EncodeSetup plan_encode(const luda_options& o, u32 W, u64 m, u64 out_cap, u64 ex
   const u32 Wb = (u32)std::min<u64>(sb, Cb);
   s.ob = plan_orbit(s.m_cap, Cb, Wb, std::max<u32>(2, std::min<u32>(32, 16384 / Wb)));
   if (Cb > kMaxChunk) throw Fail{LUDA_NOT_SUPPORTED, "block_size too large for the device block cut"};
-  if (s.size_cuts) {
-    // tables: every block but a table's last is at least block_size long
-    const u64 sf = o.max_file_size / (o.block_size + 5) + 2;
-    const u32 Cf = orbit_chunk(std::min<u64>(sf, s.nbo_cap), 1024);
-    if (Cf > kMaxChunk) throw Fail{LUDA_NOT_SUPPORTED, "too many blocks per table for the device table cut"};
-    const u32 Wf = (u32)std::min<u64>(sf, Cf);
-    const u32 Gf = std::max<u32>(2, std::min<u32>(32, 16384 / Wf));
-    s.of = plan_orbit(s.nbo_cap, Cf, Wf, Gf);
-  }
   return s;
 }
 

@@ -18,7 +18,6 @@
 namespace {
 
 constexpr u32 kThreads = 256;
-constexpr u32 kCutSmallBlocks = 16384;  // output blocks of a job whose table cuts run in one workgroup
 constexpr u32 kWaves = kThreads / 64;
 constexpr u32 kStage = 6144;   // bytes of one data block assembled in LDS per wave
 constexpr u32 kFBits = 4096;   // bytes of one bloom filter assembled in LDS per wave
@@ -377,109 +376,6 @@ struct BlockInfoEpi {
   }
 };
 
-// Items of the block offset scan: block bytes with their 5-byte trailer, 0
-// for the end.
-struct BlockTotLoad {
-  const u32* bytes;
-  const u32* nbo_dev;
-  __device__ u32 operator()(u64 i) const { return i < *nbo_dev ? bytes[i] + 5 : 0u; }
-};
-
-// The forced-block scan's epilogue: the list of forced blocks in order.
-struct ForcedListEpi : lbscan::NoEpi {
-  u32* flist;
-  __device__ void item(u64 i, u32 forced, u32 before, const Pre&) const {
-    if (forced) flist[before] = (u32)i;
-  }
-  __device__ void done(u32, u64) const {}
-};
-
-// The table DoCompactionWork would build starting at block i: blocks are
-// added until FileSize() >= max_file_size (db/db_impl.cc:1173-1180), or up to
-// the next block that ShouldStopBefore starts (:1105-1111).
-__global__ void k_fnext(const u32* nbo_dev, const u32* nforced_dev, EncParams ep, EncBufs b, DevStatus* st) {
-  const u64 nbo = *nbo_dev;
-  const u32 nforced = *nforced_dev;
-  u32 smax = 0;
-  GRID_STRIDE(i, nbo) {
-    const u64 base = b.O[i];
-    const u64 target = ep.max_file_size > ~0ull - base ? ~0ull : base + ep.max_file_size;
-    u64 lo = i + 1, hi = nbo;  // smallest k in [i+1, nbo] with O[k] >= target, else nbo
-    while (lo < hi) {
-      const u64 mid = (lo + hi) >> 1;
-      if (b.O[mid] >= target) hi = mid;
-      else lo = mid + 1;
-    }
-    u64 nx = lo;
-    const u32 before = b.ff[i] + b.ob_forced[i];  // forced blocks at or before i
-    if (before < nforced) nx = min(nx, (u64)b.flist[before]);
-    b.fnext[i] = (u32)nx;
-    smax = max(smax, (u32)(nx - i));
-  }
-  smax = wave_max(smax);
-  if ((threadIdx.x & 63) == 0 && smax) atomicMax(&st->file_step_max, smax);
-}
-
-// The table scan's epilogue (over the table-start flags fs): each table's
-// first block, and the block count after the last table.
-struct FileFirstEpi : lbscan::NoEpi {
-  u32* f_first;
-  u32 cap;  // tables the arrays hold
-  __device__ void item(u64 i, u32 start, u32 f, const Pre&) const {
-    if (start && f < cap) f_first[f] = (u32)i;
-  }
-  __device__ void done(u32 nf, u64 nbo) const { f_first[nf] = (u32)nbo; }
-};
-
-__device__ __forceinline__ u32 file_of_block(const EncBufs& b, u32 nf, u64 i) {
-  u32 lo = 0, hi = nf;
-  while (hi - lo > 1) {
-    const u32 mid = (lo + hi) >> 1;
-    if (b.f_first[mid] <= i) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
-
-// Block i's table, offset in it and index entry size (nf >= 1 tables).
-__device__ __forceinline__ void block_layout_item(const Kept& k, const EncBufs& b, u32 nf, u64 i) {
-  const u32 f = file_of_block(b, nf, i);
-  const u64 off = b.O[i] - b.O[b.f_first[f]];
-  b.ob_file[i] = f;
-  b.ob_off[i] = off;
-  const u64 last = (u64)b.ob_first[i] + b.ob_n[i] - 1;
-  const u32 kl = (i + 1 < b.f_first[f + 1]) ? index_sep(k.K, last, b.ob_first[i + 1], nullptr)
-                                            : index_succ(k.K, last, nullptr);
-  const u32 hl = varint_len(off) + varint_len(b.ob_bytes[i]);
-  b.ob_isz[i] = 1 + varint_len(kl) + varint_len(hl) + kl + hl;
-}
-
-__global__ void k_block_layout(Kept k, const u32* nbo_dev, const u32* nf_dev, EncParams ep, EncBufs b) {
-  const u64 nbo = *nbo_dev;
-  const u32 nf = *nf_dev;
-  if (blockIdx.x == 0 && threadIdx.x == 0) b.ob_isz[nbo] = 0;  // the end of the index size scan
-  if (nf == 0) return;  // no tables, or their count overflowed (reported)
-  GRID_STRIDE(i, nbo) block_layout_item(k, b, nf, i);
-}
-
-// Filter windows of a table: FilterBlockBuilder emits one filter per 2 KiB of
-// block start offsets up to the data end, plus one for keys still pending at
-// Finish (table/filter_block.cc:21-49).  Items of the window scan: table f's
-// window count (its data end to f_dend), 0 for the end.
-struct FileWindowsLoad {
-  EncBufs b;
-  EncParams ep;
-  const u32* nf_dev;
-  __device__ u32 operator()(u64 f) const {
-    if (f >= *nf_dev) return 0;
-    const u32 b0 = b.f_first[f], b1 = b.f_first[f + 1];
-    const u64 dend = b.O[b1] - b.O[b0];
-    const u64 last = b.O[b1 - 1] - b.O[b0];
-    b.f_dend[f] = dend;
-    return ep.bloom_bits ? (u32)max(dend >> 11, (last >> 11) + 1) : 0u;
-  }
-};
-
 // BloomFilterPolicy::CreateFilter sizes (util/bloom.cc:30-38).
 __device__ __forceinline__ u64 filter_bytes(u64 keys, u32 bits_per_key) {
   if (!keys) return 0;
@@ -488,218 +384,417 @@ __device__ __forceinline__ u64 filter_bytes(u64 keys, u32 bits_per_key) {
   return (bits + 7) / 8 + 1;
 }
 
-// Every filter window's key count, first entry, table and filter size, with
-// one writer per window and no atomics.  Block i lies in window
-// w(i) = f_win[f] + (ob_off[i] >> 11), which never decreases with i.  The
-// first block of a window writes the window (its keys run to the first block
-// of a later window); the last block of a window also writes the empty
-// windows up to the next block's window, or to the end of its table.
-__device__ __forceinline__ void windows_item(const EncParams& ep, const EncBufs& b, u64 nbo, u32 nw, u64 i);
-
-__global__ void k_windows(const u32* nbo_dev, const u32* nf_dev, const u32* nw_dev, EncParams ep, EncBufs b) {
-  const u64 nbo = *nbo_dev;
-  const u32 nw = *nw_dev;  // 0 when the window count overflowed its capacity
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    b.w_keys[nw] = 0;
-    b.w_first[nw] = 0xffffffffu;
-    b.w_fsz[nw] = 0;
-  }
-  if (*nf_dev == 0 || nw == 0) return;
-  GRID_STRIDE(i, nbo) windows_item(ep, b, nbo, nw, i);
-}
-
-// The windows block i starts or ends (see k_windows).
-__device__ __forceinline__ void windows_item(const EncParams& ep, const EncBufs& b, u64 nbo, u32 nw, u64 i) {
-  {
-    const u32 f = b.ob_file[i];
-    const u32 wf = b.f_win[f];
-    const u64 sw = b.ob_off[i] >> 11;
-    const u32 w = wf + (u32)sw;
-    const bool first = i == 0 || b.ob_file[i - 1] != f || (b.ob_off[i - 1] >> 11) != sw;
-    const bool more = i + 1 < nbo && b.ob_file[i + 1] == f;
-    const u32 wn = more ? wf + (u32)(b.ob_off[i + 1] >> 11) : b.f_win[f + 1];
-    if (first && w < nw) {
-      u64 j = i + 1;
-      while (j < nbo && b.ob_file[j] == f && (b.ob_off[j] >> 11) == sw) j++;
-      const u32 end = j < nbo ? b.ob_first[j] : b.ob_first[nbo - 1] + b.ob_n[nbo - 1];
-      const u32 keys = end - b.ob_first[i];
-      b.w_keys[w] = keys;
-      b.w_first[w] = b.ob_first[i];
-      b.w_file[w] = f;
-      b.w_fsz[w] = filter_bytes(keys, ep.bloom_bits);
-    }
-    if (wn != w) {
-      for (u32 e = w + 1; e < wn && e < nw; e++) {
-        b.w_keys[e] = 0;
-        b.w_first[e] = 0xffffffffu;
-        b.w_file[e] = f;
-        b.w_fsz[e] = 0;
-      }
-    }
-  }
-}
-
 __device__ __forceinline__ u64 meta_block_size(const EncParams& ep, u64 dend, u64 fb) {
   if (!ep.bloom_bits) return 8;
   const u64 hl = varint_len(dend) + varint_len(fb);
   return 3 + 34 + hl + 4 + 4;
 }
 
-// Items of the table offset scan: table f's file size (its filter, index and
-// layout fields along the way), 0 for the end.
-struct FileSizesLoad {
-  EncBufs b;
-  EncParams ep;
-  const u32* nf_dev;
-  __device__ u64 operator()(u64 f) const {
-    if (f >= *nf_dev) return 0;
-    const u32 b0 = b.f_first[f], b1 = b.f_first[f + 1];
-    const u64 dend = b.f_dend[f];
-    u64 fb = 0, size = dend;
-    if (ep.bloom_bits) {
-      const u32 w0 = b.f_win[f], w1 = b.f_win[f + 1];
-      const u64 fbytes = b.w_fsz[w1] - b.w_fsz[w0];
-      b.f_fbytes[f] = fbytes;
-      b.f_fbase[f] = dend - b.w_fsz[w0];
-      fb = fbytes + 4ull * (w1 - w0) + 5;
-      size += fb + 5;
-    }
-    const u64 ib = b.ob_isz[b1] - b.ob_isz[b0] + 4ull * (b1 - b0) + 4;
-    b.f_ibytes[f] = ib;
-    return size + meta_block_size(ep, dend, fb) + 5 + ib + 5 + 48;
-  }
-};
-
 // ---------------------------------------------------------------------------
-// The block offsets and the table cuts of a job of at most kCutSmallBlocks
-// output blocks, in one workgroup of one launch: the same results (O, the
-// tables' first blocks, their count) as the scans, table successors and
-// table orbit that larger jobs run over the device in seven launches of a
-// few microseconds each, whatever the size.  The offsets are scanned over the
-// workgroup with a carry from chunk to chunk; the cuts walk the tables in
-// order, each found by the whole workgroup testing up to 1024 blocks at once
-// against the block offsets and forced-block bits held in LDS.
+// Table cuts and table layout, over the output blocks.
+//
+// A table that starts at block t ends before the first block j > t with
+// O[j] - O[t] >= max_file_size, O being the blocks' offsets in the output's
+// data (FileSize() reached the limit after block j - 1,
+// db/db_impl.cc:1173-1180), or that ShouldStopBefore starts (:1105-1111), or
+// at the end.
+//
+// k_table_next, one workgroup per tile of 4096 blocks: the tile and the next
+// one are staged in LDS as offsets from the tile's start, and for every block
+// of the tile the table that would start there is found by a binary search:
+// its end (tnext, as a distance; 0 when it lies past the staged span), its
+// data bytes and its filter window count (FilterBlockBuilder emits one filter
+// per 2 KiB of block start offsets up to the data end, plus one for the keys
+// pending at Finish, table/filter_block.cc:21-49).  Per tile also its bytes
+// and first forced block, and per block its offset in the tile, for tables
+// longer than the span.
+//
+// k_cut_tables, one workgroup: the real tables are the chain of tnext from
+// block 0.  Waves 1-15 stage tnext in LDS, 32768 blocks at a time, while
+// wave 0 follows the chain through the chunk before (one LDS read a table);
+// a table longer than the span is followed over the tiles' totals.  Then the
+// tables' data ends and first filter windows (numbered across tables).
+//
+// k_table_layout, one workgroup per table: each block's table and offset in
+// it, its index entry (size, then offset in the index block); each filter
+// window's key count, first entry and filter size (then offset in the
+// table's filter data); the table's size, which one scan over the tables
+// turns into its offset in the output.
 
-constexpr u32 kCutThreads = 1024;
-constexpr u32 kCutWaves = kCutThreads / 64;
+constexpr u32 kTnThreads = 1024;
+constexpr u32 kTnSpan = 2 * kTableTile;       // blocks staged: the tile and the next one
+constexpr u32 kTnPer = kTnSpan / kTnThreads;  // staged blocks per thread
+constexpr u32 kTnStarts = kTableTile / kTnThreads;
+constexpr u64 kTcNone = ~0ull;
 
-// Exclusive scan of ld(i), i < n, by the whole workgroup; out(i, prefix) for
-// every item, in any order (every item of a chunk is loaded before any of its
-// outputs is written, so the scan may run in place); returns the total.
-template <typename T, typename Load, typename Out>
+__device__ __forceinline__ u64 table_windows(const EncParams& ep, u64 dend, u64 last) {
+  return ep.bloom_bits ? max(dend >> 11, (last >> 11) + 1) : 0;
+}
+
+__global__ __launch_bounds__(kTnThreads) void k_table_next(const u32* nbo_dev, EncParams ep, EncBufs b, u32 size_cuts,
+                                                             DevStatus* st) {
+  __shared__ u64 sO[kTnSpan + 1];  // bytes of the staged blocks before each (with trailers), from the tile's start
+  __shared__ u16 sNF[kTnSpan];     // the first forced block after each (kTnSpan: none staged)
+  __shared__ u64 s_w[kTnThreads / 64];
+  __shared__ u32 s_f[kTnThreads / 64];
+  if (st->err) return;
+  const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const u64 nbo = *nbo_dev;
+  for (u64 T = blockIdx.x; T * kTableTile < nbo; T += gridDim.x) {
+    const u64 c0 = T * kTableTile;
+    const u32 send = (u32)(min(c0 + kTnSpan, nbo) - c0);  // staged blocks [0, send)
+    const bool at_end = c0 + send == nbo;                   // y = send is then the stream's end
+    const u32 x0 = tid * kTnPer;
+    u64 v[kTnPer];
+    u32 fmask = 0;
+    u64 sum = 0;
+#pragma unroll
+    for (u32 k = 0; k < kTnPer; k++) {
+      const u32 x = x0 + k;
+      const bool real = x < send;
+      v[k] = real ? (u64)b.ob_bytes[c0 + x] + 5 : 0ull;
+      fmask |= (real && b.ob_forced[c0 + x] ? 1u : 0u) << k;
+      sum += v[k];
+    }
+    const u64 incl = wave_incl_sum(sum);
+    if (lane == 63) s_w[wave] = incl;
+    // the first forced block of each thread, and after it (suffix minimum)
+    u32 m = fmask ? x0 + (u32)__builtin_ctz(fmask) : kTnSpan;
+#pragma unroll
+    for (u32 d = 1; d < 64; d <<= 1) {
+      const u32 o = (u32)__shfl_down((int)m, d, 64);
+      if (lane + d < 64) m = min(m, o);
+    }
+    if (lane == 0) s_f[wave] = m;
+    __syncthreads();
+    u64 ex = incl - sum;
+    u32 later = kTnSpan;
+#pragma unroll
+    for (u32 w = 0; w < kTnThreads / 64; w++) {
+      if (w < wave) ex += s_w[w];
+      if (w > wave) later = min(later, s_f[w]);
+    }
+    const u32 mnext = (u32)__shfl_down((int)m, 1, 64);
+    u32 nxt = lane < 63 ? min(mnext, later) : later;
+#pragma unroll
+    for (u32 k = 0; k < kTnPer; k++) {
+      sO[x0 + k] = ex;
+      ex += v[k];
+    }
+    if (tid == kTnThreads - 1) sO[kTnSpan] = ex;
+#pragma unroll
+    for (int k = kTnPer - 1; k >= 0; k--) {
+      sNF[x0 + k] = (u16)nxt;
+      if ((fmask >> k) & 1) nxt = x0 + k;
+    }
+    __syncthreads();
+    const u32 tn = min(send, kTableTile);  // table starts in this tile
+    if (tid == 0) {
+      b.tbytes[T] = sO[tn];
+      const u32 f = (fmask & 1) ? 0u : (u32)sNF[0];
+      b.tforced[T] = f < tn ? (u32)(c0 + f) : 0xffffffffu;
+    }
+    // P(y) = (y is the stream's end || O[y] >= target): monotone in y, on y
+    // in (x, send] (sO[send] is the staged span's total)
+#pragma unroll
+    for (u32 k = 0; k < kTnStarts; k++) {
+      const u32 x = tid + k * kTnThreads;
+      if (x >= tn) break;
+      const u64 ox = sO[x];
+      b.torel[c0 + x] = ox;
+      const u64 target = !size_cuts || ep.max_file_size > kTcNone - ox ? kTcNone : ox + ep.max_file_size;
+      auto P = [&](u32 y) { return (at_end && y == send) || sO[y] >= target; };
+      u32 e = 0xffffffffu;  // none in the span
+      if (P(send)) {
+        u32 lo = x + 1, hi = send;  // P(hi) holds; the first y with P(y) lies in [lo, hi]
+        while (lo < hi) {
+          const u32 mid = (lo + hi) >> 1;
+          if (P(mid)) hi = mid;
+          else lo = mid + 1;
+        }
+        e = lo;
+      }
+      const u32 fx = sNF[x];
+      if (fx < send) e = min(e, fx);
+      if (e != 0xffffffffu) {
+        const u64 dend = sO[e] - ox, last = sO[e - 1] - ox;
+        b.tnext[c0 + x] = (u16)(e - x);
+        b.tdend[c0 + x] = dend;
+        b.tnw[c0 + x] = (u32)table_windows(ep, dend, last);
+      } else {
+        b.tnext[c0 + x] = 0;  // past the span: followed over the tiles' totals
+      }
+    }
+    __syncthreads();  // the next tile reuses the LDS
+  }
+}
+
+// Exclusive scan over the workgroup of ld(i), i < n, one item per thread and
+// round; out(i, prefix) for every item; returns the total.  Every item of a
+// round is loaded before any of its outputs is written, so it may run in
+// place.
+template <u32 NT, typename T, typename Load, typename Out>
 __device__ T wg_scan(u64 n, Load ld, Out out, T* s_w) {
   const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   T carry = 0;
-  for (u64 base = 0; base < n; base += 4ull * kCutThreads) {
-    T v[4];
-    T sum = 0;
-#pragma unroll
-    for (u32 j = 0; j < 4; j++) {
-      const u64 i = base + 4ull * tid + j;
-      v[j] = i < n ? (T)ld(i) : T(0);
-      sum += v[j];
-    }
-    const T incl = wave_incl_sum(sum);
+  for (u64 base = 0; base < n; base += NT) {
+    const u64 i = base + tid;
+    const T v = i < n ? (T)ld(i) : T(0);
+    const T incl = wave_incl_sum(v);
     if (lane == 63) s_w[wave] = incl;
     __syncthreads();
     T before = 0, tot = 0;
-    for (u32 w = 0; w < kCutWaves; w++) {
-      if (w < wave) before += s_w[w];
-      tot += s_w[w];
-    }
-    T ex = carry + before + incl - sum;
 #pragma unroll
-    for (u32 j = 0; j < 4; j++) {
-      const u64 i = base + 4ull * tid + j;
-      if (i < n) out(i, ex);
-      ex += v[j];
+    for (u32 w = 0; w < NT / 64; w++) {
+      const T x = s_w[w];
+      if (w < wave) before += x;
+      tot += x;
     }
+    if (i < n) out(i, carry + before + incl - v);
     carry += tot;
     __syncthreads();
   }
   return carry;
 }
 
-__global__ __launch_bounds__(kCutThreads) void k_cut_small(const u32* nbo_dev, EncParams ep, EncBufs b, u32 nf_cap,
-                                                           u32 size_cuts, DevStatus* st) {
-  __shared__ u32 sO[kCutSmallBlocks + 1];         // block offsets from block 0, when they fit 32 bits
-  __shared__ u32 sF[kCutSmallBlocks / 32 + 1];    // forced blocks, one bit each
-  __shared__ u64 s_w64[kCutWaves];
-  __shared__ u32 s_w32[kCutWaves];
-  __shared__ u32 s_hit;
-  const u32 tid = threadIdx.x;
+constexpr u32 kCtThreads = 1024;
+constexpr u32 kCtChunk = 32768;  // tnext entries staged per step
+
+// A wave-uniform value, moved to scalar registers.
+__device__ __forceinline__ u64 uni64(u64 v) {
+  return (u64)(u32)__builtin_amdgcn_readfirstlane((int)(u32)v) |
+         ((u64)(u32)__builtin_amdgcn_readfirstlane((int)(u32)(v >> 32)) << 32);
+}
+
+// The end of the table starting at block t when it lies past k_table_next's
+// span (so past the tile after t's): whole tiles are passed over by their
+// totals until one holds the size limit or a forced block, which is then
+// searched 64 blocks at a time.  Writes the table's data bytes and windows
+// at t, as k_table_next does for the others.  Wave 0 only, t uniform.
+__device__ u64 table_end_far(const EncParams& ep, const EncBufs& b, u64 nbo, u32 size_cuts, u64 t) {
+  const u32 lane = threadIdx.x & 63;
+  const u64 need = size_cuts ? ep.max_file_size : kTcNone;
+  u64 T = t / kTableTile + 2;
+  u64 acc = b.tbytes[T - 2] - b.torel[t] + b.tbytes[T - 1];  // O[T * tile] - O[t]
+  u64 e = nbo, ob = acc;  // the end and O[e] - O[t]
+  for (;;) {
+    const u64 c0 = T * kTableTile;
+    if (c0 >= nbo) break;  // the stream ends at a tile boundary
+    const u64 tb = b.tbytes[T];
+    const u64 cn = min(c0 + kTableTile, nbo);  // blocks [c0, cn); cn is the next tile's first block or the end
+    const u32 tf = b.tforced[T];
+    u64 se = kTcNone;
+    if (need != kTcNone && acc + tb >= need) {
+      // the first j in [c0, cn] with acc + O_T[j] >= need (O_T[cn] = tb):
+      // the last block of each run of 64, then the run of the first that holds
+      u64 j = c0 + 64ull * lane + 63;
+      bool p = j >= cn || acc + b.torel[j] >= need;
+      u64 mk = __ballot(p);
+      if (!mk) {
+        se = cn;
+      } else {
+        const u64 r = c0 + 64ull * (u64)__builtin_ctzll(mk);
+        j = r + lane;
+        p = j >= cn || acc + b.torel[j] >= need;
+        mk = __ballot(p);  // lane 63 holds
+        se = r + (u64)__builtin_ctzll(mk);
+      }
+      se = uni64(se);
+    }
+    const u64 fe = tf != 0xffffffffu ? (u64)tf : kTcNone;
+    if (se != kTcNone || fe != kTcNone) {
+      e = min(se, fe);
+      ob = acc + (e < cn ? b.torel[e] : tb);
+      break;
+    }
+    acc += tb;
+    ob = acc;
+    T++;
+  }
+  e = uni64(e);
+  ob = uni64(ob);
+  const u64 last = ob - ((u64)b.ob_bytes[e - 1] + 5);
+  if (lane == 0) {
+    b.tdend[t] = ob;
+    b.tnw[t] = (u32)table_windows(ep, ob, last);
+  }
+  return e;
+}
+
+__global__ __launch_bounds__(kCtThreads) void k_cut_tables(const u32* nbo_dev, EncParams ep, EncBufs b, u32 nf_cap,
+                                                             u32 nw_cap, u32 size_cuts, DevStatus* st) {
+  __shared__ u16 sN[2][kCtChunk];  // tnext of a chunk
+  __shared__ u64 s_w[kCtThreads / 64];
+  __shared__ u64 s_nf;
+  const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const u64 nbo = *nbo_dev;
-  if (nbo > kCutSmallBlocks) {  // the host only picks this kernel for capacities this small
-    if (tid == 0) set_err(st, ERR_UNSUPPORTED, SITE_CAPACITY);
+  if (st->err) {
+    // the job is retried or rejected: no tables, so that the table offset
+    // scan reads no sizes of an earlier job
+    if (tid == 0) {
+      st->n_files_out = 0;
+      st->n_windows = 0;
+      b.f_first[0] = (u32)nbo;
+      b.f_win[0] = 0;
+      b.f_off[0] = 0;
+    }
     return;
   }
-  for (u64 w = tid; w <= nbo / 32; w += kCutThreads) sF[w] = 0;
-  __syncthreads();
-  // block offsets (the data block bytes with their trailers), and the forced
-  // blocks' bits from the same loads
-  const u64 data = wg_scan<u64>(
-      nbo + 1,
-      [&](u64 i) -> u64 {
-        if (i >= nbo) return 0;
-        const u32 by = b.ob_bytes[i];
-        if (b.ob_forced[i]) atomicOr(&sF[i >> 5], 1u << (i & 31));
-        return (u64)by + 5;
-      },
-      [&](u64 i, u64 ex) {
-        b.O[i] = ex;
-        sO[i] = (u32)ex;
-      },
-      s_w64);
-  const bool wide = data >> 32;  // offsets past 4 GiB: read them from global memory
-  if (tid == 0) st->data_bytes = data;
-  __syncthreads();
-  auto offset = [&](u64 i) -> u64 { return wide ? b.O[i] : (u64)sO[i]; };
-  auto forced = [&](u64 i) -> bool { return (sF[i >> 5] >> (i & 31)) & 1u; };
-  // tables: a table starting at block t ends before the first block j > t
-  // with O[j] >= O[t] + max_file_size (FileSize() reached the limit after
-  // block j - 1, db/db_impl.cc:1173-1180) or that ShouldStopBefore starts,
-  // or at the end
-  u64 nf = 0;
-  if (size_cuts) {
-    u64 t = 0;
-    while (t < nbo) {
-      if (tid == 0 && nf < nf_cap) b.f_first[nf] = (u32)t;
-      nf++;
-      const u64 ot = offset(t);
-      const u64 target = ep.max_file_size > ~0ull - ot ? ~0ull : ot + ep.max_file_size;
-      for (u64 j0 = t + 1;; j0 += kCutThreads) {
-        if (tid == 0) s_hit = 0xffffffffu;
-        __syncthreads();
-        const u64 j = j0 + tid;
-        if (j >= nbo || offset(j) >= target || forced(j)) atomicMin(&s_hit, tid);
-        __syncthreads();
-        const u32 h = s_hit;
-        __syncthreads();  // everyone has read it before it is reset
-        if (h != 0xffffffffu) {
-          t = min(j0 + h, nbo);
-          break;
+  const u64 nsteps = (nbo + kCtChunk - 1) / kCtChunk;
+  u64 t = 0, nf = 0;  // wave 0: the chain's next table start, tables so far
+  for (u64 s = 0; s <= nsteps; s++) {
+    if (wave > 0 && s < nsteps) {
+      // stage chunk s, 8 entries (16 bytes) per load; the arrays are
+      // 16-byte aligned and chunks start at multiples of 8
+      const u64 c0 = s * kCtChunk;
+      u16* dst = sN[s & 1];
+      for (u32 q = tid - 64; q < kCtChunk / 8; q += kCtThreads - 64) {
+        const u64 i = c0 + 8ull * q;
+        if (i >= nbo) break;
+        if (i + 8 <= nbo) {
+          *reinterpret_cast<uint4*>(dst + 8 * q) = *reinterpret_cast<const uint4*>(b.tnext + i);
+        } else {
+          for (u32 k = 0; i + k < nbo; k++) dst[8 * q + k] = b.tnext[i + k];
         }
       }
     }
-  } else {
-    // tables start at block 0 and at every block ShouldStopBefore starts
-    nf = wg_scan<u32>(
-        nbo, [&](u64 i) { return (u32)(i == 0 || forced(i)); },
-        [&](u64 i, u32 ex) {
-          if ((i == 0 || forced(i)) && ex < nf_cap) b.f_first[ex] = (u32)i;
-        },
-        s_w32);
-  }
-  if (nf > nf_cap) {  // reported for the host's retry; nothing is written
-    if (tid == 0) {
-      st->need_files = (u32)nf;
-      set_err(st, ERR_OVERFLOW, SITE_CAPACITY);
+    if (wave == 0 && s > 0) {
+      const u64 c0 = (s - 1) * kCtChunk, c1 = min(c0 + kCtChunk, nbo);
+      const u16* src = sN[(s - 1) & 1];
+      while (t < c1) {
+        const u32 d = src[t - c0];
+        const u64 e = d ? t + d : table_end_far(ep, b, nbo, size_cuts, t);
+        if (lane == 0 && nf < nf_cap) b.f_first[nf] = (u32)t;
+        nf++;
+        t = uni64(e);
+      }
     }
-    nf = 0;
+    __syncthreads();
   }
+  if (tid == 0) s_nf = nf;
+  __syncthreads();
+  nf = s_nf;
+  if (nf > nf_cap) {  // reported for the host's retry; the tables are not laid out
+    if (tid == 0) {
+      st->need_files = (u32)min<u64>(nf, 0xffffffffull);
+      set_err(st, ERR_OVERFLOW, SITE_CAPACITY);
+      st->n_files_out = 0;
+      st->n_windows = 0;
+      b.f_first[0] = (u32)nbo;
+      b.f_win[0] = 0;
+      b.f_off[0] = 0;
+    }
+    return;
+  }
+  // the tables' data ends, and their windows numbered across tables
+  u64 dsum = 0;
+  const u64 nwt = wg_scan<kCtThreads, u64>(
+      nf,
+      [&](u64 k) -> u64 {
+        const u32 t0 = b.f_first[k];
+        const u64 dend = b.tdend[t0];
+        dsum += dend;
+        b.f_dend[k] = dend;
+        return b.tnw[t0];
+      },
+      [&](u64 k, u64 ex) { b.f_win[k] = (u32)ex; }, s_w);
+  dsum = wave_last(wave_incl_sum(dsum));
+  if (lane == 63) s_w[wave] = dsum;
+  __syncthreads();
   if (tid == 0) {
-    st->n_files_out = (u32)nf;
-    b.f_first[nf] = (u32)nbo;
+    u64 tot = 0;
+    for (u32 w = 0; w < kCtThreads / 64; w++) tot += s_w[w];
+    st->data_bytes = tot;
+    u64 nfo = nf, nwo = nwt;
+    if (nwt > nw_cap) {
+      st->need_windows = (u32)min<u64>(nwt, 0xffffffffull);
+      set_err(st, ERR_OVERFLOW, SITE_CAPACITY);
+      nfo = 0;
+      nwo = 0;
+    }
+    st->n_files_out = (u32)nfo;
+    st->n_windows = (u32)nwo;
+    b.f_first[nfo] = (u32)nbo;
+    b.f_win[nfo] = (u32)nwo;
+    b.f_off[nfo] = 0;  // the end item of the table offset scan
+  }
+}
+
+constexpr u32 kTlThreads = 512;
+
+__global__ __launch_bounds__(kTlThreads) void k_table_layout(Kept k, const u32* nf_dev, EncParams ep, EncBufs b,
+                                                               DevStatus* st) {
+  __shared__ u64 s_w[kTlThreads / 64];
+  if (st->err) return;
+  const u64 nf = *nf_dev;
+  for (u64 f = blockIdx.x; f < nf; f += gridDim.x) {
+    const u32 b0 = b.f_first[f], b1 = b.f_first[f + 1];
+    const u64 dend = b.f_dend[f];
+    // block offsets in the table: the bytes of the blocks before, with their trailers
+    wg_scan<kTlThreads, u64>(
+        b1 - b0, [&](u64 j) -> u64 { return (u64)b.ob_bytes[b0 + j] + 5; },
+        [&](u64 j, u64 ex) {
+          b.ob_off[b0 + j] = ex;
+          b.ob_file[b0 + j] = (u32)f;
+        },
+        s_w);
+    // index entries: the key after block i (a separator from the next
+    // block's first key, or the last key's successor), the block handle
+    const u64 isum = wg_scan<kTlThreads, u64>(
+        b1 - b0,
+        [&](u64 j) -> u64 {
+          const u64 i = b0 + j;
+          const u64 off = b.ob_off[i];
+          const u64 last = (u64)b.ob_first[i] + b.ob_n[i] - 1;
+          const u32 kl = i + 1 < b1 ? index_sep(k.K, last, b.ob_first[i + 1], nullptr) : index_succ(k.K, last, nullptr);
+          const u32 hl = varint_len(off) + varint_len(b.ob_bytes[i]);
+          return 1 + varint_len(kl) + varint_len(hl) + kl + hl;
+        },
+        [&](u64 j, u64 ex) { b.ob_isz[b0 + j] = ex; }, s_w);
+    const u64 ib = isum + 4ull * (b1 - b0) + 4;
+    u64 fb = 0;
+    if (ep.bloom_bits) {
+      // Block i lies in window (its offset >> 11), which never decreases
+      // with i.  The first block of a window writes the window (its keys run
+      // to the first block of a later window, or the table's end); the last
+      // block of a window writes the empty windows up to the next block's
+      // window, or to the table's window count.
+      const u32 w0 = b.f_win[f];
+      const u64 nwin = b.f_win[f + 1] - w0;
+      for (u64 i = b0 + threadIdx.x; i < b1; i += kTlThreads) {
+        const u64 sw = b.ob_off[i] >> 11;
+        const bool first = i == b0 || (b.ob_off[i - 1] >> 11) != sw;
+        const u64 wn = i + 1 < b1 ? b.ob_off[i + 1] >> 11 : nwin;
+        if (first) {
+          u64 q = i + 1;
+          while (q < b1 && (b.ob_off[q] >> 11) == sw) q++;
+          const u32 end = q < b1 ? b.ob_first[q] : b.ob_first[b1 - 1] + b.ob_n[b1 - 1];
+          const u32 keys = end - b.ob_first[i];
+          b.w_keys[w0 + sw] = keys;
+          b.w_first[w0 + sw] = b.ob_first[i];
+          b.w_file[w0 + sw] = (u32)f;
+          b.w_fsz[w0 + sw] = filter_bytes(keys, ep.bloom_bits);
+        }
+        for (u64 e = sw + 1; e < wn; e++) {
+          b.w_keys[w0 + e] = 0;
+          b.w_first[w0 + e] = 0xffffffffu;
+          b.w_file[w0 + e] = (u32)f;
+          b.w_fsz[w0 + e] = 0;
+        }
+      }
+      __syncthreads();
+      const u64 fbytes = wg_scan<kTlThreads, u64>(
+          nwin, [&](u64 j) -> u64 { return b.w_fsz[w0 + j]; }, [&](u64 j, u64 ex) { b.w_fsz[w0 + j] = ex; }, s_w);
+      fb = fbytes + 4ull * nwin + 5;
+      if (threadIdx.x == 0) {
+        b.f_fbytes[f] = fbytes;
+        b.f_fbase[f] = dend;
+      }
+    }
+    if (threadIdx.x == 0) {
+      b.f_ibytes[f] = ib;
+      b.f_off[f] = dend + (ep.bloom_bits ? fb + 5 : 0) + meta_block_size(ep, dend, fb) + 5 + ib + 5 + 48;
+    }
   }
 }
 
@@ -1818,10 +1913,9 @@ __global__ __launch_bounds__(kTailThreads) void k_write_tail(Kept k, const u32* 
     // index block entries, restart interval 1
     {
       u8* ib = base + ioff;
-      const u64 ibase = b.ob_isz[b0];
-      const u64 isum = b.ob_isz[b1] - ibase;
+      const u64 isum = isize - 4ull * (b1 - b0) - 4;  // the entries' bytes, before the restart array
       for (u32 i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
-        const u64 eoff = b.ob_isz[i] - ibase;
+        const u64 eoff = b.ob_isz[i];
         u8* p = ib + eoff;
         const u64 last = (u64)b.ob_first[i] + b.ob_n[i] - 1;
         const bool has_b = i + 1 < b1;
@@ -1923,41 +2017,14 @@ void scan_blocks(KTimer& T, ScanState& S, Kept k, const u32* m_dev, u32 m_cap, E
   lbscan::run<u32, u32>(T, "scan_blocks", S, lbscan::ArrLoad<u32>{b.bs}, (u32*)nullptr, m_dev, 0, m_cap,
                         &st->n_blocks_out, st, &g, BlockInfoEpi{k.forced, b, nbo_cap});
 }
-void scan_block_offsets(KTimer& T, ScanState& S, const u32* nbo_dev, u32 nbo_cap, EncBufs b, DevStatus* st) {
-  lbscan::run<u32, u64>(T, "scan_offsets", S, BlockTotLoad{b.ob_bytes, nbo_dev}, b.O, nbo_dev, 1, nbo_cap + 1,
-                        &st->data_bytes, st);
+void launch_cut_tables(KTimer& T, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u32 nf_cap, u32 nw_cap,
+                       bool size_cuts, DevStatus* st) {
+  KLAUNCH(T, "table_next", k_table_next, grid_for(nbo_cap / kTableTile + 1, 1), kTnThreads, 0, nbo_dev, ep, b,
+          size_cuts ? 1u : 0u, st);
+  KLAUNCH(T, "cut_tables", k_cut_tables, 1, kCtThreads, 0, nbo_dev, ep, b, nf_cap, nw_cap, size_cuts ? 1u : 0u, st);
 }
-void scan_forced(KTimer& T, ScanState& S, const u32* nbo_dev, u32 nbo_cap, EncBufs b, DevStatus* st) {
-  lbscan::run<u32, u32>(T, "scan_forced", S, lbscan::ArrLoad<u32>{b.ob_forced}, b.ff, nbo_dev, 0, nbo_cap,
-                        &st->n_forced, st, nullptr, ForcedListEpi{{}, b.flist});
-}
-void launch_fnext(KTimer& T, const u32* nbo_dev, u32 nbo_cap, const u32* nforced_dev, EncParams ep, EncBufs b,
-                  DevStatus* st) {
-  KLAUNCH(T, "fnext", k_fnext, grid_for(nbo_cap, 256), 256, 0, nbo_dev, nforced_dev, ep, b, st);
-}
-void scan_files(KTimer& T, ScanState& S, const u32* nbo_dev, u32 nbo_cap, EncBufs b, u32 nf_cap, DevStatus* st) {
-  const ScanGuard g{nf_cap, &st->need_files};
-  lbscan::run<u32, u32>(T, "scan_files", S, lbscan::ArrLoad<u32>{b.fs}, (u32*)nullptr, nbo_dev, 0, nbo_cap,
-                        &st->n_files_out, st, &g, FileFirstEpi{{}, b.f_first, nf_cap});
-}
-void launch_block_layout(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, const u32* nf_dev, EncParams ep,
-                         EncBufs b) {
-  KLAUNCH(T, "block_layout", k_block_layout, grid_for(nbo_cap, 256), 256, 0, k, nbo_dev, nf_dev, ep, b);
-}
-void scan_windows(KTimer& T, ScanState& S, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b, u32 nw_cap,
-                  DevStatus* st) {
-  const ScanGuard g{nw_cap, &st->need_windows};
-  lbscan::run<u32, u32>(T, "scan_windows", S, FileWindowsLoad{b, ep, nf_dev}, b.f_win, nf_dev, 1, nf_cap + 1,
-                        &st->n_windows, st, ep.bloom_bits ? &g : nullptr);
-}
-void launch_windows(KTimer& T, const u32* nbo_dev, u32 nbo_cap, const u32* nf_dev, const u32* nw_dev, EncParams ep,
-                    EncBufs b) {
-  KLAUNCH(T, "windows", k_windows, grid_for(nbo_cap, 256), 256, 0, nbo_dev, nf_dev, nw_dev, ep, b);
-}
-void scan_file_offsets(KTimer& T, ScanState& S, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b,
-                       DevStatus* st) {
-  lbscan::run<u64, u64>(T, "scan_files_off", S, FileSizesLoad{b, ep, nf_dev}, b.f_off, nf_dev, 1, nf_cap + 1,
-                        &st->out_bytes, st);
+void launch_table_layout(KTimer& T, Kept k, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b, DevStatus* st) {
+  KLAUNCH(T, "table_layout", k_table_layout, grid_for(nf_cap, 1), kTlThreads, 0, k, nf_dev, ep, b, st);
 }
 void launch_write_blocks(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u8* out,
                          u64 out_cap, DevStatus* st, u32 groups) {
@@ -1971,11 +2038,6 @@ void launch_write_blocks(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, Enc
     KLAUNCH(T, "write_blocks", k_write_blocks<9>, g, kWbThreads, 0, k, nbo_dev, ep, b, out, out_cap, st);
   else
     KLAUNCH(T, "write_blocks", k_write_blocks<0>, g, kWbThreads, 0, k, nbo_dev, ep, b, out, out_cap, st);
-}
-u32 cut_small_max_blocks() { return kCutSmallBlocks; }
-void launch_cut_small(KTimer& T, const u32* nbo_dev, EncParams ep, EncBufs b, u32 nf_cap, bool size_cuts,
-                      DevStatus* st) {
-  KLAUNCH(T, "cut_small", k_cut_small, 1, kCutThreads, 0, nbo_dev, ep, b, nf_cap, size_cuts ? 1u : 0u, st);
 }
 void launch_value_addrs(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u8* out,
                         u64* vaddr) {

@@ -107,3 +107,43 @@ def test_compactions_with_table_cuts_equal_orbit(eng, seed):
     got = [(t.data, t.smallest, t.largest, t.num_entries) for t in out[0]]
     assert got == [(t.data, t.smallest, t.largest, t.num_entries) for t in out3[0]]
     assert got == [(t.data, t.smallest, t.largest, t.num_entries) for t in want]
+
+
+@pytest.mark.parametrize("max_file", [1 << 20, 1 << 30])
+def test_tables_longer_than_the_staged_span(eng, max_file):
+    """Tables of more than 8192 blocks (~45-byte entries in 64-byte blocks:
+    ~10 K blocks to a 1 MiB table, or one table of ~25 K blocks when
+    max_file_size exceeds the output) end past k_table_next's staged span, so
+    the chain follows them over the tiles' totals."""
+    rng = random.Random(11)
+    ents = [(ikey(b"%016d" % i, 5 + i, 1), rng.randbytes(20)) for i in range(50000)]
+    opts = ffi.default_options(block_size=64, max_file_size=max_file)
+    rc, out, msg = eng.build_entries(ents, opts)
+    assert rc == 0, msg
+    assert tables_of(out) == oracle_tables(ents, opts)
+
+
+def test_long_tables_cut_by_should_stop_before(eng):
+    """ShouldStopBefore cuts (grandparents of 30 MB, over the 10 MB overlap
+    limit) every 20000 keys (~10 K blocks), in a stream whose size limit
+    alone makes tables of ~10 K blocks: forced and size ends past the staged
+    span."""
+    rng = random.Random(12)
+    users = [b"g%07d" % i for i in range(100000)]
+    a, b = [], []
+    for i, u in enumerate(users):
+        (a if rng.random() < 0.3 else b).append((ikey(u, i + 1, 1), rng.randbytes(20)))
+    tj = TinyJob(0)
+    tj.level = 1
+    tj.opts = ffi.default_options(block_size=64, max_file_size=1 << 20)
+    tj.files = ([a], [b])
+    tj.snapshot = len(users)
+    tj.grandparents = [(ikey(users[i] + b"~", 7, 1), 30 << 20) for i in range(7000, 100000, 20000)]
+    tj.deeper = []
+    job, keep = tj.make_job(lambda e, o: [t.data for t in oracle.build_tables(e, o, "oracle")])
+    rc1, want = oracle.compact(job, "oracle")
+    rc2, out, msg = eng.compact(job)
+    assert rc1 == 0 and rc2 == 0, msg
+    assert len(want) >= 5
+    got = [(t.data, t.smallest, t.largest, t.num_entries) for t in out[0]]
+    assert got == [(t.data, t.smallest, t.largest, t.num_entries) for t in want]
