@@ -1573,15 +1573,26 @@ __global__ __launch_bounds__(kWbThreads) void k_write_blocks(Kept k, const u32* 
   constexpr bool kFast = NWMAX > 0;
   constexpr bool kPre = kFast && NWMAX <= 3;  // entries loaded a block ahead (registers allow keys <= 16 bytes)
   constexpr u32 NWP = kFast ? NWMAX : 1;
-  __shared__ u32 sl[8][256];         // slicing tables (the fast path uses all eight, the others four)
-  __shared__ u32 zr[4][256];         // 1024 zero bytes (the fast path) or 256 (the others)
-  __shared__ u32 zt[6][4][256];      // 16 << j zero bytes: the joins of crc_copy_out
-  __shared__ uint4 stage_all[kWbWaves][(kStage + 32) / 16];
   union WaveTab {
     ChunkTab ct;  // the LDS path
     ValueTab vt;  // the byte-at-a-time path (NWMAX = 0)
   };
-  __shared__ WaveTab wtab[kWbWaves];
+  // One LDS object with the CRC tables first: their offsets stay below
+  // 64 KiB, so a lookup's table and row go into the ds_read's immediate
+  // offset and only the byte index into a register.
+  struct Lds {
+    u32 sl[8][256];     // slicing tables (the fast path uses all eight, the others four)
+    u32 zr[4][256];     // 1024 zero bytes (the fast path) or 256 (the others)
+    u32 zt[6][4][256];  // 16 << j zero bytes: the joins of crc_copy_out
+    uint4 stage_all[kWbWaves][(kStage + 32) / 16];
+    WaveTab wtab[kWbWaves];
+  };
+  __shared__ Lds lds;
+  auto& sl = lds.sl;
+  auto& zr = lds.zr;
+  auto& zt = lds.zt;
+  auto& stage_all = lds.stage_all;
+  auto& wtab = lds.wtab;
   if (st->out_bytes > out_cap || st->err) return;
   for (u32 i = threadIdx.x; i < 8 * 256; i += blockDim.x) sl[i >> 8][i & 255] = g_crc.slice[i >> 8][i & 255];
   for (u32 i = threadIdx.x; i < 4 * 256; i += blockDim.x) zr[i >> 8][i & 255] = g_crc.zeros[kFast ? 8 : 6][i >> 8][i & 255];
