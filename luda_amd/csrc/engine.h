@@ -295,7 +295,11 @@ struct StageDone {
 };
 void launch_stage_out(KTimer& T, u8* dst, StageSeg s0, StageSeg s1, StageSeg s2, StageDone done);
 
-void launch_files(KTimer& T, const FileDesc* files, u32 nfiles, FileInfo* info, u32* nblk, DevStatus* st);
+// The status (from init) and the descriptors (from src) of a read of tables,
+// both in pinned host memory; footers; blk_base = each table's first data
+// block, the total at blk_base[nfiles].
+void launch_files_in(KTimer& T, const FileDesc* src, const DevStatus* init, FileDesc* files, u32 nfiles,
+                     FileInfo* info, u32* blk_base, DevStatus* st);
 // k_verify.hip: every block's masked CRC (data blocks from launch_index's
 // descriptors, then per table the index, metaindex and named meta blocks);
 // a failure sets ERR_CORRUPT / SITE_CHECKSUM and the lowest bad table in

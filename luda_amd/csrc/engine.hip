@@ -440,6 +440,22 @@ void stage_in(luda_ctx* c, void* dst, const void* src, u64 bytes, const DevStatu
   c->pin2_busy = true;
 }
 
+// The first launch of a read of tables: the descriptors and the initial
+// status go through the pinned staging buffer pin2, which k_files_in reads
+// in place (no copy kernel), then footers and the tables' first blocks.
+void files_in(luda_ctx* c, FileDesc* d_files, const FileDesc* fdesc, u32 n, const DevStatus& init, FileInfo* d_info,
+              u32* d_bbase) {
+  if (c->pin2_busy) stream_sync(c);
+  const u64 bytes = (u64)n * sizeof(FileDesc);
+  const u64 o_st = align_up(bytes, 64);
+  HIPCHK(c->pin2.reserve(o_st + sizeof(DevStatus)));
+  if (bytes) memcpy(c->pin2.p, fdesc, bytes);
+  memcpy(c->pin2.p + o_st, &init, sizeof(DevStatus));
+  launch_files_in(c->T, reinterpret_cast<const FileDesc*>(c->pin2.d),
+                  reinterpret_cast<const DevStatus*>(c->pin2.d + o_st), d_files, n, d_info, d_bbase, c->st);
+  c->pin2_busy = true;
+}
+
 // Device bytes (up to three segments) into the pinned buffer pin, at the
 // segments' offsets, in one kernel; the caller waits with stage_sync before
 // reading pin.  pin must already hold every segment.
@@ -530,11 +546,6 @@ bool start_generic_cuts(const luda_options& o, u64 entries, u64 bytes) {
   const char* e = getenv("LUDA_BLOCK_CUTS");
   if (e && strcmp(e, "orbit") == 0) return true;
   return entries && (double)o.block_size * (double)entries < 16.0 * (double)std::max<u64>(bytes, 1);
-}
-
-u32 grid_for(u64 n, u32 per) {
-  u64 g = (n + per - 1) / per;
-  return (u32)std::min<u64>(std::max<u64>(g, 1), 16384);
 }
 
 // The encoder over the kept stream, counts in st.
@@ -981,7 +992,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
   const bool resident = (job->flags & LUDA_JOB_INPUTS_ON_DEVICE) && find_resident(c->family, files, &dsrc);
   FileDesc* d_files = nullptr;
   FileInfo* d_info = nullptr;
-  u32 *d_nblk = nullptr, *d_bbase = nullptr, *d_nent = nullptr, *d_ebase = nullptr, *d_febase = nullptr;
+  u32 *d_bbase = nullptr, *d_nent = nullptr, *d_ebase = nullptr, *d_febase = nullptr;
   BlockDesc* d_blks = nullptr;
   DevStatus stA;
   memset(&stA, 0, sizeof(stA));
@@ -1005,11 +1016,10 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     u64 nb_cap = 0;
     for (auto* f : files) nb_cap += f->size / 41 + 1;
     if (nb_cap > 0xffffffffull / 2) throw Fail{LUDA_NOT_SUPPORTED, "inputs too large"};
-    auto carve_a = [&](Arena& a, FileDesc*& d_files, FileInfo*& d_info, u32*& d_nblk, u32*& d_bbase, BlockDesc*& d_blks,
+    auto carve_a = [&](Arena& a, FileDesc*& d_files, FileInfo*& d_info, u32*& d_bbase, BlockDesc*& d_blks,
                        u32*& d_nent, u32*& d_ebase, u32*& d_febase) {
       d_files = a.take<FileDesc>(nfiles);
       d_info = a.take<FileInfo>(nfiles);
-      d_nblk = a.take<u32>(nfiles + 1);
       d_bbase = a.take<u32>(nfiles + 1);
       d_blks = a.take<BlockDesc>(nb_cap);
       d_nent = a.take<u32>(nb_cap + 1);
@@ -1019,19 +1029,17 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     {
       Arena dry;
       dry.dry = true;
-      carve_a(dry, d_files, d_info, d_nblk, d_bbase, d_blks, d_nent, d_ebase, d_febase);
+      carve_a(dry, d_files, d_info, d_bbase, d_blks, d_nent, d_ebase, d_febase);
       HIPCHK(c->a_buf.reserve(dry.used));
       Arena a;
       a.base = c->a_buf.p;
       a.cap = c->a_buf.cap;
-      carve_a(a, d_files, d_info, d_nblk, d_bbase, d_blks, d_nent, d_ebase, d_febase);
+      carve_a(a, d_files, d_info, d_bbase, d_blks, d_nent, d_ebase, d_febase);
     }
     DevStatus st0;
     memset(&st0, 0, sizeof(st0));
-    stage_in(c, d_files, fdesc.data(), nfiles * sizeof(FileDesc), &st0);
-    launch_files(T, d_files, nfiles, d_info, d_nblk, c->st);
     HIPCHK(scan_reserve(c->scans, std::max<u64>(nb_cap, nfiles + 1), c->s));
-    scan_u32(T, c->scans, d_nblk, d_bbase, nullptr, nfiles + 1, nfiles + 1, nullptr, c->st);
+    files_in(c, d_files, fdesc.data(), nfiles, st0, d_info, d_bbase);
     launch_index(T, d_files, d_info, d_bbase, nfiles, (u32)nb_cap, d_blks, d_nent, c->st);  // sets n_blocks_in
     scan_entries(T, c->scans, d_blks, d_nent, d_ebase, (u32)nb_cap, nfiles, d_febase, c->st);
     HIPCHK(hipGetLastError());
@@ -1209,12 +1217,22 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
       dp.deep_len = deep_len;
       dp.first_level = (u32)job->level + 2;
 
-      // status for this attempt: phase A counts, phase B zeroed
+      // status for this attempt: phase A counts, phase B zeroed.  After
+      // phase A it is already so (phase A sets neither an error nor
+      // max_ulen, or the job has ended), and the decoder, which needs only
+      // phase A's arrays, is launched first: it starts as soon as the host
+      // has seen phase A end, and the host tables are staged behind it.
       DevStatus sb = stA;
       sb.err = 0;
       sb.site = 0;
       sb.max_ulen = 0;
-      stage_in(c, d_tab, blob.data(), blob.size(), &sb);
+      const bool decode_first = !resident && attempt == 1 && stA.err == 0 && stA.site == 0 && stA.max_ulen == 0;
+      auto decode = [&] {
+        launch_decode(T, d_files, d_blks, nb, d_ebase, K, vptr, vlen, c->st, stA.max_blk, stA.max_nr, stA.max_ent,
+                      tune.decode_groups);
+      };
+      if (decode_first) decode();
+      stage_in(c, d_tab, blob.data(), blob.size(), decode_first ? nullptr : &sb);
       // decode, check, merge, drop, gather, stops
       HIPCHK(scan_reserve(c->scans,
                           std::max<u64>(std::max<u64>(n, es.m_cap), std::max<u64>(es.nbo_cap, std::max(es.nf_cap, es.nw_cap))) + 1,
@@ -1222,9 +1240,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
       // the fused last merge round looks back over its merge tiles
       HIPCHK(scan_reserve_tiles(c->scans, ((u64)n + merge_items_per_tile(W) - 1) / merge_items_per_tile(W) + 1, c->s));
       if (resident) launch_gather_decoded(T, d_dsrc, nfiles, n, K, vptr, vlen);
-      else
-        launch_decode(T, d_files, d_blks, nb, d_ebase, K, vptr, vlen, c->st, stA.max_blk, stA.max_nr, stA.max_ent,
-                      tune.decode_groups);
+      else if (!decode_first) decode();
       if (W <= 8) {
         // Every round but the last writes a permutation (the first reads the
         // children in place and checks they are sorted); the last round
@@ -1371,12 +1387,11 @@ void verify_tables(luda_ctx* c, const luda_file* tables, u32 n, u32 flags, u32* 
   if (nb_cap > 0xffffffffull / 2) throw Fail{LUDA_NOT_SUPPORTED, "tables too large"};
   FileDesc* d_files;
   FileInfo* d_info;
-  u32 *d_nblk, *d_bbase, *d_nent;
+  u32 *d_bbase, *d_nent;
   BlockDesc* d_blks;
   auto carve = [&](Arena& a) {
     d_files = a.take<FileDesc>(n);
     d_info = a.take<FileInfo>(n);
-    d_nblk = a.take<u32>(n + 1);
     d_bbase = a.take<u32>(n + 1);
     d_blks = a.take<BlockDesc>(nb_cap);
     d_nent = a.take<u32>(nb_cap + 1);
@@ -1396,10 +1411,8 @@ void verify_tables(luda_ctx* c, const luda_file* tables, u32 n, u32 flags, u32* 
   DevStatus st0;
   memset(&st0, 0, sizeof(st0));
   st0.bad_file = 0xffffffffu;
-  stage_in(c, d_files, fdesc.data(), n * sizeof(FileDesc), &st0);
-  launch_files(T, d_files, n, d_info, d_nblk, c->st);
   HIPCHK(scan_reserve(c->scans, std::max<u64>(nb_cap, n + 1), c->s));
-  scan_u32(T, c->scans, d_nblk, d_bbase, nullptr, n + 1, n + 1, nullptr, c->st);
+  files_in(c, d_files, fdesc.data(), n, st0, d_info, d_bbase);
   launch_index(T, d_files, d_info, d_bbase, n, (u32)nb_cap, d_blks, d_nent, c->st);
   launch_verify(T, d_files, n, d_blks, (u32)nb_cap, c->st);
   HIPCHK(hipGetLastError());
@@ -1459,12 +1472,11 @@ luda_entries* read_tables(luda_ctx* c, const luda_file* tables, u32 nfiles, u32 
   // footers, index blocks and restart arrays: blocks and entries per table
   FileDesc* d_files;
   FileInfo* d_info;
-  u32 *d_nblk, *d_bbase, *d_nent, *d_ebase, *d_febase;
+  u32 *d_bbase, *d_nent, *d_ebase, *d_febase;
   BlockDesc* d_blks;
   auto carve_a = [&](Arena& a) {
     d_files = a.take<FileDesc>(nfiles);
     d_info = a.take<FileInfo>(nfiles);
-    d_nblk = a.take<u32>(nfiles + 1);
     d_bbase = a.take<u32>(nfiles + 1);
     d_blks = a.take<BlockDesc>(nb_cap);
     d_nent = a.take<u32>(nb_cap + 1);
@@ -1484,10 +1496,8 @@ luda_entries* read_tables(luda_ctx* c, const luda_file* tables, u32 nfiles, u32 
   DevStatus st0;
   memset(&st0, 0, sizeof(st0));
   st0.bad_file = 0xffffffffu;
-  stage_in(c, d_files, fdesc.data(), nfiles * sizeof(FileDesc), &st0);
-  launch_files(T, d_files, nfiles, d_info, d_nblk, c->st);
   HIPCHK(scan_reserve(c->scans, std::max<u64>(nb_cap, nfiles + 1), c->s));
-  scan_u32(T, c->scans, d_nblk, d_bbase, nullptr, nfiles + 1, nfiles + 1, nullptr, c->st);
+  files_in(c, d_files, fdesc.data(), nfiles, st0, d_info, d_bbase);
   launch_index(T, d_files, d_info, d_bbase, nfiles, (u32)nb_cap, d_blks, d_nent, c->st);
   if (flags & LUDA_READ_VERIFY) launch_verify(T, d_files, nfiles, d_blks, (u32)nb_cap, c->st);
   scan_entries(T, c->scans, d_blks, d_nent, d_ebase, (u32)nb_cap, nfiles, d_febase, c->st);
@@ -1643,14 +1653,13 @@ void verify_tables_noalloc(luda_ctx* c, const luda_file* tables, u32 n, u32 flag
   NoAllocDev& D = g_noalloc[c->device];
   FileDesc* d_files;
   FileInfo* d_info;
-  u32 *d_nblk, *d_bbase, *d_nent;
+  u32 *d_bbase, *d_nent;
   BlockDesc* d_blks;
   Arena a;
   a.base = D.ptr + align_up(kNoAllocScanTiles * sizeof(u64), kAlign);
   a.cap = kNoAllocScratch - align_up(kNoAllocScanTiles * sizeof(u64), kAlign);
   d_files = a.take<FileDesc>(n);
   d_info = a.take<FileInfo>(n);
-  d_nblk = a.take<u32>(n + 1);
   d_bbase = a.take<u32>(n + 1);
   d_blks = a.take<BlockDesc>(nb_cap);
   d_nent = a.take<u32>(nb_cap + 1);
@@ -1667,14 +1676,9 @@ void verify_tables_noalloc(luda_ctx* c, const luda_file* tables, u32 n, u32 flag
   hst->bad_file = 0xffffffffu;
   KTimer T;  // no per-kernel timing: it would create events
   T.s = c->s;
-  launch_stage_in(T, c->pin2.d, reinterpret_cast<u8*>(d_files), (u64)n * sizeof(FileDesc),
-                  reinterpret_cast<const DevStatus*>(c->pin2.d + o_st), c->st);
+  launch_files_in(T, reinterpret_cast<const FileDesc*>(c->pin2.d), reinterpret_cast<const DevStatus*>(c->pin2.d + o_st),
+                  d_files, n, d_info, d_bbase, c->st);
   c->pin2_busy = true;
-  launch_files(T, d_files, n, d_info, d_nblk, c->st);
-  {
-    std::lock_guard<std::mutex> lk(g_noalloc_mu);  // the words' epoch is per device
-    scan_u32(T, D.scans, d_nblk, d_bbase, nullptr, n + 1, n + 1, nullptr, c->st);
-  }
   launch_index(T, d_files, d_info, d_bbase, n, (u32)nb_cap, d_blks, d_nent, c->st);
   launch_verify(T, d_files, n, d_blks, (u32)nb_cap, c->st);
   HIPCHK(hipGetLastError());

@@ -21,54 +21,95 @@ typedef u32 u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ u32 wave_incl_scan_u32(u32 v) { return wave_incl_sum(v); }
 
-// One thread per input table: footer -> index block handle.
-__global__ void k_files(const FileDesc* files, u32 nfiles, FileInfo* info, u32* nblk, DevStatus* st) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) nblk[nfiles] = 0;  // the end of the block-count scan
-  GRID_STRIDE(f, nfiles) {
-    nblk[f] = 0;
-    const u8* base = files[f].base;
-    const u64 size = files[f].size;
-    FileInfo fi = {0, 0, 0, 0};
-    info[f] = fi;
-    if (size < 48) {
-      set_err(st, ERR_CORRUPT, SITE_FOOTER);
-      continue;
-    }
-    const u8* ft = base + size - 48;
-    if (ld32(ft + 40) != 0x8b80fb57u || ld32(ft + 44) != 0xdb477524u) {
-      set_err(st, ERR_CORRUPT, SITE_FOOTER);
-      continue;
-    }
-    u64 moff, msize, ioff, isize;
-    const u8* p = ft;
-    const u8* lim = ft + 48;
-    if (!(p = dec_varint64(p, lim, &moff)) || !(p = dec_varint64(p, lim, &msize)) ||
-        !(p = dec_varint64(p, lim, &ioff)) || !(p = dec_varint64(p, lim, &isize))) {
-      set_err(st, ERR_CORRUPT, SITE_FOOTER);
-      continue;
-    }
-    if (ioff > size || isize > size || ioff + isize + 5 > size || isize < 4) {
-      set_err(st, ERR_CORRUPT, SITE_INDEX);
-      continue;
-    }
-    if (base[ioff + isize] == 1) {  // kSnappyCompression
-      set_err(st, ERR_UNSUPPORTED, SITE_COMPRESSED);
-      continue;
-    }
-    const u8* ib = base + ioff;
-    const u32 nr = ld32(ib + isize - 4);
-    if (nr > (isize - 4) / 4) {
-      set_err(st, ERR_CORRUPT, SITE_INDEX);
-      continue;
-    }
-    u32 nb = nr;
-    if (nr == 1 && (ld32(ib + isize - 8) & 0xff) == 0) nb = 0;  // empty table
-    fi.ioff = ioff;
-    fi.isize = isize;
-    fi.nr = nr;
-    info[f] = fi;
-    nblk[f] = nb;
+// One input table: footer -> index block handle (Footer::DecodeFrom,
+// table/format.cc:42-63; Table::Open, table/table.cc:38-82); returns its
+// data block count (the index block's entries), 0 after an error.
+__device__ u32 table_footer(const FileDesc& fd, FileInfo* info, DevStatus* st) {
+  const u8* base = fd.base;
+  const u64 size = fd.size;
+  FileInfo fi = {0, 0, 0, 0};
+  *info = fi;
+  if (size < 48) {
+    set_err(st, ERR_CORRUPT, SITE_FOOTER);
+    return 0;
   }
+  const u8* ft = base + size - 48;
+  if (ld32(ft + 40) != 0x8b80fb57u || ld32(ft + 44) != 0xdb477524u) {
+    set_err(st, ERR_CORRUPT, SITE_FOOTER);
+    return 0;
+  }
+  u64 moff, msize, ioff, isize;
+  const u8* p = ft;
+  const u8* lim = ft + 48;
+  if (!(p = dec_varint64(p, lim, &moff)) || !(p = dec_varint64(p, lim, &msize)) ||
+      !(p = dec_varint64(p, lim, &ioff)) || !(p = dec_varint64(p, lim, &isize))) {
+    set_err(st, ERR_CORRUPT, SITE_FOOTER);
+    return 0;
+  }
+  if (ioff > size || isize > size || ioff + isize + 5 > size || isize < 4) {
+    set_err(st, ERR_CORRUPT, SITE_INDEX);
+    return 0;
+  }
+  if (base[ioff + isize] == 1) {  // kSnappyCompression
+    set_err(st, ERR_UNSUPPORTED, SITE_COMPRESSED);
+    return 0;
+  }
+  const u8* ib = base + ioff;
+  const u32 nr = ld32(ib + isize - 4);
+  if (nr > (isize - 4) / 4) {
+    set_err(st, ERR_CORRUPT, SITE_INDEX);
+    return 0;
+  }
+  u32 nb = nr;
+  if (nr == 1 && (ld32(ib + isize - 8) & 0xff) == 0) nb = 0;  // empty table
+  fi.ioff = ioff;
+  fi.isize = isize;
+  fi.nr = nr;
+  *info = fi;
+  return nb;
+}
+
+// The first launch of a read of tables, one workgroup: the job's status and
+// the table descriptors from the pinned staging buffer (host memory, read in
+// place), the footers (one thread per table), and each table's first data
+// block: blk_base, the exclusive scan of the tables' block counts, with the
+// total at blk_base[nfiles].
+constexpr u32 kFilesThreads = 1024;
+__global__ __launch_bounds__(kFilesThreads) void k_files_in(const FileDesc* src, const DevStatus* init, FileDesc* files,
+                                                            u32 nfiles, FileInfo* info, u32* blk_base,
+                                                            DevStatus* st) {
+  __shared__ u32 s_w[kFilesThreads / 64];
+  const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  {
+    const u32* a = reinterpret_cast<const u32*>(init);
+    u32* b = reinterpret_cast<u32*>(st);
+    for (u32 i = tid; i < sizeof(DevStatus) / 4; i += kFilesThreads) b[i] = a[i];
+  }
+  __syncthreads();  // the status is set before any error is recorded in it
+  u32 carry = 0;
+  for (u32 f0 = 0; f0 < nfiles; f0 += kFilesThreads) {
+    const u32 f = f0 + tid;
+    u32 nb = 0;
+    if (f < nfiles) {
+      const FileDesc fd = src[f];
+      files[f] = fd;
+      nb = table_footer(fd, &info[f], st);
+    }
+    const u32 incl = wave_incl_sum(nb);
+    if (lane == 63) s_w[wave] = incl;
+    __syncthreads();
+    u32 before = 0, tot = 0;
+#pragma unroll
+    for (u32 w = 0; w < kFilesThreads / 64; w++) {
+      const u32 x = s_w[w];
+      if (w < wave) before += x;
+      tot += x;
+    }
+    if (f < nfiles) blk_base[f] = carry + before + incl - nb;
+    carry += tot;
+    __syncthreads();
+  }
+  if (tid == 0) blk_base[nfiles] = carry;
 }
 
 // 24 bytes of global memory as three little-endian words, and their bytes:
@@ -966,8 +1007,9 @@ u32 grid_for(u64 n, u32 per_block) {
 
 }  // namespace
 
-void launch_files(KTimer& T, const FileDesc* files, u32 nfiles, FileInfo* info, u32* nblk, DevStatus* st) {
-  KLAUNCH(T, "files", k_files, grid_for(nfiles, 64), 64, 0, files, nfiles, info, nblk, st);
+void launch_files_in(KTimer& T, const FileDesc* src, const DevStatus* init, FileDesc* files, u32 nfiles,
+                     FileInfo* info, u32* blk_base, DevStatus* st) {
+  KLAUNCH(T, "files", k_files_in, 1, kFilesThreads, 0, src, init, files, nfiles, info, blk_base, st);
 }
 void launch_index(KTimer& T, const FileDesc* files, const FileInfo* info, const u32* blk_base, u32 nfiles,
                   u32 nb_cap, BlockDesc* blks, u32* blk_nent, DevStatus* st) {
