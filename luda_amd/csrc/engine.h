@@ -378,7 +378,7 @@ void scan_blocks(KTimer& T, ScanState& S, Kept k, const u32* m_dev, u32 m_cap, E
 // first filter window (f_dend, f_win, n_windows); f_off[n_files_out] = 0 for
 // the table offset scan.  Two launches: the table that would start at each
 // block, then the chain of those from block 0 in one workgroup.
-constexpr u32 kTableTile = 4096;
+constexpr u32 kTableTile = 1024;
 void launch_cut_tables(KTimer& T, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u32 nf_cap, u32 nw_cap,
                        bool size_cuts, DevStatus* st);
 // Per table: ob_file, ob_off, ob_isz, the windows (w_*), f_fbytes, f_fbase,

@@ -399,8 +399,8 @@ __device__ __forceinline__ u64 meta_block_size(const EncParams& ep, u64 dend, u6
 // db/db_impl.cc:1173-1180), or that ShouldStopBefore starts (:1105-1111), or
 // at the end.
 //
-// k_table_next, one workgroup per tile of 4096 blocks: the tile and the next
-// one are staged in LDS as offsets from the tile's start, and for every block
+// k_table_next, one workgroup per tile of 1024 blocks: the tile and the next
+// three are staged in LDS as offsets from the tile's start, and for every block
 // of the tile the table that would start there is found by a binary search:
 // its end (tnext, as a distance; 0 when it lies past the staged span), its
 // data bytes and its filter window count (FilterBlockBuilder emits one filter
@@ -422,7 +422,8 @@ __device__ __forceinline__ u64 meta_block_size(const EncParams& ep, u64 dend, u6
 // turns into its offset in the output.
 
 constexpr u32 kTnThreads = 1024;
-constexpr u32 kTnSpan = 2 * kTableTile;       // blocks staged: the tile and the next one
+constexpr u32 kTnAhead = 3;                             // tiles staged after the tile
+constexpr u32 kTnSpan = (1 + kTnAhead) * kTableTile;  // blocks staged
 constexpr u32 kTnPer = kTnSpan / kTnThreads;  // staged blocks per thread
 constexpr u32 kTnStarts = kTableTile / kTnThreads;
 constexpr u64 kTcNone = ~0ull;
@@ -567,15 +568,17 @@ __device__ __forceinline__ u64 uni64(u64 v) {
 }
 
 // The end of the table starting at block t when it lies past k_table_next's
-// span (so past the tile after t's): whole tiles are passed over by their
+// span (so past the three tiles after t's): whole tiles are passed over by their
 // totals until one holds the size limit or a forced block, which is then
 // searched 64 blocks at a time.  Writes the table's data bytes and windows
 // at t, as k_table_next does for the others.  Wave 0 only, t uniform.
 __device__ u64 table_end_far(const EncParams& ep, const EncBufs& b, u64 nbo, u32 size_cuts, u64 t) {
   const u32 lane = threadIdx.x & 63;
   const u64 need = size_cuts ? ep.max_file_size : kTcNone;
-  u64 T = t / kTableTile + 2;
-  u64 acc = b.tbytes[T - 2] - b.torel[t] + b.tbytes[T - 1];  // O[T * tile] - O[t]
+  const u64 T0 = t / kTableTile;
+  u64 T = T0 + 1 + kTnAhead;
+  u64 acc = b.tbytes[T0] - b.torel[t];  // O[T * tile] - O[t]
+  for (u64 k = T0 + 1; k < T; k++) acc += b.tbytes[k];
   u64 e = nbo, ob = acc;  // the end and O[e] - O[t]
   for (;;) {
     const u64 c0 = T * kTableTile;
