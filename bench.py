@@ -186,9 +186,19 @@ def run_mix(args):
         beng.set_tuning(tuning["rules"])
     # warm-up: the largest jobs, one per slot, so that every slot's working
     # memory has its final size before the timed batch
+    warm_err = []
+
+    def warm_done(rc, o, msg):
+        if rc != ffi.LUDA_OK:
+            warm_err.append(f"{rc}: {msg}")
+        else:
+            beng.free(o[2])
+
     for shape, b, job, keep, wl in sorted(mix, key=lambda x: -x[1])[:slots]:
-        beng.submit(job.job, lambda rc, o, msg: beng.free(o[2]), keep_on_device=True)
+        beng.submit(job.job, warm_done, keep_on_device=True)
     beng.drain()
+    if warm_err:
+        raise RuntimeError(f"batched warm-up job failed ({warm_err[0]})")
     got = {}
     t0 = time.perf_counter()
     for i, (shape, b, job, keep, wl) in enumerate(mix):

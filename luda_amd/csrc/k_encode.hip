@@ -1980,31 +1980,36 @@ __global__ __launch_bounds__(kTailThreads) void k_write_tail(Kept k, const u32* 
       const u64 mrows = (s_msize + 1 + 255) >> 8;
       const u32 crc = wave_crc32c_rows_global(s_msize + 1, 0, mrows, (const GMEM u8*)ms, sl, zl[6], zl) ^ 0xffffffffu;
       if (lane == 0) st32(ms + s_msize + 1, crc_mask(crc));
-    } else if (wave == 1 && lane == 0) {
-      // footer (Footer::EncodeTo, table/format.cc:31-40)
-      u8* p = base + ioff + isize + 5;
-      u8* s = p;
-      p = enc_varint(p, moff);
-      p = enc_varint(p, msize);
-      p = enc_varint(p, ioff);
-      p = enc_varint(p, isize);
-      while (p < s + 40) *p++ = 0;
-      p = put_fixed32(p, 0x8b80fb57u);
-      p = put_fixed32(p, 0xdb477524u);
-      // table metadata: Output::smallest / largest (db/db_impl.cc:1163-1166)
+    } else if (wave == 1) {
+      // table metadata: Output::smallest / largest (db/db_impl.cc:1163-1166),
+      // the keys' bytes one per lane
       const u64 qs = b.ob_first[b0];
       const u64 ql = (u64)b.ob_first[b1 - 1] + b.ob_n[b1 - 1] - 1;
-      FileMeta fm;
-      fm.offset = b.f_off[f];
-      fm.size = (u64)(p - base);
-      fm.nent = ql + 1 - qs;
-      fm.slen = K.klen[qs] + 8;
-      fm.llen = K.klen[ql] + 8;
-      b.meta[f] = fm;
+      const u32 slen = K.klen[qs] + 8, llen = K.klen[ql] + 8;
+      const u64 tags = K.tag[qs], tagl = K.tag[ql];
       u8* ks = b.keys + f * 2ull * ep.kw;
-      for (u32 t = 0; t < fm.slen; t++) ks[t] = ikey_byte(K, qs, fm.slen - 8, K.tag[qs], t);
       u8* kl = ks + ep.kw;
-      for (u32 t = 0; t < fm.llen; t++) kl[t] = ikey_byte(K, ql, fm.llen - 8, K.tag[ql], t);
+      for (u32 t = lane; t < slen; t += 64) ks[t] = ikey_byte(K, qs, slen - 8, tags, t);
+      for (u32 t = lane; t < llen; t += 64) kl[t] = ikey_byte(K, ql, llen - 8, tagl, t);
+      if (lane == 0) {
+        // footer (Footer::EncodeTo, table/format.cc:31-40)
+        u8* p = base + ioff + isize + 5;
+        u8* s = p;
+        p = enc_varint(p, moff);
+        p = enc_varint(p, msize);
+        p = enc_varint(p, ioff);
+        p = enc_varint(p, isize);
+        while (p < s + 40) *p++ = 0;
+        p = put_fixed32(p, 0x8b80fb57u);
+        p = put_fixed32(p, 0xdb477524u);
+        FileMeta fm;
+        fm.offset = b.f_off[f];
+        fm.size = (u64)(p - base);
+        fm.nent = ql + 1 - qs;
+        fm.slen = slen;
+        fm.llen = llen;
+        b.meta[f] = fm;
+      }
     }
     __syncthreads();
   }
