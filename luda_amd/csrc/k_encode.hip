@@ -1219,6 +1219,57 @@ __device__ __forceinline__ void copy_value_lane(u8* dst, u64 src, u32 vl) {
     if (t < nch) lds_st128(dst + (t + 1 == nch ? vl - 16 : 16 * t), v[t]);
 }
 
+// A value of at most 8 chunks of 16 bytes copied by its own lane in two
+// halves: the first kHoist chunks loaded before the block's other work so
+// that their latency overlaps it, then (the rest loaded) the stores into the
+// stage.  A value's later chunks mostly lie in cache lines the next entry's
+// first chunks fetched: entries are adjacent in their input block.  Chunks
+// are read from src + 16 t, up to 15 bytes past the value, which every value
+// source allows.
+constexpr u32 kHoist = 4;  // four more chunks' registers would spill
+struct ValueRegs {
+  uint4 v[kHoist];
+};
+__device__ __forceinline__ void load_value_lane(ValueRegs& r, u64 src, u32 vl) {
+  const u32 nch = (vl + 15) >> 4;
+#pragma unroll
+  for (u32 t = 0; t < kHoist; t++)
+    if (t < nch) r.v[t] = gload_16(src + 16 * t);
+}
+// The vl <= 128 bytes stored at dst as whole aligned dwords: unaligned LDS
+// stores of 16 bytes cost the block writer a quarter of its time.  The
+// dword slots run from dst & ~3 to dst + vl rounded up; the a = dst & 3
+// bytes of the first slot before the value and the bytes of the last one
+// after it get junk, which the caller then overwrites: they are the entry's
+// own header and key (at least 4 bytes) and the next entry's header or the
+// restart array (at least 3).
+__device__ __forceinline__ void store_value_lane(u8* dst, const ValueRegs& r, u64 src, u32 vl) {
+  const u32 nch = (vl + 15) >> 4;
+  uint4 late[8 - kHoist];
+#pragma unroll
+  for (u32 t = kHoist; t < 8; t++)
+    if (t < nch) late[t - kHoist] = gload_16(src + 16 * t);
+  const u32 a = (u32)(uintptr_t)dst & 3;
+  const u32 nslot = (a + vl + 3) >> 2;
+  const u32 sh = 4 - a;  // slot k = bytes 4 - a.. of source dword k - 1, then source dword k
+  u32* d32 = reinterpret_cast<u32*>(dst - a);
+  u32 prev = 0;
+#pragma unroll
+  for (u32 t = 0; t < 8; t++) {
+    if (t < nch) {
+      const uint4 v = t < kHoist ? r.v[t] : late[t - kHoist];
+      const u32 w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (u32 j = 0; j < 4; j++) {
+        const u32 k = 4 * t + j;
+        if (k < nslot) d32[k] = a ? __builtin_amdgcn_alignbyte(w[j], prev, sh) : w[j];
+        prev = w[j];
+      }
+    }
+  }
+  if (4 * nch < nslot) d32[4 * nch] = __builtin_amdgcn_alignbyte(0u, prev, sh);  // a != 0 here
+}
+
 // Which value chunk each lane of a round takes: entry lanes mark where their
 // chunks start, and a max-scan over the lanes carries each mark forward.
 struct ChunkTab {
@@ -1355,10 +1406,12 @@ __device__ __forceinline__ EntryRegs<NWMAX> load_entry(const Kept& k, const EncB
 __device__ __forceinline__ u32 wave_max_dpp(u32 v) { return (u32)__builtin_amdgcn_readlane((int)wave_incl_max(v), 63); }
 
 // assemble_block_lds for a block of at most 64 entries whose entries were
-// loaded ahead.
+// loaded ahead; with vspread false the values are already in vr
+// (load_value_lane), else the wave spreads their chunks itself.
 template <u32 NWMAX>
 __device__ __forceinline__ void assemble_block_pre(const EncParams& ep, const EntryRegs<NWMAX>& e, u32 cnt, u64 bytes,
-                                                   u8* stage, ChunkTab& ct, DevStatus* st) {
+                                                   u8* stage, ChunkTab& ct, DevStatus* st, const ValueRegs& vr,
+                                                   bool vspread) {
   const u32 j = threadIdx.x & 63;
   const u32 R = ep.interval;
   const u32 ngroups = (cnt + R - 1) / R;
@@ -1367,9 +1420,17 @@ __device__ __forceinline__ void assemble_block_pre(const EncParams& ep, const En
   const bool rs = (j % R) == 0;
   const u32 incl = wave_incl_sum(e.sz);
   const u32 off = incl - e.sz;
-  u32 voff = 0;
+  const u32 il = e.ulen + 8;
+  const u32 voff = off + varint_len(e.sh) + varint_len(il - e.sh) + varint_len(e.vl) + il - e.sh;
+  // values first: store_value_lane leaves junk around each value that the
+  // headers, keys and restart words written after it replace
+  if (!vspread) {
+    if (act) store_value_lane(stage + voff, vr, e.vptr, e.vl);
+  } else {
+    copy_values_spread(stage, ct, act, voff, e.vptr, e.vl);
+  }
+  wave_sync();
   if (act) {
-    const u32 il = e.ulen + 8;
     u64 ik[NWMAX];
     ikey_words_of<NWMAX>(e.uk, e.ulen, e.tag, ik);
     u8* p = stage + off;
@@ -1379,13 +1440,7 @@ __device__ __forceinline__ void assemble_block_pre(const EncParams& ep, const En
     u64 d[NWMAX];
     key_delta_words<NWMAX>(ik, e.sh, d);
     lds_put_words<NWMAX>(p, d, il - e.sh);
-    voff = (u32)(p - stage) + il - e.sh;
     if (rs) lds_st32(stage + dend + 4ull * (j / R), (off << 8) | min(R, cnt - j));
-  }
-  if (wave_max_dpp(act ? (e.vl + 15) >> 4 : 0) <= 8) {
-    if (act) copy_value_lane(stage + voff, e.vptr, e.vl);
-  } else {
-    copy_values_spread(stage, ct, act, voff, e.vptr, e.vl);
   }
   const u32 total = wave_last(incl);
   if (j == 0) {
@@ -1623,26 +1678,40 @@ __global__ __launch_bounds__(kWbThreads) void k_write_blocks(Kept k, const u32* 
   }
   if (i + S < nbo) m1 = block_meta(b, i + S);
   for (; i < nbo; i += S) {
+    // This block's values (entries loaded ahead, values of at most 8 chunks)
+    // are loaded first and waited for once its headers and keys are laid
+    // out; the next blocks' metadata and entries are loaded after that,
+    // while this block is checksummed (registers for both at once would
+    // spill).
+    ValueRegs vr;
+    bool vspread = true;
+    if (p0) {
+      vspread = wave_max_dpp((e0.vl + 15) >> 4) > 8;  // vl is 0 on idle lanes
+      if (!vspread) load_value_lane(vr, e0.vptr, e0.vl);
+    }
+    u8* dst = out + d0;
+    const u64 bytes = m0.bytes;
+    const u32 mis = (u32)((u64)dst & 15);
+    const bool fit0 = fits(m0, (u64)dst);
+    u8* stage = reinterpret_cast<u8*>(stage_all[wave]);
+    if constexpr (kFast) {
+      if (fit0) {
+        if (p0) assemble_block_pre<NWMAX>(ep, e0, m0.n, bytes, stage + mis, wtab[wave].ct, st, vr, vspread);
+        else assemble_block_lds<NWMAX>(k, ep, b, m0.first, m0.n, bytes, stage + mis, wtab[wave].ct, st);
+      }
+    }
     u64 d1 = 0;
     BlockMeta m2 = {0, 0, 0, 0, 0};
     if (i + S < nbo) d1 = b.f_off[m1.file] + m1.off;
     if (i + 2 * S < nbo) m2 = block_meta(b, i + 2 * S);
     const bool p1 = kPre && i + S < nbo && fits(m1, (u64)out + d1) && m1.n <= 64;
     if (p1) e1 = load_entry<NWP>(k, b, R, m1.first, m1.n);
-    u8* dst = out + d0;
-    const u64 bytes = m0.bytes;
-    const u32 mis = (u32)((u64)dst & 15);
-    if (fits(m0, (u64)dst)) {
-      u8* stage = reinterpret_cast<u8*>(stage_all[wave]);
-      if constexpr (kFast) {
-        if (p0) assemble_block_pre<NWMAX>(ep, e0, m0.n, bytes, stage + mis, wtab[wave].ct, st);
-        else assemble_block_lds<NWMAX>(k, ep, b, m0.first, m0.n, bytes, stage + mis, wtab[wave].ct, st);
-      }
+    if (fit0) {
       crc_copy_out(stage_all[wave], mis, (u32)bytes + 1, dst, sl, zr, zt);
     } else {
       if (!kFast && bytes + 5 + 16 <= kStage) {
         assemble_block<NWMAX>(k, ep, b, m0.first, m0.n, bytes,
-                              LdsBuf{reinterpret_cast<u8*>(stage_all[wave]), mis, &wtab[wave].vt}, sl, zr, st);
+                              LdsBuf{stage, mis, &wtab[wave].vt}, sl, zr, st);
         wave_copy_out(dst, stage_all[wave], mis, bytes + 5);
       } else {
         // the row shift of this path is 256 bytes: zr holds it only when !kFast
