@@ -1199,26 +1199,6 @@ __device__ __forceinline__ void lds_put_short(u8* p, const uint4& v, u32 vl) {
   if (vl & 1) *p = (u8)lo;
 }
 
-// The value of one entry lane copied by that lane, 16 bytes at a time (up to
-// 8 chunks, all loaded before any is stored); the last chunk of a value of
-// 16 bytes or more is the one ending at its end, overlapping the one before
-// with the same bytes.
-__device__ __forceinline__ void copy_value_lane(u8* dst, u64 src, u32 vl) {
-  constexpr u32 kMax = 8;
-  const u32 nch = (vl + 15) >> 4;
-  uint4 v[kMax];
-#pragma unroll
-  for (u32 t = 0; t < kMax; t++)
-    if (t < nch) v[t] = gload_16(vl >= 16 && t + 1 == nch ? src + vl - 16 : src + 16 * t);
-  if (vl < 16) {
-    if (vl) lds_put_short(dst, v[0], vl);
-    return;
-  }
-#pragma unroll
-  for (u32 t = 0; t < kMax; t++)
-    if (t < nch) lds_st128(dst + (t + 1 == nch ? vl - 16 : 16 * t), v[t]);
-}
-
 // A value of at most 8 chunks of 16 bytes copied by its own lane in two
 // halves: the first kHoist chunks loaded before the block's other work so
 // that their latency overlaps it, then (the rest loaded) the stores into the
@@ -1226,14 +1206,16 @@ __device__ __forceinline__ void copy_value_lane(u8* dst, u64 src, u32 vl) {
 // first chunks fetched: entries are adjacent in their input block.  Chunks
 // are read from src + 16 t, up to 15 bytes past the value, which every value
 // source allows.
-constexpr u32 kHoist = 4;  // four more chunks' registers would spill
+constexpr u32 kHoist = 4;  // more chunks' registers would spill
+template <u32 KH>
 struct ValueRegs {
-  uint4 v[kHoist];
+  uint4 v[KH];
 };
-__device__ __forceinline__ void load_value_lane(ValueRegs& r, u64 src, u32 vl) {
+template <u32 KH>
+__device__ __forceinline__ void load_value_lane(ValueRegs<KH>& r, u64 src, u32 vl) {
   const u32 nch = (vl + 15) >> 4;
 #pragma unroll
-  for (u32 t = 0; t < kHoist; t++)
+  for (u32 t = 0; t < KH; t++)
     if (t < nch) r.v[t] = gload_16(src + 16 * t);
 }
 // The vl <= 128 bytes stored at dst as whole aligned dwords: unaligned LDS
@@ -1242,52 +1224,69 @@ __device__ __forceinline__ void load_value_lane(ValueRegs& r, u64 src, u32 vl) {
 // bytes of the first slot before the value and the bytes of the last one
 // after it get junk, which the caller then overwrites: they are the entry's
 // own header and key (at least 4 bytes) and the next entry's header or the
-// restart array (at least 3).
-__device__ __forceinline__ void store_value_lane(u8* dst, const ValueRegs& r, u64 src, u32 vl) {
-  const u32 nch = (vl + 15) >> 4;
-  uint4 late[8 - kHoist];
+// restart array (at least 3).  Chunks past the first KH (in r) are loaded
+// KH at a time.
+template <u32 KH>
+__device__ __forceinline__ void store_value_chunks(u32* d32, const uint4 (&v)[KH], u32 t0, u32 nch, u32 nslot, u32 a,
+                                                   u32& prev) {
 #pragma unroll
-  for (u32 t = kHoist; t < 8; t++)
-    if (t < nch) late[t - kHoist] = gload_16(src + 16 * t);
-  const u32 a = (u32)(uintptr_t)dst & 3;
-  const u32 nslot = (a + vl + 3) >> 2;
-  const u32 sh = 4 - a;  // slot k = bytes 4 - a.. of source dword k - 1, then source dword k
-  u32* d32 = reinterpret_cast<u32*>(dst - a);
-  u32 prev = 0;
-#pragma unroll
-  for (u32 t = 0; t < 8; t++) {
-    if (t < nch) {
-      const uint4 v = t < kHoist ? r.v[t] : late[t - kHoist];
-      const u32 w[4] = {v.x, v.y, v.z, v.w};
+  for (u32 t = 0; t < KH; t++) {
+    if (t0 + t < nch) {
+      const u32 w[4] = {v[t].x, v[t].y, v[t].z, v[t].w};
 #pragma unroll
       for (u32 j = 0; j < 4; j++) {
-        const u32 k = 4 * t + j;
-        if (k < nslot) d32[k] = a ? __builtin_amdgcn_alignbyte(w[j], prev, sh) : w[j];
+        const u32 k = 4 * (t0 + t) + j;
+        if (k < nslot) d32[k] = a ? __builtin_amdgcn_alignbyte(w[j], prev, 4 - a) : w[j];
         prev = w[j];
       }
     }
   }
-  if (4 * nch < nslot) d32[4 * nch] = __builtin_amdgcn_alignbyte(0u, prev, sh);  // a != 0 here
+}
+template <u32 KH>
+__device__ __forceinline__ void store_value_lane(u8* dst, const ValueRegs<KH>& r, u64 src, u32 vl) {
+  const u32 nch = (vl + 15) >> 4;
+  const u32 a = (u32)(uintptr_t)dst & 3;
+  const u32 nslot = (a + vl + 3) >> 2;
+  // slot k = bytes 4 - a.. of source dword k - 1, then source dword k
+  u32* d32 = reinterpret_cast<u32*>(dst - a);
+  u32 prev = 0;
+  store_value_chunks<KH>(d32, r.v, 0, nch, nslot, a, prev);
+#pragma unroll
+  for (u32 t0 = KH; t0 < 8; t0 += KH) {
+    if (t0 < nch) {
+      uint4 late[KH];
+#pragma unroll
+      for (u32 t = 0; t < KH; t++)
+        if (t0 + t < nch) late[t] = gload_16(src + 16 * (t0 + t));
+      store_value_chunks<KH>(d32, late, t0, nch, nslot, a, prev);
+    }
+  }
+  if (4 * nch < nslot) d32[4 * nch] = __builtin_amdgcn_alignbyte(0u, prev, 4 - a);  // a != 0 here
 }
 
-// Which value chunk each lane of a round takes: entry lanes mark where their
-// chunks start, and a max-scan over the lanes carries each mark forward.
+// Which value slot each lane of a round takes: entry lanes mark where their
+// slots start, and a max-scan over the lanes carries each mark forward.
 struct ChunkTab {
   u32 mark[64];
-  u32 cst[64];  // first chunk of the entry lane
+  u32 cst[64];  // first slot of the entry lane
   u32 off[64];  // its value's offset in the block
   u32 vl[64];
   u64 src[64];
 };
 
-// The values of a group of up to 64 entries spread over the wave in 16-byte
-// chunks (for values longer than 8 chunks somewhere in the group).
+// The values of a group of up to 64 entries (values longer than 8 chunks
+// somewhere in the group) spread over the wave by 16-byte slots of the
+// stage: a slot inside a value is one unaligned 16-byte load and one aligned
+// LDS store (unaligned LDS stores cost several aligned ones); the partial
+// slots at a value's two ends are stored byte-exact.  Reads stay within 15
+// bytes past a value, which every value source allows.
 __device__ __forceinline__ void copy_values_spread(u8* stage, ChunkTab& ct, bool act, u32 voff, u64 vsrc, u32 vl) {
   const u32 lane = threadIdx.x & 63;
-  const u32 nch = act ? (vl + 15) >> 4 : 0;
-  const u32 cincl = wave_incl_sum(nch);
+  const u32 g = (u32)(uintptr_t)stage & 15;  // stage byte x is 16-byte aligned when (g + x) % 16 == 0
+  const u32 nsl = act && vl ? ((g + voff + vl + 15) >> 4) - ((g + voff) >> 4) : 0;
+  const u32 cincl = wave_incl_sum(nsl);
   const u32 C = wave_last(cincl);
-  const u32 cst = cincl - nch;
+  const u32 cst = cincl - nsl;
   ct.cst[lane] = cst;
   ct.off[lane] = voff;
   ct.vl[lane] = vl;
@@ -1296,7 +1295,7 @@ __device__ __forceinline__ void copy_values_spread(u8* stage, ChunkTab& ct, bool
   for (u32 c0 = 0; c0 < C; c0 += 64) {
     ct.mark[lane] = 0;
     wave_sync();
-    if (nch && cst >= c0 && cst < c0 + 64) ct.mark[cst - c0] = lane + 1;
+    if (nsl && cst >= c0 && cst < c0 + 64) ct.mark[cst - c0] = lane + 1;
     wave_sync();
     u32 m = ct.mark[lane];
     m = wave_incl_max(m);
@@ -1304,12 +1303,12 @@ __device__ __forceinline__ void copy_values_spread(u8* stage, ChunkTab& ct, bool
     carry = wave_last(e);
     const u32 c = c0 + lane;
     if (c < C) {
-      const u32 k = c - ct.cst[e], evl = ct.vl[e];
-      const u32 at = evl >= 16 ? min(16 * k, evl - 16) : 0;
-      const uint4 v = gload_16(ct.src[e] + at);
-      u8* d = stage + ct.off[e] + at;
-      if (evl >= 16) lds_st128(d, v);
-      else lds_put_short(d, v, evl);
+      const int eo = (int)ct.off[e], evl = (int)ct.vl[e];
+      const int x = eo - (int)((g + (u32)eo) & 15) + 16 * (int)(c - ct.cst[e]);  // the slot's first byte
+      const int lo = x > eo ? x : eo, hi = min(x + 16, eo + evl);
+      const uint4 v = gload_16(ct.src[e] + (u32)(lo - eo));
+      if (lo == x && hi == x + 16) *reinterpret_cast<uint4*>(stage + x) = v;
+      else lds_put_short(stage + lo, v, (u32)(hi - lo));
     }
     wave_sync();
   }
@@ -1337,14 +1336,30 @@ __device__ __forceinline__ void assemble_block_lds(const Kept& k, const EncParam
     const u32 sz = act ? (rs ? b.szr[q] : b.sznr[q]) : 0;
     const u32 incl = wave_incl_scan(sz);
     const u64 off = base + incl - sz;
-    u32 vl = 0, voff = 0;
+    u32 vl = 0, voff = 0, ulen = 0, sh = 0;
     u64 vsrc = 0;
     if (act) {
-      const u32 ulen = K.klen[q];
-      const u32 il = ulen + 8;
-      const u32 sh = rs ? 0 : b.shared[q];
+      ulen = K.klen[q];
+      sh = rs ? 0 : b.shared[q];
       vl = k.vlen[q];
       vsrc = k.vptr[q];
+      const u32 il = ulen + 8;
+      voff = (u32)off + varint_len(sh) + varint_len(il - sh) + varint_len(vl) + il - sh;
+    }
+    // values first, as in assemble_block_pre
+    const u32 maxch = wave_max(act ? (vl + 15) >> 4 : 0);
+    if (maxch <= 8) {
+      if (act) {
+        ValueRegs<(NWMAX <= 3 ? 2 : kHoist)> vr;  // registers are short with the loaded-ahead path beside
+        load_value_lane(vr, vsrc, vl);
+        store_value_lane(stage + voff, vr, vsrc, vl);
+      }
+    } else {
+      copy_values_spread(stage, ct, act, voff, vsrc, vl);
+    }
+    wave_sync();
+    if (act) {
+      const u32 il = ulen + 8;
       u64 ik[NWMAX];
       ikey_words<NWMAX>(K, q, ulen, K.tag[q], ik);
       u8* p = stage + off;
@@ -1354,19 +1369,15 @@ __device__ __forceinline__ void assemble_block_lds(const Kept& k, const EncParam
       u64 d[NWMAX];
       key_delta_words<NWMAX>(ik, sh, d);
       lds_put_words<NWMAX>(p, d, il - sh);
-      p += il - sh;
-      voff = (u32)(p - stage);
       if (rs) lds_st32(stage + dend + 4ull * (j / R), ((u32)off << 8) | min(R, cnt - j));
-    }
-    const u32 maxch = wave_max(act ? (vl + 15) >> 4 : 0);
-    if (maxch <= 8) {
-      if (act) copy_value_lane(stage + voff, vsrc, vl);
-    } else {
-      copy_values_spread(stage, ct, act, voff, vsrc, vl);
     }
     base += wave_last(incl);
   }
+  wave_sync();
   if (lane == 0) {
+    // the last value's junk may have reached the first restart word, which
+    // an earlier round wrote: entry 0 starts the block
+    lds_st32(stage + dend, min(R, cnt));
     lds_st32(stage + dend + 4ull * ngroups, ngroups);
     if (base != dend) set_err(st, ERR_CORRUPT, SITE_CAPACITY);
     stage[bytes] = 0;  // kNoCompression
@@ -1410,7 +1421,7 @@ __device__ __forceinline__ u32 wave_max_dpp(u32 v) { return (u32)__builtin_amdgc
 // (load_value_lane), else the wave spreads their chunks itself.
 template <u32 NWMAX>
 __device__ __forceinline__ void assemble_block_pre(const EncParams& ep, const EntryRegs<NWMAX>& e, u32 cnt, u64 bytes,
-                                                   u8* stage, ChunkTab& ct, DevStatus* st, const ValueRegs& vr,
+                                                   u8* stage, ChunkTab& ct, DevStatus* st, const ValueRegs<kHoist>& vr,
                                                    bool vspread) {
   const u32 j = threadIdx.x & 63;
   const u32 R = ep.interval;
@@ -1683,7 +1694,7 @@ __global__ __launch_bounds__(kWbThreads) void k_write_blocks(Kept k, const u32* 
     // out; the next blocks' metadata and entries are loaded after that,
     // while this block is checksummed (registers for both at once would
     // spill).
-    ValueRegs vr;
+    ValueRegs<kHoist> vr;
     bool vspread = true;
     if (p0) {
       vspread = wave_max_dpp((e0.vl + 15) >> 4) > 8;  // vl is 0 on idle lanes
