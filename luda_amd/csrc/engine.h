@@ -267,7 +267,15 @@ struct StageSeg {
   u64 off;
   u64 bytes;
 };
-void launch_stage_in(KTimer& T, const u8* src, u8* dst, u64 bytes, const DevStatus* init, DevStatus* st);
+// n words at p set to v by the same launch (device memory the staging itself
+// does not touch)
+struct StageFill {
+  u32* p;
+  u64 n;
+  u32 v;
+};
+void launch_stage_in(KTimer& T, const u8* src, u8* dst, u64 bytes, const DevStatus* init, DevStatus* st,
+                     StageFill fill = StageFill{nullptr, 0, 0});
 // k_cuts.hip: the output block cuts by tiles of kCutTile entries (staged with
 // kCutSpan more), for chains with at most kCutCand entry points per tile.
 // ftab: kCutCand words per tile (exit | blocks << 12); cand, entry, base:

@@ -429,14 +429,15 @@ void stage_sync(luda_ctx* c) {
 // kernel that reads the pinned staging buffer pin2.  Every job synchronises
 // its stream between two staging calls, so pin2 is free again by then; the
 // flag only guards against a caller that does not.
-void stage_in(luda_ctx* c, void* dst, const void* src, u64 bytes, const DevStatus* init) {
+void stage_in(luda_ctx* c, void* dst, const void* src, u64 bytes, const DevStatus* init,
+              StageFill fill = StageFill{nullptr, 0, 0}) {
   if (c->pin2_busy) stream_sync(c);
   const u64 o_st = align_up(bytes, 64);
   HIPCHK(c->pin2.reserve(o_st + sizeof(DevStatus)));
   if (bytes) memcpy(c->pin2.p, src, bytes);
   if (init) memcpy(c->pin2.p + o_st, init, sizeof(DevStatus));
   launch_stage_in(c->T, c->pin2.d, static_cast<u8*>(dst), bytes,
-                  init ? reinterpret_cast<const DevStatus*>(c->pin2.d + o_st) : nullptr, c->st);
+                  init ? reinterpret_cast<const DevStatus*>(c->pin2.d + o_st) : nullptr, c->st, fill);
   c->pin2_busy = true;
 }
 
@@ -1232,7 +1233,10 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
                       tune.decode_groups);
       };
       if (decode_first) decode();
-      stage_in(c, d_tab, blob.data(), blob.size(), decode_first ? nullptr : &sb);
+      // the fused last merge round's crossing slots start empty (filled
+      // here rather than by a launch of their own)
+      const StageFill slots_fill = W <= 8 && ngp ? StageFill{events, (u64)ngp + 1, 0xffffffffu} : StageFill{nullptr, 0, 0};
+      stage_in(c, d_tab, blob.data(), blob.size(), decode_first ? nullptr : &sb, slots_fill);
       // decode, check, merge, drop, gather, stops
       HIPCHK(scan_reserve(c->scans,
                           std::max<u64>(std::max<u64>(n, es.m_cap), std::max<u64>(es.nbo_cap, std::max(es.nf_cap, es.nw_cap))) + 1,
@@ -1259,7 +1263,6 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
           nbuf ^= 1;
           poff += r.size();
         }
-        if (ngp) fill_u32(T, events, 0xffffffffu, (u64)ngp + 1);  // no crossing yet
         launch_merge_select(T, c->scans, d_pairs + (allp.size() - 1), fin, in, rounds.size() <= 1, K, vptr, vlen, dp,
                             EA.kept, gcnt, kpos, events, splits, c->st);
       } else {

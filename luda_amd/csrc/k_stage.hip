@@ -25,9 +25,10 @@ __device__ __forceinline__ void copy_bytes(const u8* __restrict__ src, u8* __res
 
 __global__ __launch_bounds__(kStageThreads) void k_stage_in(const u8* __restrict__ src, u8* __restrict__ dst,
                                                             u64 bytes, const DevStatus* __restrict__ init,
-                                                            DevStatus* st) {
+                                                            DevStatus* st, StageFill fill) {
   const u64 tid = (u64)blockIdx.x * blockDim.x + threadIdx.x, nth = (u64)gridDim.x * blockDim.x;
   copy_bytes(src, dst, bytes, tid, nth);
+  for (u64 i = tid; i < fill.n; i += nth) fill.p[i] = fill.v;
   if (init && blockIdx.x == 0) {
     const u32* a = reinterpret_cast<const u32*>(init);
     u32* b = reinterpret_cast<u32*>(st);
@@ -70,8 +71,10 @@ u32 stage_grid(u64 bytes) {
 
 static_assert(sizeof(DevStatus) % 4 == 0, "the status is copied as 32-bit words");
 
-void launch_stage_in(KTimer& T, const u8* src, u8* dst, u64 bytes, const DevStatus* init, DevStatus* st) {
-  KLAUNCH(T, "stage_in", k_stage_in, stage_grid(bytes), kStageThreads, 0, src, dst, bytes, init, st);
+void launch_stage_in(KTimer& T, const u8* src, u8* dst, u64 bytes, const DevStatus* init, DevStatus* st,
+                     StageFill fill) {
+  KLAUNCH(T, "stage_in", k_stage_in, stage_grid(std::max<u64>(bytes, 4 * fill.n)), kStageThreads, 0, src, dst, bytes,
+          init, st, fill);
 }
 
 void launch_stage_out(KTimer& T, u8* dst, StageSeg s0, StageSeg s1, StageSeg s2, StageDone done) {
