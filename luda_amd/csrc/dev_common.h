@@ -510,6 +510,18 @@ __device__ __forceinline__ S wave_incl_scan_op(S x, Op op) {
   return x;
 }
 
+// wave_incl_scan_op correct in lanes [0, 2^STEPS) only (STEPS = 3, 4, 5):
+// the steps that carry into the lanes past them are left out.
+template <u32 STEPS, typename S, typename Op>
+__device__ __forceinline__ S wave_incl_scan_op_low(S x, Op op) {
+  x = op(x.template dpp<0x111, 0xf>(), x);  // row_shr:1
+  x = op(x.template dpp<0x112, 0xf>(), x);  // row_shr:2
+  x = op(x.template dpp<0x114, 0xf>(), x);  // row_shr:4
+  if constexpr (STEPS >= 4) x = op(x.template dpp<0x118, 0xf>(), x);  // row_shr:8
+  if constexpr (STEPS >= 5) x = op(x.template dpp<0x142, 0xa>(), x);  // row_bcast:15 into rows 1 and 3
+  return x;
+}
+
 __device__ __forceinline__ u32 wave_incl_sum(u32 v) {
   v += dpp_u32<0x111, 0xf>(v);
   v += dpp_u32<0x112, 0xf>(v);

@@ -667,7 +667,7 @@ __device__ __forceinline__ void decode_entries(const Src& src, u32 n, u64 gp, u6
         x.m[w] = m[w];
         x.v[w] = v[w];
       }
-      x = wave_incl_scan_op(x, [](const KeyBytes<NWMAX>& a, const KeyBytes<NWMAX>& bb) {
+      auto op = [](const KeyBytes<NWMAX>& a, const KeyBytes<NWMAX>& bb) {
         KeyBytes<NWMAX> r;
 #pragma unroll
         for (u32 w = 0; w < NWMAX; w++) {
@@ -675,7 +675,20 @@ __device__ __forceinline__ void decode_entries(const Src& src, u32 n, u64 gp, u6
           r.m[w] = a.m[w] | bb.m[w];
         }
         return r;
-      });
+      };
+      // with wide planes, a last chunk of few entries (blocks of large
+      // values) scans only the lanes it uses; the carry out of a chunk is
+      // read only by a next one.  (Narrow planes keep the one scan: the
+      // extra code would take their registers past four waves per SIMD.)
+      const u32 used = total - e0;
+      if constexpr (NWMAX > 3) {
+        if (used <= 8) x = wave_incl_scan_op_low<3>(x, op);
+        else if (used <= 16) x = wave_incl_scan_op_low<4>(x, op);
+        else if (used <= 32) x = wave_incl_scan_op_low<5>(x, op);
+        else x = wave_incl_scan_op(x, op);
+      } else {
+        x = wave_incl_scan_op(x, op);
+      }
 #pragma unroll
       for (u32 w = 0; w < NWMAX; w++) {
         m[w] = x.m[w];
