@@ -46,21 +46,33 @@ def measure_batched(jobs_list, total_bytes, slots, rules, repeats=2):
         # warm-up: the largest jobs, one per slot, so that every slot's
         # working memory has its final size
         order = sorted(range(len(jobs_list)), key=lambda i: -job_bytes(jobs_list[i]))
+        failed = []
+
+        def warm(rc, o, msg):
+            if rc != ffi.LUDA_OK:
+                failed.append((rc, msg))
+            else:
+                eng.free(o[2])
+
         for i in order[:slots]:
-            eng.submit(jobs_list[i], lambda rc, o, msg: eng.free(o[2]), keep_on_device=True)
+            eng.submit(jobs_list[i], warm, keep_on_device=True)
         eng.drain()
+        if failed:
+            raise RuntimeError(f"warm-up job failed ({failed[0][0]}): {failed[0][1]}")
         best = 0.0
         for _ in range(repeats):
             res = []
             t0 = time.perf_counter()
             for j in jobs_list:
-                eng.submit(j, lambda rc, o, msg: res.append((rc, o)), keep_on_device=True)
+                eng.submit(j, lambda rc, o, msg: res.append((rc, o, msg)), keep_on_device=True)
             eng.drain()
             dt = time.perf_counter() - t0
-            for rc, o in res:
-                if rc != ffi.LUDA_OK:
-                    raise RuntimeError(f"batched job failed ({rc})")
-                eng.free(o[2])
+            bad = [(rc, msg) for rc, o, msg in res if rc != ffi.LUDA_OK]
+            for rc, o, msg in res:
+                if rc == ffi.LUDA_OK:
+                    eng.free(o[2])
+            if bad:
+                raise RuntimeError(f"batched job failed ({bad[0][0]}): {bad[0][1]}")
             best = max(best, total_bytes / dt / 1e9)
         return best
     finally:

@@ -13,6 +13,7 @@
 #include <algorithm>
 
 #include "engine.h"
+#include "stage_values.h"
 #include "scan.h"
 
 namespace {
@@ -1209,59 +1210,27 @@ __device__ __forceinline__ void lds_put_short(u8* p, const uint4& v, u32 vl) {
 constexpr u32 kHoist = 4;  // more chunks' registers would spill
 template <u32 KH>
 struct ValueRegs {
-  uint4 v[KH];
+  sv::Chunk c[KH];
 };
+__device__ __forceinline__ sv::Chunk chunk_at(u64 addr) {
+  const uint4 v = gload_16(addr);
+  return sv::Chunk{{v.x, v.y, v.z, v.w}};
+}
 template <u32 KH>
 __device__ __forceinline__ void load_value_lane(ValueRegs<KH>& r, u64 src, u32 vl) {
   const u32 nch = (vl + 15) >> 4;
 #pragma unroll
   for (u32 t = 0; t < KH; t++)
-    if (t < nch) r.v[t] = gload_16(src + 16 * t);
+    if (t < nch) r.c[t] = chunk_at(src + 16 * t);
 }
-// The vl <= 128 bytes stored at dst as whole aligned dwords: unaligned LDS
-// stores of 16 bytes cost the block writer a quarter of its time.  The
-// dword slots run from dst & ~3 to dst + vl rounded up; the a = dst & 3
-// bytes of the first slot before the value and the bytes of the last one
-// after it get junk, which the caller then overwrites: they are the entry's
-// own header and key (at least 4 bytes) and the next entry's header or the
-// restart array (at least 3).  Chunks past the first KH (in r) are loaded
+// The vl <= 128 bytes from src stored at dst as whole aligned dwords, junk
+// at both ends (sv::store_value); chunks past the first KH (in r) are loaded
 // KH at a time.
 template <u32 KH>
-__device__ __forceinline__ void store_value_chunks(u32* d32, const uint4 (&v)[KH], u32 t0, u32 nch, u32 nslot, u32 a,
-                                                   u32& prev) {
-#pragma unroll
-  for (u32 t = 0; t < KH; t++) {
-    if (t0 + t < nch) {
-      const u32 w[4] = {v[t].x, v[t].y, v[t].z, v[t].w};
-#pragma unroll
-      for (u32 j = 0; j < 4; j++) {
-        const u32 k = 4 * (t0 + t) + j;
-        if (k < nslot) d32[k] = a ? __builtin_amdgcn_alignbyte(w[j], prev, 4 - a) : w[j];
-        prev = w[j];
-      }
-    }
-  }
-}
-template <u32 KH>
 __device__ __forceinline__ void store_value_lane(u8* dst, const ValueRegs<KH>& r, u64 src, u32 vl) {
-  const u32 nch = (vl + 15) >> 4;
-  const u32 a = (u32)(uintptr_t)dst & 3;
-  const u32 nslot = (a + vl + 3) >> 2;
-  // slot k = bytes 4 - a.. of source dword k - 1, then source dword k
-  u32* d32 = reinterpret_cast<u32*>(dst - a);
-  u32 prev = 0;
-  store_value_chunks<KH>(d32, r.v, 0, nch, nslot, a, prev);
-#pragma unroll
-  for (u32 t0 = KH; t0 < 8; t0 += KH) {
-    if (t0 < nch) {
-      uint4 late[KH];
-#pragma unroll
-      for (u32 t = 0; t < KH; t++)
-        if (t0 + t < nch) late[t] = gload_16(src + 16 * (t0 + t));
-      store_value_chunks<KH>(d32, late, t0, nch, nslot, a, prev);
-    }
-  }
-  if (4 * nch < nslot) d32[4 * nch] = __builtin_amdgcn_alignbyte(0u, prev, 4 - a);  // a != 0 here
+  const u32 at = (u32)(uintptr_t)dst;
+  u32* d32 = reinterpret_cast<u32*>(dst - (at & 3));
+  sv::store_value<KH>(at, vl, r.c, [&](u32 t) { return chunk_at(src + 16 * t); }, [&](u32 k, u32 w) { d32[k] = w; });
 }
 
 // Which value slot each lane of a round takes: entry lanes mark where their
@@ -1283,7 +1252,7 @@ struct ChunkTab {
 __device__ __forceinline__ void copy_values_spread(u8* stage, ChunkTab& ct, bool act, u32 voff, u64 vsrc, u32 vl) {
   const u32 lane = threadIdx.x & 63;
   const u32 g = (u32)(uintptr_t)stage & 15;  // stage byte x is 16-byte aligned when (g + x) % 16 == 0
-  const u32 nsl = act && vl ? ((g + voff + vl + 15) >> 4) - ((g + voff) >> 4) : 0;
+  const u32 nsl = act && vl ? sv::spread_slots(g, voff, vl) : 0;
   const u32 cincl = wave_incl_sum(nsl);
   const u32 C = wave_last(cincl);
   const u32 cst = cincl - nsl;
@@ -1303,12 +1272,11 @@ __device__ __forceinline__ void copy_values_spread(u8* stage, ChunkTab& ct, bool
     carry = wave_last(e);
     const u32 c = c0 + lane;
     if (c < C) {
-      const int eo = (int)ct.off[e], evl = (int)ct.vl[e];
-      const int x = eo - (int)((g + (u32)eo) & 15) + 16 * (int)(c - ct.cst[e]);  // the slot's first byte
-      const int lo = x > eo ? x : eo, hi = min(x + 16, eo + evl);
-      const uint4 v = gload_16(ct.src[e] + (u32)(lo - eo));
-      if (lo == x && hi == x + 16) *reinterpret_cast<uint4*>(stage + x) = v;
-      else lds_put_short(stage + lo, v, (u32)(hi - lo));
+      const int eo = (int)ct.off[e];
+      const sv::Slot sl = sv::spread_slot(g, eo, (int)ct.vl[e], c - ct.cst[e]);
+      const uint4 v = gload_16(ct.src[e] + (u32)(sl.lo - eo));
+      if (sl.full) *reinterpret_cast<uint4*>(stage + sl.x) = v;
+      else lds_put_short(stage + sl.lo, v, (u32)(sl.hi - sl.lo));
     }
     wave_sync();
   }
