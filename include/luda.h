@@ -167,7 +167,12 @@ typedef struct luda_ctx luda_ctx;
 
 /* Creates an engine on HIP device `device`.  hbm_budget_bytes bounds the
  * device memory one call may hold (0 = 64 GiB).  Replaces the per-DB
- * gpu::HostAndDeviceMemory (cuda/decode_kv.cu:43-168, db/db_impl.h:215). */
+ * gpu::HostAndDeviceMemory (cuda/decode_kv.cu:43-168, db/db_impl.h:215).
+ * Jobs that should run side by side on one device go through one context's
+ * worker slots (luda_submit): its single-pass scans are built for that.  Two
+ * contexts compacting on one device at once can stall each other's scans,
+ * which then fail the job (LUDA_NOT_SUPPORTED, "prefix sum") after about two
+ * seconds rather than hang. */
 int luda_init(int device, uint64_t hbm_budget_bytes, luda_ctx** out, char* err, size_t errlen);
 /* Frees the context and its device memory: results' device tables live in
  * it, so free every result first.  The context keeps device memory it has

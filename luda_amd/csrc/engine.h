@@ -221,9 +221,12 @@ struct ScanState {
   u64* words = nullptr;
   u64 tiles = 0;
   u32 epoch = 0;  // 1 .. 2^14 - 1; the words are zeroed again when it wraps
+  u32* ticket = nullptr;  // the word after the tiles' words: next tile number of the running scan, 0 between launches
+  bool tickets = false;   // tiles by ticket (worker slots), else by blockIdx
 };
 struct ScanArgs {
   u64* words;
+  u32* ticket;  // null: tile t is workgroup t
   u32 epoch;
   u32 cap;    // with need: a total above cap is stored to *need and replaced by 0
   u32* need;
@@ -241,7 +244,7 @@ hipError_t scan_reserve_tiles(ScanState& S, u64 tiles, hipStream_t s);
 void scan_release(ScanState& S);
 // Look-back words the caller owns (`tiles` u64 at `words`), zeroed on the
 // stream; such a state is never grown or released by the scans.
-hipError_t scan_attach(ScanState& S, u64* words, u64 tiles, hipStream_t s);
+hipError_t scan_attach(ScanState& S, u64* words, u64 words_n, hipStream_t s);  // words_n - 1 tiles, then the ticket
 u32 scan_next_epoch(ScanState& S, hipStream_t s);
 [[noreturn]] void throw_scan_capacity(u64 need, u64 have);
 // Exclusive scan of n = (n_dev ? *n_dev : 0) + n_add items (n <= n_cap);

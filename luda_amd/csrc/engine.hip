@@ -1611,7 +1611,7 @@ luda_entries* read_tables(luda_ctx* c, const luda_file* tables, u32 nfiles, u32 
 constexpr u64 kNoAllocScratch = 72ull << 20;
 constexpr u64 kNoAllocMaxBytes = 128ull << 20;  // sum of table sizes per call
 constexpr u32 kNoAllocMaxTables = 4096;
-constexpr u64 kNoAllocScanTiles = 4096;         // look-back words: scans of up to 16 Mi items
+constexpr u64 kNoAllocScanTiles = 4096;         // look-back words and the tile ticket: scans of up to 16 Mi - 4 Ki items
 static __device__ __align__(256) u8 scratch[kNoAllocScratch];
 
 namespace {
@@ -1982,6 +1982,7 @@ JobPool* pool_of(luda_ctx* c) {
     char err[256];
     if (ctx_open(c->device, c->budget, &w, err, sizeof(err)) != LUDA_OK) break;
     w->family = c;
+    w->scans.tickets = true;  // slots run scans side by side (scan.h, take_ticket)
     w->rules = c->rules;
     w->use_heap(c->heap);
     P->ctxs.push_back(w);

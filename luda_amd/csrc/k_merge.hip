@@ -326,12 +326,12 @@ __global__ __launch_bounds__(256) void k_merge_select(MergePair pr, const u32* i
   __shared__ u16 kx[kTile];    // kept output r of the tile -> its staged item
   __shared__ u32 lastg[256];   // grandparent count of each thread's last output
   __shared__ u32 wsum[4];
-  __shared__ u32 s_pre;
+  __shared__ u32 s_pre, s_ticket;
   const Planes& K = s.K;
   const DropParams& dp = s.dp;
   DevStatus* st = s.sa.st;
   const u32 tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const u32 t = blockIdx.x;
+  const u32 t = lbscan::take_ticket(s.sa, &s_ticket);  // the tile, in the order workgroups start
   const u32 na = pr.a1 - pr.a0, nb = pr.b1 - pr.a1;
   const u64 n = (u64)na + nb;
   const u64 tiles = (n + kTile - 1) / kTile;
@@ -513,6 +513,7 @@ void launch_merge_select(KTimer& T, ScanState& S, const MergePair* d_pair, Merge
   a.kpos = kpos;
   a.slot = slot;
   a.sa.words = S.words;
+  a.sa.ticket = S.tickets ? S.ticket : nullptr;
   a.sa.epoch = scan_next_epoch(S, T.s);
   a.sa.cap = 0;
   a.sa.need = nullptr;

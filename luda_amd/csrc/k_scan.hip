@@ -40,35 +40,38 @@ hipError_t scan_reserve_tiles(ScanState& S, u64 tiles, hipStream_t s) {
   }
   scan_release(S);
   const u64 want = std::max<u64>(tiles, 4096);
-  hipError_t e = hipMalloc(&S.words, want * sizeof(u64));
+  hipError_t e = hipMalloc(&S.words, (want + 1) * sizeof(u64));  // + the ticket word
   if (e != hipSuccess) {
     S.words = nullptr;
     return e;
   }
-  e = zero_words(S.words, want, s);
+  e = zero_words(S.words, want + 1, s);
   if (e != hipSuccess) return e;
   S.tiles = want;
+  S.ticket = reinterpret_cast<u32*>(S.words + want);
   S.epoch = 0;
   return hipSuccess;
 }
 
-hipError_t scan_attach(ScanState& S, u64* words, u64 tiles, hipStream_t s) {
+hipError_t scan_attach(ScanState& S, u64* words, u64 words_n, hipStream_t s) {
   S.words = words;
-  S.tiles = tiles;
+  S.tiles = words_n - 1;  // the last word is the ticket
+  S.ticket = reinterpret_cast<u32*>(words + S.tiles);
   S.epoch = 0;
-  return zero_words(words, tiles, s);
+  return zero_words(words, words_n, s);
 }
 
 void scan_release(ScanState& S) {
   if (S.words) hipFree(S.words);
   S.words = nullptr;
   S.tiles = 0;
+  S.ticket = nullptr;
   S.epoch = 0;
 }
 
 u32 scan_next_epoch(ScanState& S, hipStream_t s) {
   if (++S.epoch >= 0x4000) {
-    const hipError_t e = zero_words(S.words, S.tiles, s);
+    const hipError_t e = zero_words(S.words, S.tiles + 1, s);
     if (e != hipSuccess) launch_refused("scan words reset", e);
     S.epoch = 1;
   }

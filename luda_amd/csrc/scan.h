@@ -91,6 +91,29 @@ __device__ Tout look_back(const ScanArgs& a, u64 t) {
   }
 }
 
+// The tile a workgroup scans.  Workgroups of a launch are spread over the
+// XCDs, which dispatch independently, so workgroup t - 1 may still wait for
+// a CU behind another stream's kernels while t runs and waits on it; when
+// the scans of several streams wait so on one another's unstarted tiles,
+// none moves (worker slots running jobs side by side showed it: the look-
+// backs gave up after their spin limit).  With a.ticket, a tile is taken by
+// ticket first thing, so every tile below a workgroup's is held by one
+// already running, which will publish; the workgroup that takes the
+// launch's last ticket sets the counter back to 0 for the stream's next
+// launch.  The ticket costs an atomic on one word per workgroup (about
+// 15 us on a 256 MiB job's merge), so a context alone on the device keeps
+// tile t = workgroup t.  Called by every thread.
+__device__ __forceinline__ u32 take_ticket(const ScanArgs& a, u32* s_ticket) {
+  if (!a.ticket) return blockIdx.x;
+  if (threadIdx.x == 0) {
+    const u32 t = __hip_atomic_fetch_add(a.ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (t + 1 == gridDim.x) __hip_atomic_store(a.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *s_ticket = t;
+  }
+  __syncthreads();
+  return *s_ticket;
+}
+
 // Publishes tile t's sum, then (t >= 1) looks back and publishes its
 // inclusive prefix; returns the tile's exclusive prefix.  Wave 0 only.
 // 64-bit sums must stay below 2^48 (byte offsets do); larger ones are errors.
@@ -149,9 +172,10 @@ __global__ __launch_bounds__(kThreads) void k_scan(Load ld, Tout* out, const u32
   static_assert(kItems * kWaves % 64 == 0 && kPer <= 4, "wave 0 scans the wave sums");
   __shared__ Tout wsum[kItems * kWaves];  // (row, wave) in item order, then exclusive
   __shared__ Tout s_pre, s_agg;
+  __shared__ u32 s_ticket;
+  const u64 t = take_ticket(a, &s_ticket);
   const u64 n = (n_dev ? *n_dev : 0) + n_add;
   const u64 tiles = (n + kTile - 1) / kTile;
-  const u64 t = blockIdx.x;
   if (t >= (tiles ? tiles : 1)) return;
   const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const u64 base = t * kTile + threadIdx.x;
@@ -226,6 +250,7 @@ void run(KTimer& T, const char* name, ScanState& S, Load ld, Tout* out, const u3
   if (tiles > S.tiles) throw_scan_capacity(tiles, S.tiles);
   ScanArgs a;
   a.words = S.words;
+  a.ticket = S.tickets ? S.ticket : nullptr;
   a.epoch = scan_next_epoch(S, T.s);
   a.st = st;
   a.need = g ? g->need : nullptr;
