@@ -71,8 +71,10 @@ def run_batched(mix, slots, budget):
 
 
 # 100 MiB: every job fits alone, but the admission estimate (64 MiB or more
-# per job) lets only one run at a time
-@pytest.mark.parametrize("slots,budget", [(3, 0), (2, 0), (3, 100 << 20)])
+# per job) lets only one run at a time.  12 slots: many streams' scans side
+# by side, whose look-backs take their tiles by ticket (an 8-slot mix once
+# failed jobs when tiles followed workgroup order)
+@pytest.mark.parametrize("slots,budget", [(3, 0), (2, 0), (3, 100 << 20), (12, 0)])
 def test_batched_matches_alone(mix, slots, budget):
     print("batched", slots, budget, len(mix), flush=True)
     got, n = run_batched(mix, slots, budget)
