@@ -23,8 +23,10 @@ import ctypes as C
 import hashlib
 import json
 import os
+import shutil
 import statistics
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -318,35 +320,62 @@ def run_serve(args):
     seng.set_slots(args.slots)
     # the plain path is the drop-in as it is: every job decodes its inputs;
     # the served form keeps decoded entries resident between chained jobs
-    pbe = served.EngineBackend(peng, opts, resident=False)
-    sbe = served.EngineBackend(seng, opts, resident=True)
+    pbe = served.EngineBackend(peng, opts, resident=False, io_sync=args.io_sync)
+    sbe = served.EngineBackend(seng, opts, resident=True, io_sync=args.io_sync)
+    io_root = tempfile.mkdtemp(prefix="luda_serve_") if args.io else None
+
+    def run_on(be, run, interval, tag, **kw):
+        """One run of the stream; with --io into a fresh directory, whose
+        files must then be exactly the final tree's."""
+        lsm = served.Lsm()
+        if io_root:
+            be.io_dir = os.path.join(io_root, tag)
+            os.makedirs(be.io_dir)
+            be.io = dict.fromkeys(be.io, 0)
+            lsm.on_delete = be.delete_file
+        be.acct = dict.fromkeys(be.acct, 0)
+        lsm, st = run(be, mems, interval, lsm=lsm, **kw)
+        summ = st.summary()
+        summ["engine_calls"] = {k: round(v, 3) for k, v in be.acct.items()}
+        if io_root:
+            tree = {"%06d.ldb" % f.number: f for level in lsm.levels for f in level}
+            on_disk = set(os.listdir(be.io_dir))
+            same = on_disk == set(tree) and all(os.path.getsize(os.path.join(be.io_dir, n)) == tree[n].size
+                                                for n in tree)
+            summ["io"] = dict(be.io, write_ms=round(be.io["write_ms"], 3), tree_files_on_disk=same)
+            if not same:
+                raise RuntimeError(f"{tag}: the directory does not hold the tree's files")
+        return lsm, st, summ
+
+    def end_run(be, lsm):
+        release_tree(lsm)
+        if io_root:
+            shutil.rmtree(be.io_dir)
+            be.io_dir = None
+
     # warm-up: the whole stream through both paths, so that every working
     # buffer and output buffer is allocated before anything is timed
     for be, run in ((pbe, served.run_plain), (sbe, served.run_served)):
-        lsm, wst = run(be, mems, 0.0)
-        release_tree(lsm)
-        log(f"warm-up {run.__name__}: {wst.summary()['jobs']} jobs in {wst.summary()['seconds']} s")
-    pbe.acct = dict.fromkeys(pbe.acct, 0)
-    lsm, st = served.run_plain(pbe, mems, 0.0)
-    cal = st.summary()
-    cal["engine_calls"] = {k: round(v, 3) for k, v in pbe.acct.items()}
-    release_tree(lsm)
+        lsm, wst, ws = run_on(be, run, 0.0, "warm_" + run.__name__)
+        end_run(be, lsm)
+        log(f"warm-up {run.__name__}: {ws['jobs']} jobs in {ws['seconds']} s")
+    lsm, st, cal = run_on(pbe, served.run_plain, 0.0, "calibration")
+    end_run(pbe, lsm)
     per_flush_ms = (st.end_ms - st.t0_ms) / len(mems)
     log(f"plain path, whole stream at once: {cal}")
     out = {"metric": "served compaction: aggregate input GB/s and p50/p99 job latency, served form vs the plain path",
            "puts": args.serve, "memtables": len(mems), "slots": args.slots,
            "workload": "fillrandom: 16-byte keys uniform over 1e9, 100-byte random values, 4 MiB memtables "
-                       "(MemTable::Add bytes), LevelDB picking/scoring rules, tables resident in HBM",
+                       "(MemTable::Add bytes), LevelDB picking/scoring rules, tables resident in HBM"
+                       + ("; every table also written to a file" + (" and synced" if args.io_sync else "")
+                          if args.io else ""),
            "calibration": cal, "sustainable_flush_interval_ms": round(per_flush_ms, 4), "rates": []}
     checked = [0, 0]
     for k in [float(x) for x in args.rates.split(",")]:
         interval = per_flush_ms / k
         row = {"rate": k, "flush_interval_ms": round(interval, 4)}
-        pbe.acct = dict.fromkeys(pbe.acct, 0)
-        lsm, st = served.run_plain(pbe, mems, interval)
-        row["plain"] = st.summary()
-        row["plain"]["engine_calls"] = {k: round(v, 3) for k, v in pbe.acct.items()}
-        release_tree(lsm)
+        lsm, st, row["plain"] = run_on(pbe, served.run_plain, interval, f"plain_{k}")
+        end_run(pbe, lsm)
         samples = []
 
         def on_job(spec, files):
@@ -358,12 +387,9 @@ def run_serve(args):
                 samples.append((spec, files))
             samples_seen.append(1)
         samples_seen = []
-        sbe.acct = dict.fromkeys(sbe.acct, 0)
-        lsm, st = served.run_served(sbe, mems, interval, on_job=on_job)
-        row["served"] = st.summary()
-        row["served"]["engine_calls"] = {k: round(v, 3) for k, v in sbe.acct.items()}
+        lsm, st, row["served"] = run_on(sbe, served.run_served, interval, f"served_{k}", on_job=on_job)
         lsm.check()
-        release_tree(lsm)
+        end_run(sbe, lsm)
         scratch = served.Lsm()
         for spec, kept in samples:
             want = [sha(sbe.table_bytes(f)) for f in kept]
@@ -381,6 +407,10 @@ def run_serve(args):
     out["jobs_checked_alone"] = checked[0]
     out["same_alone"] = checked[0] == checked[1]
     print(json.dumps(out), flush=True)
+    sbe.close()
+    pbe.close()
+    if io_root:
+        shutil.rmtree(io_root, ignore_errors=True)
     seng.close()
     peng.close()
 
@@ -425,6 +455,11 @@ def main():
                     help="instead: a fillrandom stream of PUTS puts through the served form against the plain "
                          "path at --rates multiples of the plain path's sustainable rate, one JSON line")
     ap.add_argument("--rates", default="1,2,4")
+    ap.add_argument("--io", action="store_true",
+                    help="with --serve: every table is also written to a file (a fresh directory under $TMPDIR per "
+                         "run), the served form on a writer thread beside its kernels, the plain path before its "
+                         "next job; compactions delete their inputs' files")
+    ap.add_argument("--io-sync", action="store_true", help="with --io: fsync every file")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (the known-size kernel that rocprofv3's "
                          "FETCH_SIZE/WRITE_SIZE are scaled against) and exit")

@@ -219,6 +219,7 @@ class Engine:
 
     def __init__(self, device=0, budget=0):
         self.lib = ffi.engine()
+        self.device = device
         self.ctx = C.c_void_p()
         err = C.create_string_buffer(512)
         rc = self.lib.luda_init(device, budget, C.byref(self.ctx), err, 512)

@@ -32,6 +32,16 @@ Rules followed (reference files):
     BackgroundCompaction db/db_impl.cc:694-711);
   * install: inputs deleted, outputs added at level+1 (db/db_impl.cc:1040-1056).
 
+With an output directory (EngineBackend io_dir) every table is also written
+to a file, as DoCompactionWork's outputs are (db/db_impl.cc:1490-1565,
+FinishCompactionOutputFile :1000-1014): the plain path writes a job's files
+before it takes the next job, the served form hands finished results to a
+writer thread whose luda_write_outputs copies them from HBM on the context's
+I/O stream, beside the slots' kernels, and a job counts as done (and is
+installed) once its files are written; a compaction's inputs are deleted
+from the directory when it installs (DeleteObsoleteFiles, db/db_impl.cc:
+206-284).
+
 The served form may run compactions of disjoint level pairs at the same time
 (the reference runs one at a time, db/db_impl.cc:865-871); a job's inputs,
 grandparents and deeper-level ranges are fixed when it is submitted, so its
@@ -39,6 +49,9 @@ output is the same as luda_compact's on the same job.
 """
 import bisect
 import ctypes as C
+import os
+import queue
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -222,6 +235,7 @@ class Lsm:
         self.levels = [[] for _ in range(NUM_LEVELS)]
         self.compact_pointer = [None] * NUM_LEVELS
         self.busy = set()          # levels a running compaction reads or writes
+        self.on_delete = None      # called with each file a compaction deletes
         self.next_number = 2
         self.last_seq = 0
         self.max_file_size = max_file_size
@@ -450,6 +464,8 @@ class Lsm:
         for which in (0, 1):
             for f in spec.inputs[which]:
                 self._remove(spec.level + which, f)
+                if self.on_delete is not None:
+                    self.on_delete(f)
         for f in outputs:
             self.add_file(spec.level + 1, f)
         for which in (0, 1):
@@ -519,7 +535,7 @@ class EngineBackend:
     `resident` their decoded entries too (LUDA_JOB_KEEP_DECODED), so that a
     compaction of tables the engine wrote decodes nothing."""
 
-    def __init__(self, engine, opts, resident=True):
+    def __init__(self, engine, opts, resident=True, io_dir=None, io_sync=False, io_threads=1):
         self.eng = engine
         self.opts = opts
         self.resident = resident
@@ -527,6 +543,72 @@ class EngineBackend:
         C.pointer(self.flush_opts)[0] = opts
         self.flush_opts.max_file_size = (1 << 64) - 1  # BuildTable writes one table
         self.acct = {"jobs": 0, "device_ms": 0.0, "call_ms": 0.0}  # summed over the engine's calls
+        self.io_dir = io_dir    # tables also written to files here (None: HBM only)
+        self.io_sync = io_sync  # fsync each file, as FinishCompactionOutputFile does
+        self.io = {"files": 0, "bytes": 0, "write_ms": 0.0, "deleted": 0}
+        # queued jobs' writer threads, each on a context of its own (a
+        # context's I/O staging buffers serve one write at a time)
+        self.io_threads = io_threads
+        self._wq = self._doneq = None
+        self._writers = []
+        self._io_lock = threading.Lock()
+
+    # -- output files
+    def path_of(self, f):
+        return os.path.join(self.io_dir, "%06d.ldb" % f.number)
+
+    def _write(self, res, files, eng=None):
+        t = time.perf_counter()
+        rc, msg = (eng or self.eng).write_outputs(res, [self.path_of(f) for f in files], sync=self.io_sync)
+        if rc != ffi.LUDA_OK:
+            raise RuntimeError(f"luda_write_outputs failed ({rc}): {msg}")
+        with self._io_lock:
+            self.io["files"] += len(files)
+            self.io["bytes"] += sum(f.size for f in files)
+            self.io["write_ms"] += (time.perf_counter() - t) * 1e3
+
+    def delete_file(self, f):
+        """Lsm.on_delete: a compaction's input leaves the directory."""
+        if self.io_dir is not None:
+            os.unlink(self.path_of(f))
+            self.io["deleted"] += 1
+
+    def _writer_loop(self, eng):
+        while True:
+            item = self._wq.get()
+            if item is None:
+                return
+            res, files, finish = item
+            try:
+                self._write(res, files, eng)
+                self._doneq.put((finish, None, _now_ms()))
+            except Exception as e:  # re-raised by poll on the driving thread
+                self._doneq.put((finish, e, None))
+
+    def _finish_async(self, res, files, finish):
+        """Queued jobs: finish(done_ms) runs from poll once the files are
+        written (at once without an output directory)."""
+        if self.io_dir is None or not files:
+            finish(None)
+            return
+        if not self._writers:
+            from . import jobs
+            self._wq, self._doneq = queue.Queue(), queue.Queue()
+            for i in range(max(1, self.io_threads)):
+                eng = self.eng if i == 0 else jobs.Engine(self.eng.device)
+                t = threading.Thread(target=self._writer_loop, args=(eng,), daemon=True)
+                t.start()
+                self._writers.append((t, eng))
+        self._wq.put((res, files, finish))
+
+    def close(self):
+        for _ in self._writers:
+            self._wq.put(None)
+        for t, eng in self._writers:
+            t.join()
+            if eng is not self.eng:
+                eng.close()
+        self._writers = []
 
     def _account(self, stats):
         self.acct["jobs"] += 1
@@ -562,7 +644,11 @@ class EngineBackend:
         if rc != ffi.LUDA_OK:
             raise RuntimeError(f"luda_build_tables failed ({rc}): {msg}")
         self._account(out[1])
-        return self._files(lsm, out[2]), out[1]["end_clock_ms"]
+        files = self._files(lsm, out[2])
+        if self.io_dir is not None and files:
+            self._write(out[2], files)
+            return files, _now_ms()
+        return files, out[1]["end_clock_ms"]
 
     def compact(self, lsm, spec):
         job, keep = _job_struct(spec, self.opts)
@@ -570,7 +656,11 @@ class EngineBackend:
         if rc != ffi.LUDA_OK:
             raise RuntimeError(f"luda_compact failed ({rc}): {msg}")
         self._account(out[1])
-        return self._files(lsm, out[2]), out[1]["end_clock_ms"]
+        files = self._files(lsm, out[2])
+        if self.io_dir is not None and files:
+            self._write(out[2], files)
+            return files, _now_ms()
+        return files, out[1]["end_clock_ms"]
 
     # -- queued on the worker slots
     def submit_flush(self, lsm, mem, done):
@@ -581,7 +671,8 @@ class EngineBackend:
             if rc != ffi.LUDA_OK:
                 raise RuntimeError(f"queued flush failed ({rc}): {msg}")
             self._account(out[1])
-            done(self._files(lsm, out[2]), out[1]["end_clock_ms"], hold)
+            files, end = self._files(lsm, out[2]), out[1]["end_clock_ms"]
+            self._finish_async(out[2], files, lambda t: done(files, t or end, hold))
         self.eng.submit_build(self.flush_opts, mem.n, k.ctypes.data, ko.ctypes.data, v.ctypes.data, vo.ctypes.data,
                               cb, keep_on_device=True, keep_decoded=self.resident)
 
@@ -593,11 +684,18 @@ class EngineBackend:
             if rc != ffi.LUDA_OK:
                 raise RuntimeError(f"queued compaction failed ({rc}): {msg}")
             self._account(out[1])
-            done(self._files(lsm, out[2]), out[1]["end_clock_ms"])
+            files, end = self._files(lsm, out[2]), out[1]["end_clock_ms"]
+            self._finish_async(out[2], files, lambda t: done(files, t or end))
         self.eng.submit(job, cb, keep_on_device=True, keep_decoded=self.resident)
 
     def poll(self):
-        return self.eng.poll()
+        n = self.eng.poll()
+        while self._doneq is not None and not self._doneq.empty():
+            finish, err, done_ms = self._doneq.get()
+            if err is not None:
+                raise err
+            finish(done_ms)
+        return n
 
 
 # ---------------------------------------------------------------------------

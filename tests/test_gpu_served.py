@@ -71,3 +71,36 @@ def test_served_stream_matches_alone(slots):
     finally:
         peng.close()
         beng.close()
+
+
+@pytest.mark.parametrize("mode", ["served", "plain"])
+def test_stream_with_output_files(tmp_path, mode):
+    """Every table also written to a file (the served form on its writer
+    thread beside the slots' kernels, the plain path before its next job):
+    the directory then holds exactly the finished tree's files, byte for byte
+    the tables in HBM, every compaction having deleted its inputs' files, and
+    the tree holds the newest put of every key."""
+    print(f"stream with output files, {mode}", flush=True)
+    mems = list(served.write_stream(40000, 12, key_space=30000, entries_per_flush=2000))
+    opts = small_options(32 << 10)
+    eng = jobs.Engine(0)
+    if mode == "served":
+        eng.set_slots(3)
+    try:
+        be = served.EngineBackend(eng, opts, io_dir=str(tmp_path), io_sync=True)
+        lsm = served.Lsm(max_file_size=opts.max_file_size, level1_bytes=96 << 10)
+        lsm.on_delete = be.delete_file
+        run = served.run_served if mode == "served" else served.run_plain
+        lsm, st = run(be, mems, 0.0, lsm=lsm)
+        be.close()
+        lsm.check()
+        summ = st.summary()
+        assert summ["compactions"] >= 5
+        tree = {"%06d.ldb" % f.number: f for level in lsm.levels for f in level}
+        assert set(p.name for p in tmp_path.iterdir()) == set(tree)
+        for name, f in tree.items():
+            assert (tmp_path / name).read_bytes() == be.table_bytes(f), name
+        assert be.io["files"] == be.io["deleted"] + len(tree)
+        assert tree_contents(lsm, be.table_bytes) == expected_contents(mems)
+    finally:
+        eng.close()
