@@ -1552,8 +1552,8 @@ __device__ __forceinline__ void crc_copy_out(const uint4* stage4, u32 mis, u32 L
   }
   const u32 crc = crc_mask(s ^ 0xffffffffu);
   if (mis && lane < 16 - mis) gd[lane] = stage[mis + lane];
-  if (lane < e) gd[16 * F - mis + lane] = stage[16 * F + lane];
-  if (lane < 4) gd[L + lane] = (u8)(crc >> (8 * lane));
+  // the last e message bytes and the 4 CRC bytes after them in one store
+  if (lane < e + 4) gd[16 * F - mis + lane] = lane < e ? stage[16 * F + lane] : (u8)(crc >> (8 * (lane - e)));
 }
 
 // Copy n bytes from the LDS stage (src, which agrees with dst mod 16) to dst.
