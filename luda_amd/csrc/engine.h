@@ -394,7 +394,10 @@ void launch_cut_tables(KTimer& T, const u32* nbo_dev, u32 nbo_cap, EncParams ep,
                        bool size_cuts, DevStatus* st);
 // Per table: ob_file, ob_off, ob_isz, the windows (w_*), f_fbytes, f_fbase,
 // f_ibytes and the table's size in f_off.
-void launch_table_layout(KTimer& T, Kept k, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b, DevStatus* st);
+// each table's blocks, index entries and filter windows, and (look-back over
+// the tables) its offset in the output; the output's size into st->out_bytes
+void launch_table_layout(KTimer& T, ScanState& S, Kept k, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b,
+                         DevStatus* st);
 // groups: workgroups of the block writer, 0 = one per CU
 void launch_write_blocks(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u8* out,
                          u64 out_cap, DevStatus* st, u32 groups = 0);

@@ -562,6 +562,7 @@ void run_encode(luda_ctx* c, const EncodeSetup& s, const EncodeArrays& A, u8* ou
   ScanState& SC = c->scans;
   HIPCHK(scan_reserve(SC, std::max<u64>(std::max<u64>(s.m_cap, s.nbo_cap), std::max<u64>(s.nf_cap, s.nw_cap)) + 1,
                       c->s));
+  HIPCHK(scan_reserve_tiles(SC, (u64)s.nf_cap + 1, c->s));  // the table layout's look-back: one tile a table
   launch_entry_sizes(T, k, m_dev, s.m_cap, b, st);
   if (s.generic_cuts) {
     launch_bnext(T, k, m_dev, s.m_cap, s.ep, b, st);
@@ -571,8 +572,7 @@ void run_encode(luda_ctx* c, const EncodeSetup& s, const EncodeArrays& A, u8* ou
     run_block_cuts(T, k, m_dev, s.m_cap, s.ep, b, A.cut, s.nbo_cap, st);
   }
   launch_cut_tables(T, nbo_dev, s.nbo_cap, s.ep, b, s.nf_cap, s.nw_cap, s.size_cuts, st);
-  launch_table_layout(T, k, nf_dev, s.nf_cap, s.ep, b, st);
-  scan_u64(T, SC, b.f_off, b.f_off, nf_dev, 1, s.nf_cap + 1, &st->out_bytes, st);
+  launch_table_layout(T, SC, k, nf_dev, s.nf_cap, s.ep, b, st);
   launch_write_blocks(T, k, nbo_dev, s.nbo_cap, s.ep, b, out, s.out_cap, st, s.wb_groups);
   if (s.ep.bloom_bits) launch_write_filters(T, k, nw_dev, s.nw_cap, nf_dev, s.ep, b, out, s.out_cap, st);
   launch_write_tail(T, k, nf_dev, s.nf_cap, s.ep, b, out, s.out_cap, st);
@@ -1241,8 +1241,12 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
       HIPCHK(scan_reserve(c->scans,
                           std::max<u64>(std::max<u64>(n, es.m_cap), std::max<u64>(es.nbo_cap, std::max(es.nf_cap, es.nw_cap))) + 1,
                           c->s));
-      // the fused last merge round looks back over its merge tiles
-      HIPCHK(scan_reserve_tiles(c->scans, ((u64)n + merge_items_per_tile(W) - 1) / merge_items_per_tile(W) + 1, c->s));
+      // the fused last merge round looks back over its merge tiles, the
+      // table layout over the tables
+      HIPCHK(scan_reserve_tiles(c->scans,
+                                std::max<u64>(((u64)n + merge_items_per_tile(W) - 1) / merge_items_per_tile(W) + 1,
+                                              (u64)es.nf_cap + 1),
+                                c->s));
       if (resident) launch_gather_decoded(T, d_dsrc, nfiles, n, K, vptr, vlen);
       else if (!decode_first) decode();
       if (W <= 8) {

@@ -720,18 +720,23 @@ __global__ __launch_bounds__(kCtThreads) void k_cut_tables(const u32* nbo_dev, E
     st->n_windows = (u32)nwo;
     b.f_first[nfo] = (u32)nbo;
     b.f_win[nfo] = (u32)nwo;
-    b.f_off[nfo] = 0;  // the end item of the table offset scan
   }
 }
 
 constexpr u32 kTlThreads = 512;
 
 __global__ __launch_bounds__(kTlThreads) void k_table_layout(Kept k, const u32* nf_dev, EncParams ep, EncBufs b,
-                                                               DevStatus* st) {
+                                                               ScanArgs sa, DevStatus* st) {
   __shared__ u64 s_w[kTlThreads / 64];
-  if (st->err) return;
+  __shared__ u32 s_ticket;
+  if (st->err) return;  // nothing in this kernel sets it: every workgroup returns, or none
   const u64 nf = *nf_dev;
-  for (u64 f = blockIdx.x; f < nf; f += gridDim.x) {
+  const u64 f0 = lbscan::take_ticket(sa, &s_ticket);
+  if (nf == 0 && f0 == 0 && threadIdx.x == 0) {
+    b.f_off[0] = 0;
+    st->out_bytes = 0;
+  }
+  for (u64 f = f0; f < nf; f += gridDim.x) {
     const u32 b0 = b.f_first[f], b1 = b.f_first[f + 1];
     const u64 dend = b.f_dend[f];
     // block offsets in the table: the bytes of the blocks before, with their trailers
@@ -795,9 +800,20 @@ __global__ __launch_bounds__(kTlThreads) void k_table_layout(Kept k, const u32* 
         b.f_fbase[f] = dend;
       }
     }
-    if (threadIdx.x == 0) {
-      b.f_ibytes[f] = ib;
-      b.f_off[f] = dend + (ep.bloom_bits ? fb + 5 : 0) + meta_block_size(ep, dend, fb) + 5 + ib + 5 + 48;
+    if (threadIdx.x == 0) b.f_ibytes[f] = ib;
+    // the table's offset in the output: the sizes of the tables before it,
+    // by look-back over the tables (one a tile); the last one also gives
+    // the output's size
+    if (threadIdx.x < 64) {
+      const u64 size = dend + (ep.bloom_bits ? fb + 5 : 0) + meta_block_size(ep, dend, fb) + 5 + ib + 5 + 48;
+      const u64 ex = lbscan::publish<u64>(sa, f, size);
+      if (threadIdx.x == 0) {
+        b.f_off[f] = ex;
+        if (f + 1 == nf) {
+          b.f_off[nf] = ex + size;
+          st->out_bytes = ex + size;
+        }
+      }
     }
   }
 }
@@ -2090,8 +2106,17 @@ void launch_cut_tables(KTimer& T, const u32* nbo_dev, u32 nbo_cap, EncParams ep,
           size_cuts ? 1u : 0u, st);
   KLAUNCH(T, "cut_tables", k_cut_tables, 1, kCtThreads, 0, nbo_dev, ep, b, nf_cap, nw_cap, size_cuts ? 1u : 0u, st);
 }
-void launch_table_layout(KTimer& T, Kept k, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b, DevStatus* st) {
-  KLAUNCH(T, "table_layout", k_table_layout, grid_for(nf_cap, 1), kTlThreads, 0, k, nf_dev, ep, b, st);
+void launch_table_layout(KTimer& T, ScanState& S, Kept k, const u32* nf_dev, u32 nf_cap, EncParams ep, EncBufs b,
+                         DevStatus* st) {
+  if ((u64)nf_cap + 1 > S.tiles) throw_scan_capacity((u64)nf_cap + 1, S.tiles);
+  ScanArgs sa;
+  sa.words = S.words;
+  sa.ticket = S.tickets ? S.ticket : nullptr;
+  sa.epoch = scan_next_epoch(S, T.s);
+  sa.cap = 0;
+  sa.need = nullptr;
+  sa.st = st;
+  KLAUNCH(T, "table_layout", k_table_layout, grid_for(nf_cap, 1), kTlThreads, 0, k, nf_dev, ep, b, sa, st);
 }
 void launch_write_blocks(KTimer& T, Kept k, const u32* nbo_dev, u32 nbo_cap, EncParams ep, EncBufs b, u8* out,
                          u64 out_cap, DevStatus* st, u32 groups) {
