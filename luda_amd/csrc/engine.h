@@ -320,8 +320,14 @@ void launch_index(KTimer& T, const FileDesc* files, const FileInfo* info, const 
                   u32 nb_cap, BlockDesc* blks, u32* blk_nent, DevStatus* st);
 // blk_ebase = exclusive scan of blk_nent over st->n_blocks_in blocks,
 // st->n_entries, and file_ebase[f] = the first entry of table f (f <= nfiles)
-void scan_entries(KTimer& T, ScanState& S, const BlockDesc* blks, const u32* blk_nent, u32* blk_ebase, u32 nb_cap,
-                  u32 nfiles, u32* file_ebase, DevStatus* st);
+// launch_index that also scans the blocks' entry counts: each block's first
+// entry (blk_ebase), each table's (file_ebase, nfiles + 1 of them) and the
+// total (st->n_entries), by a look-back over tiles of 256 blocks; the look-
+// back words must hold index_scan_tiles(nb_cap)
+u64 index_scan_tiles(u32 nb_cap);
+void launch_index_entries(KTimer& T, ScanState& S, const FileDesc* files, const FileInfo* info, const u32* blk_base,
+                          u32 nfiles, u32 nb_cap, BlockDesc* blks, u32* blk_nent, u32* blk_ebase, u32* file_ebase,
+                          DevStatus* st);
 // groups: wavefronts of the decoder, 0 = the default
 void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 nb, const u32* blk_ebase, Planes K,
                    u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr, u32 max_ent, u32 groups = 0);

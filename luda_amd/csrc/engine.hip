@@ -1039,10 +1039,11 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     }
     DevStatus st0;
     memset(&st0, 0, sizeof(st0));
-    HIPCHK(scan_reserve(c->scans, std::max<u64>(nb_cap, nfiles + 1), c->s));
+    HIPCHK(scan_reserve_tiles(c->scans, index_scan_tiles((u32)nb_cap), c->s));
     files_in(c, d_files, fdesc.data(), nfiles, st0, d_info, d_bbase);
-    launch_index(T, d_files, d_info, d_bbase, nfiles, (u32)nb_cap, d_blks, d_nent, c->st);  // sets n_blocks_in
-    scan_entries(T, c->scans, d_blks, d_nent, d_ebase, (u32)nb_cap, nfiles, d_febase, c->st);
+    // sets n_blocks_in, each block's and table's first entry, n_entries
+    launch_index_entries(T, c->scans, d_files, d_info, d_bbase, nfiles, (u32)nb_cap, d_blks, d_nent, d_ebase, d_febase,
+                         c->st);
     HIPCHK(hipGetLastError());
     stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)}, StageSeg{d_febase, 1024, (nfiles + 1) * 4ull});
     stage_sync(c);
@@ -1503,11 +1504,11 @@ luda_entries* read_tables(luda_ctx* c, const luda_file* tables, u32 nfiles, u32 
   DevStatus st0;
   memset(&st0, 0, sizeof(st0));
   st0.bad_file = 0xffffffffu;
-  HIPCHK(scan_reserve(c->scans, std::max<u64>(nb_cap, nfiles + 1), c->s));
+  HIPCHK(scan_reserve_tiles(c->scans, index_scan_tiles((u32)nb_cap), c->s));
   files_in(c, d_files, fdesc.data(), nfiles, st0, d_info, d_bbase);
-  launch_index(T, d_files, d_info, d_bbase, nfiles, (u32)nb_cap, d_blks, d_nent, c->st);
+  launch_index_entries(T, c->scans, d_files, d_info, d_bbase, nfiles, (u32)nb_cap, d_blks, d_nent, d_ebase, d_febase,
+                       c->st);
   if (flags & LUDA_READ_VERIFY) launch_verify(T, d_files, nfiles, d_blks, (u32)nb_cap, c->st);
-  scan_entries(T, c->scans, d_blks, d_nent, d_ebase, (u32)nb_cap, nfiles, d_febase, c->st);
   stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)}, StageSeg{d_febase, 1024, (nfiles + 1) * 4ull});
   stage_sync(c);
   DevStatus stA;

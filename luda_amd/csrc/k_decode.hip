@@ -119,8 +119,11 @@ struct Win24 {
   u64 w0, w1, w2;
 };
 __device__ __forceinline__ Win24 load_win24(const GMEM u8* p) { return {gload_u64(p), gload_u64(p + 8), gload_u64(p + 16)}; }
+// (the word chosen by masks: a select between the fields can be turned into
+// an indexed load from a stack copy of the window)
 __device__ __forceinline__ u32 win_byte(const Win24& w, u32 i) {
-  const u64 x = i < 8 ? w.w0 : (i < 16 ? w.w1 : w.w2);
+  const u64 m0 = 0ull - (u64)(i < 8), m2 = 0ull - (u64)(i >= 16);
+  const u64 x = (w.w0 & m0) | (w.w1 & ~(m0 | m2)) | (w.w2 & m2);
   return (u32)(x >> (8 * (i & 7))) & 0xff;
 }
 // GetVarint32Ptr / GetVarint64Ptr (util/coding.cc:121-165) on bytes [i, avail)
@@ -200,8 +203,9 @@ constexpr u32 kIndexStage = 2048;
 
 __global__ __launch_bounds__(256) void k_index(const FileDesc* files, const FileInfo* info, const u32* blk_base,
                                                u32 nfiles, u32 nb_cap, BlockDesc* blks, u32* blk_nent,
-                                               DevStatus* st) {
+                                               u32* blk_ebase, u32* file_ebase, ScanArgs sa, DevStatus* st) {
   __shared__ u32 s_bb[kIndexStage];
+  __shared__ u32 s_wsum[4], s_ex, s_ticket;
   const u32 total = blk_base[nfiles];
   if (blockIdx.x == 0 && threadIdx.x == 0) st->n_blocks_in = total > nb_cap ? 0 : total;
   if (total > nb_cap) {  // more handles than the tables' sizes allow
@@ -214,129 +218,155 @@ __global__ __launch_bounds__(256) void k_index(const FileDesc* files, const File
     __syncthreads();
   }
   u32 maxr = 0, maxn = 0, maxg = 0, maxe = 0;
-  GRID_STRIDE(jj, total) {
-    const u32 j = (u32)jj;
-    blk_nent[j] = 0;
-    u32 f, fb;
-    if (staged) f = last_le_lds(s_bb, nfiles, j, &fb);
-    else f = last_le_global(blk_base, nfiles, j, &fb);
-    const u32 jf = j - fb;
-    BlockDesc bd = {0, 0, f};
-    blks[j] = bd;
-    const u8* base = files[f].base;
-    const u64 size = files[f].size;
-    const FileInfo fi = info[f];
-    const GMEM u8* ib = gbytes((u64)base + fi.ioff);
-    const u64 iroff = fi.isize - 4 - 4ull * fi.nr;
-    const u32 iw = gload_u32(ib + iroff + 4ull * jf);
-    const u32 ioff = iw >> 8;
-    if ((iw & 0xff) != 1 || ioff >= iroff) {
-      set_err(st, ERR_CORRUPT, SITE_INDEX);
-      continue;
-    }
-    const u64 iavail = iroff - ioff;  // bytes up to the restart array
-    const Win24 e = load_win24(ib + ioff);
-    const u32 ea = (u32)min<u64>(iavail, 24);
-    u32 ish = 0, ins = 0, ivl = 0, x = 0;
-    if ((x = win_varint32(e, 0, ea, &ish)) == ~0u || (x = win_varint32(e, x, ea, &ins)) == ~0u ||
-        (x = win_varint32(e, x, ea, &ivl)) == ~0u || ish != 0 || iavail - x < (u64)ins + ivl) {
-      set_err(st, ERR_CORRUPT, SITE_INDEX);
-      continue;
-    }
-    const Win24 hw = load_win24(ib + ioff + x + ins);
-    const u32 ha = min(ivl, 24u);
-    u64 boff = 0, bsz = 0;
-    u32 y = 0;
-    if ((y = win_varint64(hw, 0, ha, &boff)) == ~0u || (y = win_varint64(hw, y, ha, &bsz)) == ~0u ||
-        boff > size || bsz > size || boff + bsz + 5 > size || bsz > 0xffffffffull) {
-      set_err(st, ERR_CORRUPT, SITE_INDEX);
-      continue;
-    }
-    bd.addr = (u64)(base + boff);
-    bd.size = (u32)bsz;
-    blks[j] = bd;
+  // Without blk_ebase, thread g takes blocks g, g + S, ...  With it (the
+  // entry scan), workgroup w takes tiles w, w + G, ... of 256 blocks (by
+  // ticket on worker slots) and looks back for the entries before each
+  // (scan.h): a table's first block gives its first entry (file_ebase, the
+  // empty tables before it the same), the last block the total (n_entries,
+  // and file_ebase of the empty tables after it and of nfiles).
+  const bool scan = blk_ebase != nullptr;
+  const u64 ntile = ((u64)total + 255) / 256;
+  const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const u64 step = scan ? (u64)gridDim.x * 256 : (u64)gridDim.x * blockDim.x;
+  const u64 first = scan ? lbscan::take_ticket(sa, &s_ticket) * 256 + threadIdx.x
+                         : (u64)blockIdx.x * blockDim.x + threadIdx.x;
+  const u64 stop = scan ? (ntile ? ntile : 1) * 256 : (u64)total;
+  auto table_of = [&](u32 j) {
+    u32 fb;
+    return staged ? last_le_lds(s_bb, nfiles, j, &fb) : last_le_global(blk_base, nfiles, j, &fb);
+  };
+  for (u64 jl = first; jl < stop; jl += step) {
+    const u32 j = (u32)jl;
+    u32 f = 0, c = 0;
+    // one block; a failed check leaves it (`continue` ends the do-while)
+    if (jl < total) do {
+        blk_nent[j] = 0;
+        u32 fb;
+        if (staged) f = last_le_lds(s_bb, nfiles, j, &fb);
+        else f = last_le_global(blk_base, nfiles, j, &fb);
+        const u32 jf = j - fb;
+        BlockDesc bd = {0, 0, f};
+        blks[j] = bd;
+        const u8* base = files[f].base;
+        const u64 size = files[f].size;
+        const FileInfo fi = info[f];
+        const GMEM u8* ib = gbytes((u64)base + fi.ioff);
+        const u64 iroff = fi.isize - 4 - 4ull * fi.nr;
+        const u32 iw = gload_u32(ib + iroff + 4ull * jf);
+        const u32 ioff = iw >> 8;
+        if ((iw & 0xff) != 1 || ioff >= iroff) {
+          set_err(st, ERR_CORRUPT, SITE_INDEX);
+          continue;
+        }
+        const u64 iavail = iroff - ioff;  // bytes up to the restart array
+        const Win24 e = load_win24(ib + ioff);
+        const u32 ea = (u32)min<u64>(iavail, 24);
+        u32 ish = 0, ins = 0, ivl = 0, x = 0;
+        if ((x = win_varint32(e, 0, ea, &ish)) == ~0u || (x = win_varint32(e, x, ea, &ins)) == ~0u ||
+            (x = win_varint32(e, x, ea, &ivl)) == ~0u || ish != 0 || iavail - x < (u64)ins + ivl) {
+          set_err(st, ERR_CORRUPT, SITE_INDEX);
+          continue;
+        }
+        const Win24 hw = load_win24(ib + ioff + x + ins);
+        const u32 ha = min(ivl, 24u);
+        u64 boff = 0, bsz = 0;
+        u32 y = 0;
+        if ((y = win_varint64(hw, 0, ha, &boff)) == ~0u || (y = win_varint64(hw, y, ha, &bsz)) == ~0u ||
+            boff > size || bsz > size || boff + bsz + 5 > size || bsz > 0xffffffffull) {
+          set_err(st, ERR_CORRUPT, SITE_INDEX);
+          continue;
+        }
+        bd.addr = (u64)(base + boff);
+        bd.size = (u32)bsz;
+        blks[j] = bd;
 
-    const GMEM u8* p = gbytes(bd.addr);
-    const u32 n = (u32)bsz;
-    if (n < 4) {
-      set_err(st, p[n] == 1 ? ERR_UNSUPPORTED : ERR_CORRUPT, p[n] == 1 ? SITE_COMPRESSED : SITE_BLOCK_TRAILER);
-      continue;
+        const GMEM u8* p = gbytes(bd.addr);
+        const u32 n = (u32)bsz;
+        if (n < 4) {
+          set_err(st, p[n] == 1 ? ERR_UNSUPPORTED : ERR_CORRUPT, p[n] == 1 ? SITE_COMPRESSED : SITE_BLOCK_TRAILER);
+          continue;
+        }
+        const u32 nr = gload_u32(p + n - 4);
+        if (p[n] == 1) {
+          set_err(st, ERR_UNSUPPORTED, SITE_COMPRESSED);
+          continue;
+        }
+        if (nr > (n - 4) / 4) {
+          set_err(st, ERR_CORRUPT, SITE_BLOCK_TRAILER);
+          continue;
+        }
+        maxn = max(maxn, n);
+        maxg = max(maxg, nr);
+        if (nr == 0) continue;  // Block::NewIterator -> NewEmptyIterator
+        const u32 roff = n - 4 - 4 * nr;
+        u32 cnt_total = 0, prev = 0;
+        bool bad = false;
+        // restart words and the first bytes of their entries, two at a time
+        auto one = [&](u32 kk, u32 w, u64 h0, u64 h1) {
+          const u32 off = w >> 8, cnt = w & 0xff;
+          if (roff == 0) {  // no entries at all
+            if (cnt) bad = true;
+            return;
+          }
+          if ((kk == 0 && off != 0) || (kk > 0 && off <= prev) || off >= roff || cnt == 0) bad = true;
+          prev = off;
+          cnt_total += cnt;
+          // shared (0 at a restart) and key length: two varint32s, at most 10 bytes
+          const Win24 hwin = {h0, h1, 0};
+          const u32 avail = (u32)min<u64>(roff - off, 16);
+          u32 sh = 0, ns = 0, z = 0;
+          if (off < roff && (z = win_varint32(hwin, 0, avail, &sh)) != ~0u && win_varint32(hwin, z, avail, &ns) != ~0u &&
+              sh == 0 && ns >= 8)
+            maxr = max(maxr, ns - 8);
+        };
+        for (u32 k0 = 0; k0 < nr && !bad; k0 += 2) {
+          const bool two = k0 + 1 < nr;
+          const u32 wa = gload_u32(p + roff + 4 * k0);
+          const u32 wb = two ? gload_u32(p + roff + 4 * (k0 + 1)) : 0;
+          const bool ina = (wa >> 8) < roff, inb = two && (wb >> 8) < roff;
+          const u64 a0 = ina ? gload_u64(p + (wa >> 8)) : 0, a1 = ina ? gload_u64(p + (wa >> 8) + 8) : 0;
+          const u64 b0 = inb ? gload_u64(p + (wb >> 8)) : 0, b1 = inb ? gload_u64(p + (wb >> 8) + 8) : 0;
+          one(k0, wa, a0, a1);
+          if (two && !bad) one(k0 + 1, wb, b0, b1);
+        }
+        if (bad) {
+          set_err(st, ERR_CORRUPT, SITE_BLOCK_TRAILER);
+          continue;
+        }
+      blk_nent[j] = cnt_total;
+      maxe = max(maxe, cnt_total);
+      c = cnt_total;
+    } while (0);
+    if (!scan) continue;
+    const u64 t = jl / 256;
+    const u32 incl = wave_incl_sum(c);
+    if (lane == 63) s_wsum[wave] = incl;
+    __syncthreads();
+    u32 wpre = 0;
+    for (u32 w = 0; w < wave; w++) wpre += s_wsum[w];
+    const u32 agg = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+    if (wave == 0) {
+      const u32 ex = lbscan::publish<u32>(sa, t, agg);
+      if (lane == 0) s_ex = ex;
     }
-    const u32 nr = gload_u32(p + n - 4);
-    if (p[n] == 1) {
-      set_err(st, ERR_UNSUPPORTED, SITE_COMPRESSED);
-      continue;
+    __syncthreads();
+    const u32 eb = s_ex + wpre + incl - c;
+    if (jl < total) {
+      blk_ebase[j] = eb;
+      for (u32 g = j ? table_of(j - 1) + 1 : 0u; g <= f; g++) file_ebase[g] = eb;
     }
-    if (nr > (n - 4) / 4) {
-      set_err(st, ERR_CORRUPT, SITE_BLOCK_TRAILER);
-      continue;
+    if (jl + 1 == total || (total == 0 && jl == 0)) {
+      const u32 n = s_ex + agg;
+      for (u32 g = total ? table_of(total - 1) + 1 : 0u; g <= nfiles; g++) file_ebase[g] = n;
+      st->n_entries = n;
     }
-    maxn = max(maxn, n);
-    maxg = max(maxg, nr);
-    if (nr == 0) continue;  // Block::NewIterator -> NewEmptyIterator
-    const u32 roff = n - 4 - 4 * nr;
-    u32 cnt_total = 0, prev = 0;
-    bool bad = false;
-    // restart words and the first bytes of their entries, two at a time
-    auto one = [&](u32 kk, u32 w, u64 h0, u64 h1) {
-      const u32 off = w >> 8, cnt = w & 0xff;
-      if (roff == 0) {  // no entries at all
-        if (cnt) bad = true;
-        return;
-      }
-      if ((kk == 0 && off != 0) || (kk > 0 && off <= prev) || off >= roff || cnt == 0) bad = true;
-      prev = off;
-      cnt_total += cnt;
-      // shared (0 at a restart) and key length: two varint32s, at most 10 bytes
-      const Win24 hwin = {h0, h1, 0};
-      const u32 avail = (u32)min<u64>(roff - off, 16);
-      u32 sh = 0, ns = 0, z = 0;
-      if (off < roff && (z = win_varint32(hwin, 0, avail, &sh)) != ~0u && win_varint32(hwin, z, avail, &ns) != ~0u &&
-          sh == 0 && ns >= 8)
-        maxr = max(maxr, ns - 8);
-    };
-    for (u32 k0 = 0; k0 < nr && !bad; k0 += 2) {
-      const bool two = k0 + 1 < nr;
-      const u32 wa = gload_u32(p + roff + 4 * k0);
-      const u32 wb = two ? gload_u32(p + roff + 4 * (k0 + 1)) : 0;
-      const bool ina = (wa >> 8) < roff, inb = two && (wb >> 8) < roff;
-      const u64 a0 = ina ? gload_u64(p + (wa >> 8)) : 0, a1 = ina ? gload_u64(p + (wa >> 8) + 8) : 0;
-      const u64 b0 = inb ? gload_u64(p + (wb >> 8)) : 0, b1 = inb ? gload_u64(p + (wb >> 8) + 8) : 0;
-      one(k0, wa, a0, a1);
-      if (two && !bad) one(k0 + 1, wb, b0, b1);
-    }
-    if (bad) {
-      set_err(st, ERR_CORRUPT, SITE_BLOCK_TRAILER);
-      continue;
-    }
-    blk_nent[j] = cnt_total;
-    maxe = max(maxe, cnt_total);
+    __syncthreads();
   }
   if (maxr) atomicMax(&st->max_rulen, maxr);
   if (maxn) atomicMax(&st->max_blk, maxn);
   if (maxg) atomicMax(&st->max_nr, maxg);
   if (maxe) atomicMax(&st->max_ent, maxe);
 }
-
-// The entry scan's epilogue (over the blocks' entry counts): each table's
-// first entry.  A table's first block writes it, for the empty tables just
-// before it too; the last tile writes the empty tables at the end and the
-// entry count after the last table.
-struct FileEbaseEpi {
-  const BlockDesc* blks;
-  u32* file_ebase;
-  u32 nfiles;
-  struct Pre {
-    u32 f, fp;
-  };
-  __device__ Pre pre(u64 b, u32) const { return {blks[b].file, b ? blks[b - 1].file + 1 : 0u}; }
-  __device__ void item(u64, u32, u32 ebase, const Pre& p) const {
-    for (u32 g = p.fp; g <= p.f; g++) file_ebase[g] = ebase;
-  }
-  __device__ void done(u32 n_entries, u64 nb) const {
-    for (u32 g = nb ? blks[nb - 1].file + 1 : 0u; g <= nfiles; g++) file_ebase[g] = n_entries;
-  }
-};
 
 // Byte sources for decode_block: the block staged in LDS, or read in place
 // from global memory.  Each holds one kind of pointer only, so every read is
@@ -1026,13 +1056,24 @@ void launch_files_in(KTimer& T, const FileDesc* src, const DevStatus* init, File
 }
 void launch_index(KTimer& T, const FileDesc* files, const FileInfo* info, const u32* blk_base, u32 nfiles,
                   u32 nb_cap, BlockDesc* blks, u32* blk_nent, DevStatus* st) {
+  const ScanArgs none{nullptr, nullptr, 0, 0, nullptr, nullptr};
   KLAUNCH(T, "index", k_index, std::min<u32>(grid_for(nb_cap, 256), 1024), 256, 0, files, info, blk_base, nfiles, nb_cap, blks, blk_nent,
-          st);
+          (u32*)nullptr, (u32*)nullptr, none, st);
 }
-void scan_entries(KTimer& T, ScanState& S, const BlockDesc* blks, const u32* blk_nent, u32* blk_ebase, u32 nb_cap,
-                  u32 nfiles, u32* file_ebase, DevStatus* st) {
-  lbscan::run<u32, u32>(T, "scan_entries", S, lbscan::ArrLoad<u32>{blk_nent}, blk_ebase, &st->n_blocks_in, 0, nb_cap,
-                        &st->n_entries, st, nullptr, FileEbaseEpi{blks, file_ebase, nfiles});
+u64 index_scan_tiles(u32 nb_cap) { return ((u64)nb_cap + 255) / 256 + 1; }
+void launch_index_entries(KTimer& T, ScanState& S, const FileDesc* files, const FileInfo* info, const u32* blk_base,
+                          u32 nfiles, u32 nb_cap, BlockDesc* blks, u32* blk_nent, u32* blk_ebase, u32* file_ebase,
+                          DevStatus* st) {
+  if (index_scan_tiles(nb_cap) > S.tiles) throw_scan_capacity(index_scan_tiles(nb_cap), S.tiles);
+  ScanArgs sa;
+  sa.words = S.words;
+  sa.ticket = S.tickets ? S.ticket : nullptr;
+  sa.epoch = scan_next_epoch(S, T.s);
+  sa.cap = 0;
+  sa.need = nullptr;
+  sa.st = st;
+  KLAUNCH(T, "index", k_index, std::min<u32>(grid_for(nb_cap, 256), 1024), 256, 0, files, info, blk_base, nfiles,
+          nb_cap, blks, blk_nent, blk_ebase, file_ebase, sa, st);
 }
 void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 nb, const u32* blk_ebase, Planes K,
                    u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr, u32 max_ent, u32 groups) {
