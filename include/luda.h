@@ -150,6 +150,10 @@ typedef struct luda_stats {
   uint64_t data_block_bytes;  /* output data blocks with their trailers */
   uint64_t top_kernel_bytes;  /* algorithmic HBM bytes of top_kernel's launches in this call */
   double end_clock_ms;        /* CLOCK_MONOTONIC, in milliseconds, when the result was complete */
+  uint32_t decode_ahead;      /* 1 when the decoder launched behind the input scan, before the host had
+                                 read its counts (sized like the context's last jobs), fitted them and
+                                 was used; 0 when the decoder ran after the counts were known */
+  uint32_t reserved_stats;
 } luda_stats;
 
 typedef struct luda_result {

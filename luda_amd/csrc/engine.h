@@ -328,9 +328,22 @@ u64 index_scan_tiles(u32 nb_cap);
 void launch_index_entries(KTimer& T, ScanState& S, const FileDesc* files, const FileInfo* info, const u32* blk_base,
                           u32 nfiles, u32 nb_cap, BlockDesc* blks, u32* blk_nent, u32* blk_ebase, u32* file_ebase,
                           DevStatus* st);
+// A decoder launched behind phase A before the host has read its counts:
+// it decodes only when they fit the launch (no error, at most n_cap entries,
+// restart keys of at most max_rulen bytes, at most max_nr restart points a
+// block: the max_nr of the launch), and then over st->n_blocks_in blocks;
+// otherwise every workgroup returns before its first write.
+struct DecodeGuard {
+  u32 on;
+  u32 n_cap;
+  u32 max_rulen;
+};
 // groups: wavefronts of the decoder, 0 = the default
 void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 nb, const u32* blk_ebase, Planes K,
-                   u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr, u32 max_ent, u32 groups = 0);
+                   u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr, u32 max_ent, u32 groups = 0,
+                   DecodeGuard guard = DecodeGuard{0, 0, 0});
+// LDS of k_decode_entries for these maxima (over 64 KiB: it cannot run)
+u64 decode_lds_bytes(u32 max_blk, u32 max_nr);
 void launch_ingest(KTimer& T, u64 n, const u8* keys, const u64* key_off, const u8* vals, const u64* val_off, Kept out,
                    DevStatus* st);
 // entries 0 .. n-1 from resident decoded sources (src sorted by dst)

@@ -334,6 +334,15 @@ struct luda_ctx {
   PinnedBuf pin;     // staging for small host<->device copies
   PinnedBuf pin2;
   u32 kw_hint = 0;   // key words that fit the keys seen so far
+  // The last two jobs read through phase A: entries and data blocks per
+  // input byte, and the decoder's LDS maxima, from which the decoder is
+  // launched behind phase A before its counts are back (compact_files).
+  struct {
+    bool valid;
+    double ent_per_byte[2], blk_per_byte[2];
+    u32 max_blk, max_ent, max_nr[2];
+  } shape{};
+  bool decode_ahead = true;  // LUDA_DECODE_AHEAD=0 turns that off
   KTimer T;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool pin2_busy = false;  // a staging kernel that reads pin2 may not have run yet
@@ -406,6 +415,23 @@ void stream_sync(luda_ctx* c) {
   c->pin2_busy = false;
 }
 
+// Waits for a stage_out's completion flag only: kernels queued behind the
+// read-back keep the stream busy while the host reads it.  After 200 ms it
+// waits for the stream instead.
+void stage_wait_flag(luda_ctx* c) {
+  const double t0 = now_ms();
+  while (__atomic_load_n(c->done_h, __ATOMIC_ACQUIRE) < c->stage_seq) {
+    const double t = now_ms() - t0;
+    if (t > 200.0) {
+      stream_sync(c);
+      return;
+    }
+    if (t < 1.0) __builtin_ia32_pause();
+    else std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
+  c->pin2_busy = false;  // the kernels that read pin2 ran before the read-back
+}
+
 // Waits for the stream after a stage_out, which ends it.  A worker slot
 // first watches the stage_out's completion flag in host memory (no runtime
 // call, so the slots' threads do not contend for the runtime while they
@@ -460,13 +486,13 @@ void files_in(luda_ctx* c, FileDesc* d_files, const FileDesc* fdesc, u32 n, cons
 // Device bytes (up to three segments) into the pinned buffer pin, at the
 // segments' offsets, in one kernel; the caller waits with stage_sync before
 // reading pin.  pin must already hold every segment.
-void stage_out(luda_ctx* c, StageSeg a, StageSeg b = {}, StageSeg d = {}) {
+void stage_out(luda_ctx* c, StageSeg a, StageSeg b = {}, StageSeg d = {}, bool with_flag = false) {
   for (const StageSeg& g : {a, b, d})
     if (g.bytes && g.off + g.bytes > c->pin.cap) throw Fail{LUDA_IO_ERROR, "read-back staging too small"};
   // only worker slots watch the flag (stage_sync): a context alone on the
   // device finds the stream done sooner by polling the runtime, and skips
   // the flag's system-scope fences
-  const bool flag = c->family != c;
+  const bool flag = c->family != c || with_flag;
   launch_stage_out(c->T, c->pin.d, a, b, d,
                    StageDone{flag ? c->done_d : nullptr, c->done_ctr, flag ? ++c->stage_seq : 0});
 }
@@ -637,7 +663,10 @@ void validate_options(const luda_options& o) {
 
 // LUDA_KERNEL_TIMING=1 times every kernel with HIP events on the engine's
 // stream; LUDA_KERNEL_TIMING=name[,name...] times only those kernels.
+// Also LUDA_DECODE_AHEAD=0: no decoder launched behind phase A's read-back.
 void init_ctx_timer(luda_ctx* c) {
+  const char* da = getenv("LUDA_DECODE_AHEAD");
+  c->decode_ahead = !(da && strcmp(da, "0") == 0);
   const char* e = getenv("LUDA_KERNEL_TIMING");
   c->T.on = e && *e && *e != '0';
   c->T.only.clear();
@@ -999,6 +1028,14 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
   memset(&stA, 0, sizeof(stA));
   std::vector<u32> febase(nfiles + 1);
   KTimer& T = c->T;
+  // the decoder launched behind phase A (below): its arrays, in phase A's
+  // memory, and whether phase A's counts turned out to fit it
+  Planes gK{};
+  u64* g_vptr = nullptr;
+  u32* g_vlen = nullptr;
+  u64 n_guess = 0;
+  const u32 W_guess = std::max<u32>(1, c->kw_hint);
+  bool guessed = false;
   if (resident) {
     u64 e = 0;
     u32 wmax = 1;
@@ -1017,6 +1054,20 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     u64 nb_cap = 0;
     for (auto* f : files) nb_cap += f->size / 41 + 1;
     if (nb_cap > 0xffffffffull / 2) throw Fail{LUDA_NOT_SUPPORTED, "inputs too large"};
+    // The decoder goes behind phase A's read-back, shaped like the context's
+    // last two jobs (entries per input byte with a quarter more, their
+    // largest restart arrays, the key width seen so far), before the host
+    // has read phase A's counts: it decodes only when they fit that launch
+    // (k_decode_entries checks them first), and the host, which finds the
+    // same from the counts, then plans phase B while it runs.  Otherwise it
+    // is launched again once the counts are known, as without a guess.
+    const auto& sh = c->shape;
+    const u32 g_max_nr = std::max(sh.max_nr[0], sh.max_nr[1]);
+    bool guess = c->decode_ahead && sh.valid && W_guess <= 8 && decode_lds_bytes(sh.max_blk, g_max_nr) <= 65536;
+    if (guess) {
+      n_guess = (u64)((double)bytes_in * std::max(sh.ent_per_byte[0], sh.ent_per_byte[1]) * 1.25) + 1024;
+      guess = n_guess <= 0x7fffffffull && n_guess * (8ull * W_guess + 32) <= c->budget / 4;
+    }
     auto carve_a = [&](Arena& a, FileDesc*& d_files, FileInfo*& d_info, u32*& d_bbase, BlockDesc*& d_blks,
                        u32*& d_nent, u32*& d_ebase, u32*& d_febase) {
       d_files = a.take<FileDesc>(nfiles);
@@ -1026,6 +1077,15 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
       d_nent = a.take<u32>(nb_cap + 1);
       d_ebase = a.take<u32>(nb_cap + 1);
       d_febase = a.take<u32>(nfiles + 1);
+      if (guess) {
+        gK.P = a.take<u64>((u64)W_guess * n_guess);
+        gK.klen = a.take<u32>(n_guess);
+        gK.tag = a.take<u64>(n_guess);
+        gK.stride = n_guess;
+        gK.W = W_guess;
+        g_vptr = a.take<u64>(n_guess);
+        g_vlen = a.take<u32>(n_guess);
+      }
     };
     {
       Arena dry;
@@ -1045,13 +1105,37 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     launch_index_entries(T, c->scans, d_files, d_info, d_bbase, nfiles, (u32)nb_cap, d_blks, d_nent, d_ebase, d_febase,
                          c->st);
     HIPCHK(hipGetLastError());
-    stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)}, StageSeg{d_febase, 1024, (nfiles + 1) * 4ull});
-    stage_sync(c);
+    stage_out(c, StageSeg{c->st, 0, sizeof(DevStatus)}, StageSeg{d_febase, 1024, (nfiles + 1) * 4ull}, StageSeg{},
+              guess);
+    if (guess) {
+      const u64 nb_guess =
+          std::min<u64>(nb_cap, (u64)((double)bytes_in * std::max(sh.blk_per_byte[0], sh.blk_per_byte[1]) * 1.25) + 64);
+      launch_decode(T, d_files, d_blks, (u32)nb_guess, d_ebase, gK, g_vptr, g_vlen, c->st, sh.max_blk, g_max_nr,
+                    sh.max_ent, pick_tuning(c, bytes_in, 8 * W_guess).decode_groups,
+                    DecodeGuard{1, (u32)n_guess, 8 * W_guess});
+      HIPCHK(hipGetLastError());
+      stage_wait_flag(c);
+    } else {
+      stage_sync(c);
+    }
     memcpy(&stA, c->pin.p, sizeof(DevStatus));
     memcpy(febase.data(), c->pin.p + 1024, (nfiles + 1) * 4);
     if (status_code_of(stA) != LUDA_OK)
       throw Fail{status_code_of(stA), std::string("input rejected: ") + site_name(stA.site)};
     if (stA.n_blocks_in > nb_cap) throw Fail{LUDA_CORRUPTION, "more data blocks than the tables can hold"};
+    // the test k_decode_entries made
+    guessed = guess && stA.err == 0 && stA.site == 0 && stA.max_ulen == 0 && stA.n_entries <= n_guess &&
+              stA.max_rulen <= 8 * W_guess && stA.max_nr <= g_max_nr;
+    auto& shw = c->shape;
+    shw.ent_per_byte[1] = shw.ent_per_byte[0];
+    shw.blk_per_byte[1] = shw.blk_per_byte[0];
+    shw.max_nr[1] = shw.max_nr[0];
+    shw.ent_per_byte[0] = (double)stA.n_entries / (double)std::max<u64>(bytes_in, 1);
+    shw.blk_per_byte[0] = (double)stA.n_blocks_in / (double)std::max<u64>(bytes_in, 1);
+    shw.max_nr[0] = stA.max_nr;
+    shw.max_blk = stA.max_blk;
+    shw.max_ent = stA.max_ent;
+    shw.valid = true;
   }
   const u32 nb = stA.n_blocks_in;
   const u32 n = stA.n_entries;
@@ -1165,14 +1249,23 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     MergePair* d_pairs;
     u8* d_tab;
     EncodeArrays EA;
+    // the first attempt takes the decoded arrays of the decoder launched
+    // behind phase A when the counts fitted it (same key width)
+    const bool use_guess = guessed && attempt == 1 && W == W_guess;
     auto carve_b = [&](Arena& a) {
-      K.P = a.take<u64>((u64)W * n);
-      K.klen = a.take<u32>(n);
-      K.tag = a.take<u64>(n);
-      K.stride = n;
-      K.W = W;
-      vptr = a.take<u64>(n);
-      vlen = a.take<u32>(n);
+      if (use_guess) {
+        K = gK;
+        vptr = g_vptr;
+        vlen = g_vlen;
+      } else {
+        K.P = a.take<u64>((u64)W * n);
+        K.klen = a.take<u32>(n);
+        K.tag = a.take<u64>(n);
+        K.stride = n;
+        K.W = W;
+        vptr = a.take<u64>(n);
+        vlen = a.take<u32>(n);
+      }
       permA = a.take<u32>(n);
       permB = a.take<u32>(n);
       keep = a.take<u32>(n);
@@ -1222,16 +1315,18 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
       // status for this attempt: phase A counts, phase B zeroed.  After
       // phase A it is already so (phase A sets neither an error nor
       // max_ulen, or the job has ended), and the decoder, which needs only
-      // phase A's arrays, is launched first: it starts as soon as the host
-      // has seen phase A end, and the host tables are staged behind it.
+      // phase A's arrays, is launched first (or ran behind phase A's
+      // read-back): it starts as soon as the host has seen phase A end, and
+      // the host tables are staged behind it.
       DevStatus sb = stA;
       sb.err = 0;
       sb.site = 0;
       sb.max_ulen = 0;
       const bool decode_first = !resident && attempt == 1 && stA.err == 0 && stA.site == 0 && stA.max_ulen == 0;
       auto decode = [&] {
-        launch_decode(T, d_files, d_blks, nb, d_ebase, K, vptr, vlen, c->st, stA.max_blk, stA.max_nr, stA.max_ent,
-                      tune.decode_groups);
+        if (!use_guess)
+          launch_decode(T, d_files, d_blks, nb, d_ebase, K, vptr, vlen, c->st, stA.max_blk, stA.max_nr, stA.max_ent,
+                        tune.decode_groups);
       };
       if (decode_first) decode();
       // the fused last merge round's crossing slots start empty (filled
@@ -1350,6 +1445,7 @@ luda_result* compact(luda_ctx* c, const luda_job* job) {
     }
     c->kw_hint = std::max<u32>(c->kw_hint, (stB.max_ulen + 7) / 8);
     stats.attempts = attempt;
+    stats.decode_ahead = use_guess ? 1 : 0;
     stats.key_width = 8 * W;
     stats.entries_kept = stB.n_kept;
     stats.data_block_bytes = stB.data_bytes;

@@ -799,7 +799,14 @@ constexpr u32 kDescAhead = 64;
 template <u32 NWMAX>
 __global__ __launch_bounds__(64) void k_decode_entries(const BlockDesc* blks, u32 nb, const u32* blk_ebase, Planes K,
                                                        u64* vptr, u32* vlen, DevStatus* st, u32 stage, u32 max_nr,
-                                                       u32 ecap) {
+                                                       u32 ecap, DecodeGuard guard) {
+  if (guard.on) {
+    // phase A's fields, final before this launch; none that the decoder
+    // itself raises (max_ulen) is read
+    if (st->err || st->site || st->n_entries > guard.n_cap || st->max_rulen > guard.max_rulen || st->max_nr > max_nr)
+      return;
+    nb = st->n_blocks_in;
+  }
   extern __shared__ uint4 smem[];
   BlockDesc* dtab = reinterpret_cast<BlockDesc*>(smem + 2 * (stage / 16));
   u64* etab = reinterpret_cast<u64*>(dtab + kDescAhead);
@@ -1075,16 +1082,22 @@ void launch_index_entries(KTimer& T, ScanState& S, const FileDesc* files, const 
   KLAUNCH(T, "index", k_index, std::min<u32>(grid_for(nb_cap, 256), 1024), 256, 0, files, info, blk_base, nfiles,
           nb_cap, blks, blk_nent, blk_ebase, file_ebase, sa, st);
 }
+u64 decode_lds_bytes(u32 max_blk, u32 max_nr) {
+  const u64 stage = (std::min<u64>(max_blk, 16384) + 15 + 16 + 15) & ~15ull;
+  return 2 * stage + kDescAhead * (sizeof(BlockDesc) + 8) + 4ull * ((u64)max_nr + 1);
+}
 void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 nb, const u32* blk_ebase, Planes K,
-                   u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr, u32 max_ent, u32 groups) {
+                   u64* vptr, u32* vlen, DevStatus* st, u32 max_blk, u32 max_nr, u32 max_ent, u32 groups,
+                   DecodeGuard guard) {
   if (!nb) return;
+  if (guard.on && K.W > 8) launch_refused("decode (a guarded launch needs key planes of at most 8 words)", hipErrorInvalidValue);
   if (K.W <= 8) {
     // two stages of the largest block, its misalignment, and the 16 bytes an
     // 8-byte window may read past it; blocks over 16 KiB are read in place.
     // Then the descriptors of the next blocks, the restart groups' entry
     // prefix and, as far as 64 KiB allow, the entry offsets.
     const u32 stage = (u32)((std::min<u64>(max_blk, 16384) + 15 + 16 + 15) & ~15ull);
-    size_t lds = 2ull * stage + kDescAhead * (sizeof(BlockDesc) + 8) + 4ull * (max_nr + 1);
+    size_t lds = decode_lds_bytes(max_blk, max_nr);
     if (lds > 65536) launch_refused("decode (restart arrays too long for LDS)", hipErrorInvalidValue);
     const u32 ecap = (u32)std::min<u64>((u64)max_ent, (65536 - lds) / 2);
     lds += ((2ull * ecap + 3) & ~3ull);
@@ -1092,13 +1105,13 @@ void launch_decode(KTimer& T, const FileDesc* files, const BlockDesc* blks, u32 
     const u32 grid = std::max<u32>(1, std::min<u32>((nb + 3) / 4, groups ? groups : 4096));
     if (K.W <= 2)
       KLAUNCH(T, "decode", k_decode_entries<3>, grid, 64, lds, blks, nb, blk_ebase, K, vptr, vlen, st, stage, max_nr,
-              ecap);
+              ecap, guard);
     else if (K.W <= 4)
       KLAUNCH(T, "decode", k_decode_entries<5>, grid, 64, lds, blks, nb, blk_ebase, K, vptr, vlen, st, stage, max_nr,
-              ecap);
+              ecap, guard);
     else
       KLAUNCH(T, "decode", k_decode_entries<9>, grid, 64, lds, blks, nb, blk_ebase, K, vptr, vlen, st, stage, max_nr,
-              ecap);
+              ecap, guard);
     return;
   }
   const u32 grid = nb < 65536 ? nb : 65536;

@@ -116,6 +116,8 @@ class luda_stats(C.Structure):
         ("data_block_bytes", C.c_uint64),
         ("top_kernel_bytes", C.c_uint64),
         ("end_clock_ms", C.c_double),
+        ("decode_ahead", C.c_uint32),
+        ("reserved_stats", C.c_uint32),
     ]
 
 

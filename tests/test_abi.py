@@ -58,8 +58,9 @@ def test_queued_entry_points_reject_a_missing_context():
 
 
 def test_stats_layout():
-    # luda_stats ends with end_clock_ms (include/luda.h)
-    assert ffi.luda_stats.end_clock_ms.offset == C.sizeof(ffi.luda_stats) - 8
+    # luda_stats ends with end_clock_ms, decode_ahead and a reserved word (include/luda.h)
+    assert ffi.luda_stats.end_clock_ms.offset == C.sizeof(ffi.luda_stats) - 16
+    assert ffi.luda_stats.decode_ahead.offset == C.sizeof(ffi.luda_stats) - 8
 
 
 def _init_with_fault(kind):

@@ -205,6 +205,42 @@ def test_many_children_match_oracle(eng, nl0):
         assert out[1]["entries_in"] > 0
 
 
+def test_decoder_ahead_of_the_input_counts():
+    """The decoder is launched behind the input scan before the host has read
+    its counts, sized like the context's last jobs: used when the counts fit
+    (stats decode_ahead 1), launched again after them when they do not (a
+    first job, longer keys, a corrupt input, more entries than guessed).
+    Every job's tables equal the oracle's."""
+    e = jobs.Engine(0)
+    try:
+        def run(tj, label):
+            out = check_against_oracle(e, tj, label)
+            return out[1]["decode_ahead"]
+
+        print("decode ahead: repeated job", flush=True)
+        assert run(many_children_job(4, 3), "first") == 0
+        assert run(many_children_job(4, 3), "again") == 1
+        print("decode ahead: longer keys", flush=True)
+        assert run(wide_key_job(1, lo=40, hi=64, nl0=3), "longer keys") == 0
+        assert run(wide_key_job(1, lo=40, hi=64, nl0=3), "longer keys again") == 1
+        print("decode ahead: corrupt input", flush=True)
+        tj, (job, keep) = small_job(4)
+        t0 = bytearray(tj.tables[0][0])
+        bad = bytes(t0[:-1] + bytes([t0[-1] ^ 1]))
+        b = C.create_string_buffer(bad, len(bad))
+        keep.append(b)
+        job.inputs[0][0].data = C.cast(b, C.c_void_p)
+        job.inputs[0][0].size = len(bad)
+        rc, _, msg = e.compact(job)
+        assert rc == ffi.LUDA_CORRUPTION, (rc, msg)
+        run(many_children_job(4, 3), "after corrupt")
+        print("decode ahead: tiny jobs", flush=True)
+        seen = [run(TinyJob(seed), seed) for seed in range(700, 760)]
+        assert 0 in seen and 1 in seen, seen
+    finally:
+        e.close()
+
+
 @pytest.mark.parametrize("seed", range(4))
 def test_wide_keys_match_oracle(eng, seed):
     """User keys of 65-256 bytes: key planes of 9-32 words (the grouped
